@@ -1,0 +1,51 @@
+"""Line catalogues shipped with the package (radiobear_amd/data).
+
+The five .npz files are the reference's line lists, unchanged in content and
+keys (constituents/nh3/ammonia.npz, nh3/nh3.npz, h2s/h2s.npz, ph3/ph3jh.npz,
+ph3/PH3WGT.npz), so that a user's own catalogue in the reference schema drops
+in.  The 15 H2O lines of h2o_bk live in ``h2o_bk_lines`` (the reference types
+them in constituents/h2o/h2o_bk.py:23-49).
+"""
+import os
+
+import numpy as np
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data')
+CATALOGUES = ('ammonia', 'nh3', 'h2s', 'ph3jh', 'PH3WGT')
+
+
+def load(name):
+    """One catalogue as a dict of float64 arrays (pickle-free load)."""
+    with np.load(os.path.join(DATA_DIR, name + '.npz'), allow_pickle=False) as d:
+        return {k: np.asarray(d[k], dtype=np.float64) for k in d.files}
+
+
+def raw_catalog():
+    return {name: load(name) for name in CATALOGUES}
+
+
+def h2o_bk_lines():
+    """Karpowicz/Rosenkranz H2O lines in the h2o_bk schema, widths in GHz/bar."""
+    f_o = [22.2351, 183.3101, 321.2256, 325.1529, 380.1974, 439.1508, 443.0183,
+           448.0011, 470.8890, 474.6891, 488.4911, 556.9360, 620.7008, 752.0332, 916.1712]
+    I_o = [0.1314E-13, 0.2279E-11, 0.8058E-13, 0.2701E-11, 0.2444E-10, 0.2185E-11,
+           0.4637E-12, 0.2568E-10, 0.8392E-12, 0.3272E-11, 0.6676E-12, 0.1535E-08,
+           0.1711E-10, 0.1014E-08, 0.4238E-10]
+    E_o = [2.144, 0.668, 6.179, 1.541, 1.048, 3.595, 5.048, 1.405, 3.597, 2.379,
+           2.852, 0.159, 2.391, 0.396, 1.441]
+    w_s_mbar = [0.01349, 0.01466, 0.01057, 0.01381, 0.01454, 0.009715, 0.00788,
+                0.01275, 0.00983, 0.01095, 0.01313, 0.01405, 0.011836, 0.01253, 0.01275]
+    x_s = [0.61, 0.85, 0.54, 0.74, 0.89, 0.62, 0.50, 0.67, 0.65, 0.64, 0.72, 1.0,
+           0.68, 0.84, 0.78]
+    odd = {1: (2.4000, 0.71, 0.950, 0.490), 3: (2.395, 0.67, 0.900, 0.490),
+           4: (2.390, 0.63, 0.850, 0.540)}
+    w_h2, w_he, x_h2, x_he = [], [], [], []
+    for i in range(15):
+        a, b, c, d = odd.get(i, (2.395, 0.67, 0.900, 0.515))
+        w_h2.append(a)
+        w_he.append(b)
+        x_h2.append(c)
+        x_he.append(d)
+    out = dict(f_o=f_o, I_o=I_o, E_o=E_o, w_s=np.asarray(w_s_mbar) / 0.001, x_s=x_s,
+               w_h2=w_h2, w_he=w_he, x_h2=x_h2, x_he=x_he, SR=np.zeros(15))
+    return {k: np.asarray(v, dtype=np.float64) for k, v in out.items()}
