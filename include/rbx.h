@@ -1,0 +1,193 @@
+/*
+ * rbx.h -- C ABI of librbx.so: MI355X (gfx950) kernels for RadioBEAR's
+ * per-layer absorption and per-ray brightness-temperature integration.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: RBX_OK (0) or a negative code;
+ *     rbx_last_error() gives a thread-local message.  No C++ exception crosses
+ *     the ABI.
+ *   - Array arguments are DEVICE pointers (HBM), owned by the caller, dense
+ *     and C-ordered as documented; the only host pointers are the small
+ *     descriptor structs and the host columns given to rbx_catalog_set_lines.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream); all
+ *     work is enqueued asynchronously on it.
+ *   - The library allocates device memory only inside an rbx_catalog handle,
+ *     which it frees in rbx_catalog_destroy.
+ *
+ * Reference interfaces replaced (RadioBEAR 2.0.1, paths relative to the
+ * reference repository):
+ *   rbx_alpha           Alpha.get_layers -> get_single_layer -> get_alpha_from_calc
+ *                       -> <constituent>.alpha -> total_layer_alpha
+ *                       (radiobear/alpha.py:151-305; constituents/nh3/nh3_hs_sjs.py:6-26,
+ *                        nh3/nh3_hs.py:96-340, nh3/nh3_sjs.py:26-128, h2o/h2o_bk.py:65-189,
+ *                        h2s/h2s_ddb.py:42-87, ph3/ph3_jh.py:64-108, h2/h2_jj_ddb.py:7-38)
+ *   rbx_alpha_tangent   (no reference counterpart: d alpha / d{T, q_NH3, q_H2O} for the adjoint)
+ *   rbx_catalog_*       the line-list loads readInputFiles of the same modules
+ *                       (nh3_hs.py:88-93, nh3_sjs.py:17-23, h2s_ddb.py:14-39,
+ *                        ph3_jh.py:18-61, h2o_bk.py:17-62)
+ *   rbx_raypaths        raypath.compute_ds / findEdge / Shape._calcEllipse
+ *                       (radiobear/raypath.py:39-273, radiobear/shape.py:223-298)
+ *   rbx_integrate       Brightness.single's tau / W / Tb loop and final spectrum
+ *                       (radiobear/brightness.py:30-126)
+ *   rbx_jacobian        (no reference counterpart: dTb/dT and dTb/dq per layer)
+ */
+#ifndef RBX_H
+#define RBX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBX_ABI_VERSION 1
+
+/* status codes */
+#define RBX_OK          0
+#define RBX_ERR_ARG    -1   /* invalid argument (shape, null pointer, enum) */
+#define RBX_ERR_HIP    -2   /* HIP runtime error */
+#define RBX_ERR_NOMEM  -3   /* device allocation failed */
+#define RBX_ERR_STATE  -4   /* catalogue lacks a block the selected absorbers need */
+
+/* arithmetic modes */
+#define RBX_FP64        0   /* plain FP64 everywhere (IEEE division, FP64 exp) */
+#define RBX_FP32        1   /* FP32 line sums, compensated (FP64 outer accumulation);
+                               FP64 detuning, tau prefix sum and geometry; exp(-tau)
+                               from an FP64 range reduction and an FP32 exp2 */
+
+/* line-block kinds and their columns (float64, one value per line) */
+#define RBX_LINES_NH3_INV  0  /* fo, Io, Eo, gammaNH3o                     (ammonia.npz) */
+#define RBX_LINES_NH3_ROT  1  /* fo_rot, Io_rot, Eo_rot, gNH3_rot, gH2_rot, gHe_rot */
+#define RBX_LINES_NH3_V2   2  /* fo_v2, Io_v2, Eo_v2 */
+#define RBX_LINES_NH3_SJS  3  /* f0, I0, E, G0                             (nh3.npz) */
+#define RBX_LINES_H2S      4  /* f0, I0, E, GH2S                           (h2s.npz) */
+#define RBX_LINES_PH3      5  /* f0, I0, E, WgtI0, WgtFGB, WgtSB  (ph3jh.npz + PH3WGT.npz) */
+#define RBX_LINES_H2O      6  /* f_o, I_o, E_o, w_s, x_s, w_h2, x_h2, w_he, x_he, SR
+                                 (h2o_bk schema, widths in GHz/bar) */
+#define RBX_NUM_LINE_KINDS 7
+
+/* constituent slots, in the reference's sorted order (alpha.py:80) */
+#define RBX_C_H2   0
+#define RBX_C_H2O  1
+#define RBX_C_H2S  2
+#define RBX_C_NH3  3
+#define RBX_C_PH3  4
+#define RBX_NUM_CONSTITUENTS 5
+
+/* rows of the mixing-ratio table mix[RBX_NUM_SPECIES][nl] */
+#define RBX_X_H2   0
+#define RBX_X_HE   1
+#define RBX_X_CH4  2
+#define RBX_X_NH3  3
+#define RBX_X_H2O  4
+#define RBX_X_H2S  5
+#define RBX_X_PH3  6
+#define RBX_NUM_SPECIES 7
+
+typedef struct rbx_catalog rbx_catalog;
+
+/* Absorber selection: which formalism each constituent uses (0 = not loaded). */
+typedef struct rbx_absorbers {
+    int h2;    /* 1: h2_jj_ddb h2state 'e'; 2: h2_jj_ddb h2state 'n' */
+    int h2o;   /* 1: h2o_bk */
+    int h2s;   /* 1: h2s_ddb */
+    int nh3;   /* 1: nh3_hs_sjs; 2: nh3_hs; 3: nh3_sjs */
+    int ph3;   /* 1: ph3_jh */
+} rbx_absorbers;
+
+/* Planet shape and view for the ray tracer. */
+typedef struct rbx_planet {
+    double req;             /* km, equatorial radius at p_ref (config.Req) */
+    double rpol;            /* km, polar radius at p_ref (config.Rpol) */
+    double orientation[2];  /* deg: [position angle, sub-observer planetographic latitude] */
+    int gtype;              /* 0: 'ellipse'; 1: 'sphere' / 'circle' */
+    int limb;               /* 0: 'shape'; 1: 'sec' (raypath.py:218-219) */
+} rbx_planet;
+
+int rbx_abi_version(void);
+
+/* Copy the calling thread's last error message (NUL-terminated) into buf. */
+int rbx_last_error(char *buf, size_t len);
+
+/* Number of HIP devices visible to the library (host call, no GPU work). */
+int rbx_device_count(int *count);
+
+/* ---- line catalogue (device-resident, owned by the library) ---- */
+int rbx_catalog_create(rbx_catalog **out);
+/* Replace block `kind` with n lines; host_columns[i] is a HOST array of n
+ * doubles for column i of that kind (see RBX_LINES_*).  n = 0 clears it. */
+int rbx_catalog_set_lines(rbx_catalog *cat, int kind, int n, const double *const *host_columns);
+int rbx_catalog_lines(const rbx_catalog *cat, int kind, int *n);
+int rbx_catalog_destroy(rbx_catalog *cat);
+
+/* ---- absorption: alpha[nl][nf] in cm^-1 ----
+ * P, T            [nl]   bar, K (device)
+ * mix             [RBX_NUM_SPECIES][nl] mixing ratios (device)
+ * freq            [nf]   GHz (device), the request order
+ * freq_nh3        [nf]   GHz (device) or NULL: the frequency each output column
+ *                 uses for NH3 (nh3_hs returns its <= 30 GHz half first,
+ *                 nh3_hs.py:96-114); its first nf_nh3_low entries must be <= 30 GHz
+ *                 and the rest > 30 GHz.  NULL means freq (which must then be
+ *                 ordered that way, e.g. ascending) with nf_nh3_low computed by the caller.
+ * scale           [RBX_NUM_CONSTITUENTS][nl] per-constituent per-layer scale
+ *                 (total_layer_alpha, alpha.py:151-192) or NULL for 1.0
+ * alpha           [nl][nf] float64 (RBX_FP64) or float32 (RBX_FP32), layer-major
+ * by_constituent  [RBX_NUM_CONSTITUENTS][nl][nf] float64 unscaled per-constituent
+ *                 alpha, or NULL (zero rows for constituents that are not loaded)
+ */
+int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
+              const double *P, const double *T, const double *mix, int nl,
+              const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
+              const double *scale, int precision, void *alpha, double *by_constituent,
+              void *stream);
+
+/* ---- absorption tangents for the adjoint ----
+ * dalpha          [3][nl][nf] float64: d alpha / d T, d alpha / d q_NH3,
+ *                 d alpha / d q_H2O (q = mixing ratio), same layout as alpha.
+ */
+int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers,
+                      const double *P, const double *T, const double *mix, int nl,
+                      const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
+                      const double *scale, double *dalpha, void *stream);
+
+/* ---- ray geometry (FP64) ----
+ * R, N            [nl] layer radii (km) and refractive indices, top layer first
+ * b               [nray][2] impact parameters (fraction of R[0])
+ * ds              [nray][ld] step lengths; out_units 0: km (Ray.ds),
+ *                 1: half step in cm ((ds*1e3/1e-2)/2, what rbx_integrate consumes)
+ * nsteps          [nray] entries of ds used; -1 = off planet (Ray.ds None)
+ * ld >= nl - 1.
+ */
+int rbx_raypaths(const rbx_planet *planet, const double *R, const double *N, int nl,
+                 const double *b, int nray, int out_units, double *ds, int ld, int *nsteps,
+                 void *stream);
+
+/* ---- brightness integration ----
+ * alpha           [nl][nf] float64 (alpha_precision RBX_FP64) or float32 (RBX_FP32)
+ * T               [nl] K
+ * hds, ld, nsteps from rbx_raypaths with out_units 1
+ * mode            0: point ray, W = alpha e^-tau; 1: disc average, W = 2 alpha E2(tau)
+ * precision       RBX_FP64 or RBX_FP32 (how e^-tau is evaluated; tau is FP64 in both)
+ * tau_cut         stop a ray's frequency group once every tau > tau_cut (<= 0: never)
+ * Tb              [nray][nf] K; intW [nray][nf] integrated weight or NULL
+ */
+int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                  const double *hds, int ld, const int *nsteps, int nray,
+                  int mode, int precision, double tau_cut, double *Tb, double *intW,
+                  void *stream);
+
+/* ---- Jacobian rows ----
+ * dalpha          [3][nl][nf] from rbx_alpha_tangent
+ * J               [nray][2][nl][nf] float32: dTb/dT[l] and dTb/dq_NH3[l] of row (ray, f);
+ *                 layers a ray does not reach (or past tau_cut) are 0
+ * Tb              [nray][nf] float64 or NULL
+ */
+int rbx_jacobian(const void *alpha, int alpha_precision, const double *dalpha, const double *T,
+                 int nl, int nf, const double *hds, int ld, const int *nsteps, int nray,
+                 double tau_cut, float *J, double *Tb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBX_H */

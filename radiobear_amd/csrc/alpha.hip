@@ -1,0 +1,917 @@
+// Per-layer absorption alpha[layer][freq] (cm^-1) and its tangents: line sums on VALU.
+//
+// Every line shape of the Jupiter absorbers is written in one canonical form
+// in x = f^2 (GHz^2):
+//
+//     term(x) = (a x + c) / ((x - D)^2 + g x + G),   alpha_c(f) = x * (sum term + K)
+//
+//   Ben-Reuven (nh3_hs inversion, h2s_ddb, ph3_jh), nh3_hs.py:198-255,
+//     h2s_ddb.py:76-79, ph3_jh.py:80-87:
+//       D = (f0+delta)^2 + gamma^2 - zeta^2, a = W (gamma-zeta), c = W (gamma+zeta) D,
+//       g = 4 gamma^2, G = 0, W = line weight / f0^2 ((f/f0)^2 of the shape).
+//   Gross (nh3_hs rotational and v2), nh3_hs.py:285-291, 322-328:
+//       D = f0^2, a = 0, c = W gamma, g = 4 gamma^2, G = 0.
+//   Van Vleck-Weisskopf with base (h2o_bk), h2o_bk.py:174-189, after putting the
+//   two resonances over one denominator:
+//       D = s^2 - df^2, G = 4 s^2 df^2, g = 0, a = 2 df W, c = a (s^2 + df^2),
+//       and the per-line base subtraction -2 W base x becomes the layer term K.
+//   Collision-induced H2 (h2_jj_ddb.py:7-38) and the H2O continuum
+//   (h2o_bk.py:102-119) are alpha = K x.
+//   nh3_sjs (nh3_sjs.py:93-122) has frequency-dependent widths, so its 200
+//   lines are summed with per-(line, frequency) coefficients; it is only used
+//   below 400 bar (nh3_hs_sjs.py:14-25).
+//
+// Grid: blockIdx.y = layer, blockIdx.x = frequency tile (tiles never straddle
+// the 30 GHz switch of nh3_hs).  Each 256-thread block stages 256 lines at a
+// time in LDS: thread t derives line t's canonical coefficients for this
+// layer (FP64, with exp/pow), then every thread sums all staged lines for its
+// K frequencies, reading the coefficients as LDS broadcasts.
+//
+// FP64 mode sums in FP64 with IEEE division.  FP32 mode: the detuning x - D
+// is formed in FP64 (cancellation near line centre) and rounded, the rest of
+// the term is FP32 with the hardware reciprocal, partial sums of 16 lines are
+// FP32 and are accumulated in FP64 (compensated, blocked summation).  Line
+// weights are scaled per layer and block by a power of two so FP32 never sees
+// the 1e-39..1e-19 raw intensities.
+//
+// The tangent kernel runs the same coefficient code on dual numbers
+// (value, d/dT, d/dq_NH3, d/dq_H2O) and differentiates the canonical term in
+// the line sum: d term = (dN - term dQ) / Q.
+#include <type_traits>
+
+#include "catalog.hpp"
+#include "dual.hpp"
+
+namespace rbx {
+
+constexpr int kThreads = 256;
+constexpr int kSub = 16;   // FP32 partial-sum length
+
+// physical constants as the plug-ins spell them
+constexpr double kHsHcKb = 19.858252418E-24 / 1.38E-23;    // nh3_hs.py:75-76
+constexpr double kHsCoef = 1.0E6 * 6.02297E23 / 8.31432E7; // nh3_hs.py:81
+constexpr double kGHzToInvCm = 1.0 / 29.9792458;           // nh3_hs.py:70
+constexpr double kHck = 1.438396;                          // h2s_ddb.py:8
+constexpr double kGeisa = 7.244E+21;                       // h2s_ddb.py:6
+constexpr double kGHz = 29.9792458;                        // h2s_ddb.py:9
+constexpr double kDbPerInvCm = 434294.5;
+
+// nh3_hs inversion coefficient sets, <= 30 GHz and > 30 GHz (nh3_hs.py:165-194):
+// gnu_H2, gnu_He, gnu_NH3, GAMMA_H2, GAMMA_He, GAMMA_NH3,
+// zeta_H2, zeta_He, zeta_NH3, Z_H2, Z_He, Z_NH3, d, Con
+__constant__ double kHsInv[2][14] = {
+    {1.640, 0.75, 0.852, 0.7756, 0.666, 1.0, 1.262, 0.3, 0.5296, 0.7964, 0.667, 1.554, -0.0498, 0.9301},
+    {1.7465, 0.9779, 0.7298, 0.8202, 1.0, 1.0, 1.2163, 0.0291, 0.5152, 0.8873, 0.8994, 2.0 / 3.0,
+     -0.0627, 0.9862}};
+
+// one layer: T and partial pressures (bar); S = double or Dual
+template <class S>
+struct LayerT {
+    double P;
+    S T, h2, he, ch4, nh3, h2o, h2s, ph3;
+};
+
+template <class S>
+struct CoefT {
+    S D, a, c, g, G;
+};
+
+struct AlphaArgs {
+    DevCatalog cat;
+    rbx_absorbers abs;
+    const double *P, *T, *mix;
+    int nl;
+    const double *freq, *freq_nh3;
+    int nf, nf_low;
+    const double *scale;
+    void *alpha;       // forward: alpha [nl][nf]; tangent: dalpha [3][nl][nf] (double)
+    double *by_c;
+    int tiles_low;
+    int tile_width;
+};
+
+// ---- per-layer coefficient sets: each derives (D, a, c, g, G) of one line ----
+template <class S>
+struct InvPrep {   // nh3_hs inversion lines (nh3_hs.py:198-255)
+    S Ag, Bg, Az, Bz, eT, pref;
+    double d;
+    __device__ InvPrep(const LayerT<S> &L, int hi) {
+        const double *k = kHsInv[hi];
+        S Tdiv = 300.0 / L.T, t295 = 295.0 / L.T;
+        Ag = (k[0] * L.h2) * pow(Tdiv, k[3]) + (k[1] * L.he) * pow(Tdiv, k[4]);
+        Bg = k[2] * L.nh3 * pow(t295, k[5]);
+        Az = (k[6] * L.h2) * pow(Tdiv, k[9]) + (k[7] * L.he) * pow(Tdiv, k[10]);
+        Bz = k[8] * L.nh3 * pow(t295, k[11]);
+        d = k[12];
+        eT = -(1.0 / L.T - 1.0 / 300.0) * kHsHcKb;
+        pref = k[13] * kHsCoef * (L.nh3 / 300.0) * pow(300.0 / L.T, 3.5) * ((2.0 / kPi) * (1.0 / kGHzToInvCm));
+    }
+    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+        const double fo = B.col[0][i], Io = B.col[1][i], Eo = B.col[2][i], go = B.col[3][i];
+        S gam = Ag + Bg * go;
+        S zet = Az + Bz * go;
+        S fd = fo + d * gam;
+        S W = pref * Io * exp(eT * Eo) / (fo * fo);
+        S Dd = fd * fd + gam * gam - zet * zet;
+        return {Dd, W * (gam - zet), W * (gam + zet) * Dd, 4.0 * gam * gam, S(0.0)};
+    }
+};
+
+template <class S>
+struct RotPrep {   // nh3_hs rotational lines, Gross (nh3_hs.py:257-291)
+    S c1, c2, c3, eT, pref;
+    __device__ explicit RotPrep(const LayerT<S> &L) {
+        S Tdiv = 300.0 / L.T;
+        c1 = 0.2984 * L.h2 * pow(Tdiv, 0.8730);
+        c2 = 0.75 * L.he * pow(Tdiv, 2.0 / 3.0);
+        c3 = 3.1789 * L.nh3 * Tdiv;
+        eT = (1.0 / 300.0 - 1.0 / L.T) * kHsHcKb;
+        pref = 2.4268 * kHsCoef * (L.nh3 / 300.0) * pow(300.0 / L.T, 3.5) * ((1.0 / kGHzToInvCm) * (4.0 / kPi));
+    }
+    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+        const double fo = B.col[0][i];
+        S gam = c1 * B.col[4][i] + c2 * B.col[5][i] + c3 * B.col[3][i];
+        S W = pref * B.col[1][i] * exp(eT * B.col[2][i]);
+        return {S(fo * fo), S(0.0), W * gam, 4.0 * gam * gam, S(0.0)};
+    }
+};
+
+template <class S>
+struct V2Prep {    // nh3_hs v2 roto-vibrational lines, Gross (nh3_hs.py:293-328)
+    S gam, eT, pref;
+    __device__ explicit V2Prep(const LayerT<S> &L) {
+        S Tdiv = 300.0 / L.T;
+        gam = L.h2 * 1.4 * pow(Tdiv, 0.73) + L.he * 0.68 * pow(Tdiv, 0.5716) + L.nh3 * 9.5 * Tdiv;
+        eT = (1.0 / 300.0 - 1.0 / L.T) * kHsHcKb;
+        pref = 1.1206 * kHsCoef * (L.nh3 / 300.0) * pow(300.0 / L.T, 3.5) * ((1.0 / kGHzToInvCm) * (4.0 / kPi));
+    }
+    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+        const double fo = B.col[0][i];
+        S W = pref * B.col[1][i] * exp(eT * B.col[2][i]);
+        return {S(fo * fo), S(0.0), W * gam, 4.0 * gam * gam, S(0.0)};
+    }
+};
+
+template <class S>
+struct H2sPrep {   // h2s_ddb, Ben-Reuven with zeta = gamma (h2s_ddb.py:42-87)
+    S Ag, Bg, del, eT, pref;
+    __device__ explicit H2sPrep(const LayerT<S> &L) {
+        const double T0 = 296.0;
+        S tg = pow(T0 / L.T, 0.7);
+        Ag = tg * (1.960 * L.h2 + 1.200 * L.he);
+        Bg = tg * L.h2s;
+        del = 1.28 * L.h2s;
+        eT = -((1.0 / L.T) - (1.0 / T0)) * kHck;
+        pref = kGeisa * (L.h2s / T0) * pow(T0 / L.T, 3.5) * (kGHz * 2.0 / kPi);
+    }
+    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+        const double f0 = B.col[0][i];
+        S gam = Ag + Bg * B.col[3][i];
+        S W = pref * B.col[1][i] * exp(eT * B.col[2][i]) / (f0 * f0);
+        S fd = f0 + del;
+        S D = fd * fd;
+        return {D, S(0.0), W * 2.0 * gam * D, 4.0 * gam * gam, S(0.0)};
+    }
+};
+
+template <class S>
+struct Ph3Prep {   // ph3_jh, Ben-Reuven with zeta = delta = 0 (ph3_jh.py:64-108)
+    S A, Bs, eT, pref;
+    __device__ explicit Ph3Prep(const LayerT<S> &L) {
+        const double T0 = 300.0;
+        A = pow(T0 / L.T, 2.0 / 3.0) * (3.2930 * L.h2 + 1.6803 * L.he);
+        Bs = (T0 / L.T) * 4.2157 * L.ph3;
+        eT = -((1.0 / L.T) - (1.0 / T0)) * kHck;
+        pref = kGeisa * (L.ph3 / T0) * pow(T0 / L.T, 3.5) * (kGHz * 2.0 / kPi);
+    }
+    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+        const double f0 = B.col[0][i];
+        S gam = A * B.col[4][i] + Bs * B.col[5][i];
+        S W = pref * (B.col[1][i] * B.col[3][i]) * exp(eT * B.col[2][i]) / (f0 * f0);
+        S D = f0 * f0 + gam * gam;
+        return {D, W * gam, W * gam * D, 4.0 * gam * gam, S(0.0)};
+    }
+};
+
+template <class S>
+struct H2oPrep {   // h2o_bk lines, VVW with base (h2o_bk.py:83-99, 130-189)
+    S lnTh, Th25, Th, h2o, h2, he, pref;
+    __device__ explicit H2oPrep(const LayerT<S> &L) {
+        const double M = 8.314472 / 0.46151805;
+        Th = 300.0 / L.T;
+        lnTh = log(Th);
+        Th25 = pow(Th, 2.5);
+        S density = (M * L.h2o) / (8.314472e-5 * L.T);
+        density = 0.997317 * (density / M) * 6.0221415e23 * (1.0 / 1e6);
+        pref = density * (4.342945 * 1e-4 / kDbPerInvCm / kPi);
+        h2o = L.h2o;
+        h2 = L.h2;
+        he = L.he;
+    }
+    // also adds the line's base contribution (-2 W base) to `base`
+    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &base) const {
+        const double fo = B.col[0][i];
+        S Sl = B.col[1][i] * Th25 * exp(B.col[2][i] * (1.0 - Th));
+        S df = B.col[3][i] * h2o * exp(B.col[4][i] * lnTh) + B.col[5][i] * h2 * exp(B.col[6][i] * lnTh) +
+               B.col[7][i] * he * exp(B.col[8][i] * lnTh);
+        const double s = fo + B.col[9][i];
+        S W = pref * Sl / (fo * fo);
+        S a = 2.0 * df * W;
+        const double s2 = s * s;
+        S df2 = df * df;
+        base = base + (-2.0) * W * (df / (562500.0 + df2));
+        return {s2 - df2, a, a * (s2 + df2), S(0.0), 4.0 * s2 * df2};
+    }
+};
+
+// ---- LDS staging ----
+struct Staged64 {
+    double D[kThreads], a[kThreads], c[kThreads], g[kThreads], G[kThreads];
+};
+struct Staged32 {
+    double D[kThreads];
+    float4 q[kThreads];  // a, c, g, G
+};
+struct StagedTan {       // value and two tangent slots (d/dT, d/dq_own) of D, a, c, g, G
+    double v[5][kThreads];
+    double d[2][5][kThreads];
+};
+struct StagedSjs {
+    double f0[kThreads], w[kThreads], g0[kThreads];
+    double dw[kNd][kThreads];
+};
+
+union SharedFwd {
+    Staged64 s64;
+    Staged32 s32;
+    StagedSjs sjs;
+    double red[kThreads];
+};
+
+union SharedTan {
+    StagedTan st;
+    StagedSjs sjs;
+    double red[kNd + 1][kThreads];
+};
+
+// ---- forward line sum over one block for K frequencies ----
+template <int K, bool F32, class Prep>
+__device__ void line_sum(const DevBlock &B, const Prep &prep, double scale, const double (&x)[K],
+                         double (&acc)[K], SharedFwd &sh, double &base) {
+    float xf[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) xf[k] = static_cast<float>(x[k]);
+    for (int l0 = 0; l0 < B.n; l0 += kThreads) {
+        const int i = l0 + threadIdx.x;
+        if (i < B.n) {
+            const CoefT<double> q = prep(B, i, base);
+            if constexpr (F32) {
+                sh.s32.D[threadIdx.x] = q.D;
+                sh.s32.q[threadIdx.x] = make_float4(static_cast<float>(q.a * scale), static_cast<float>(q.c * scale),
+                                                    static_cast<float>(q.g), static_cast<float>(q.G));
+            } else {
+                sh.s64.D[threadIdx.x] = q.D;
+                sh.s64.a[threadIdx.x] = q.a;
+                sh.s64.c[threadIdx.x] = q.c;
+                sh.s64.g[threadIdx.x] = q.g;
+                sh.s64.G[threadIdx.x] = q.G;
+            }
+        }
+        __syncthreads();
+        const int cnt = min(kThreads, B.n - l0);
+        if constexpr (F32) {
+            for (int j0 = 0; j0 < cnt; j0 += kSub) {
+                const int j1 = min(j0 + kSub, cnt);
+                float p[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) p[k] = 0.0f;
+                for (int j = j0; j < j1; ++j) {
+                    const double D = sh.s32.D[j];
+                    const float4 q = sh.s32.q[j];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        float d = static_cast<float>(x[k] - D);
+                        float num = fmaf(q.x, xf[k], q.y);
+                        float den = fmaf(d, d, fmaf(q.z, xf[k], q.w));
+                        p[k] = fmaf(num, __builtin_amdgcn_rcpf(den), p[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] += static_cast<double>(p[k]);
+            }
+        } else {
+            for (int j = 0; j < cnt; ++j) {
+                const double D = sh.s64.D[j], a = sh.s64.a[j], c = sh.s64.c[j];
+                const double g = sh.s64.g[j], G = sh.s64.G[j];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    double d = x[k] - D;
+                    double num = fma(a, x[k], c);
+                    double den = fma(d, d, fma(g, x[k], G));
+                    acc[k] += num / den;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- tangent line sum: value and the two tangent slots (T, q_own) ----
+template <int K, class Prep>
+__device__ void line_sum_tan(const DevBlock &B, const Prep &prep, int qslot, const double (&x)[K],
+                             double (&acc)[K], double (&dacc)[2][K], SharedTan &sh, Dual &base) {
+    for (int l0 = 0; l0 < B.n; l0 += kThreads) {
+        const int i = l0 + threadIdx.x;
+        if (i < B.n) {
+            const CoefT<Dual> q = prep(B, i, base);
+            const Dual *c5[5] = {&q.D, &q.a, &q.c, &q.g, &q.G};
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                sh.st.v[m][threadIdx.x] = c5[m]->v;
+                sh.st.d[0][m][threadIdx.x] = c5[m]->d[0];
+                sh.st.d[1][m][threadIdx.x] = c5[m]->d[qslot];
+            }
+        }
+        __syncthreads();
+        const int cnt = min(kThreads, B.n - l0);
+        for (int j = 0; j < cnt; ++j) {
+            const double D = sh.st.v[0][j], a = sh.st.v[1][j], c = sh.st.v[2][j];
+            const double g = sh.st.v[3][j], G = sh.st.v[4][j];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double d = x[k] - D;
+                const double num = fma(a, x[k], c);
+                const double den = fma(d, d, fma(g, x[k], G));
+                const double r = 1.0 / den;
+                const double t = num * r;
+                acc[k] += t;
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const double dN = fma(sh.st.d[m][1][j], x[k], sh.st.d[m][2][j]);
+                    const double dQ = fma(-2.0 * d, sh.st.d[m][0][j], fma(sh.st.d[m][3][j], x[k], sh.st.d[m][4][j]));
+                    dacc[m][k] += (dN - t * dQ) * r;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// power of two s with |w s| ~ 1 (FP32 range for the line weights)
+__device__ inline double pow2_scale(double w) {
+    if (!(w > 0.0) || !isfinite(w)) return 1.0;
+    return ldexp(1.0, -ilogb(w));
+}
+
+__device__ inline double block_reduce_sum(double v, double *red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ inline Dual block_reduce_sum(const Dual &v, SharedTan &sh) {
+    Dual r;
+    r.v = block_reduce_sum(v.v, sh.red[0]);
+    for (int m = 0; m < kNd; ++m) r.d[m] = block_reduce_sum(v.d[m], sh.red[m + 1]);
+    return r;
+}
+
+// nh3_sjs Spilker coefficient set (nh3_sjs.py:52-77): GH2, GHe, GNH3, ZH2, ZHe, ZNH3, D
+template <class S>
+__device__ inline void sjs_spilker(const S &T, const S &Ph2, S *c) {
+    const double EPS = 1E-12;
+    S rexp = 8.79 * exp(-T / 83.0);
+    S GH2a = exp(9.024 - T / 20.3) - 0.9918 + Ph2;
+    GH2a = pow(GH2a, rexp);  // NaN for a negative base, as numpy
+    if (val(GH2a) < EPS) {
+        c[0] = 1.690; c[1] = 0.750; c[2] = 0.60; c[3] = 1.35; c[4] = 0.30; c[5] = 0.20; c[6] = -0.45;
+    } else {
+        GH2a = 2.122 * exp(-T / 116.8) / GH2a;
+        GH2a = 2.34 * (1.0 - GH2a);
+        c[0] = GH2a;
+        c[1] = 0.46 + T / 3000.0;
+        c[2] = 0.74;
+        c[3] = 5.7465 - 7.7644 * GH2a + 9.1931 * GH2a * GH2a - 5.6816 * GH2a * GH2a * GH2a +
+               1.2307 * GH2a * GH2a * GH2a * GH2a;
+        c[4] = 0.28 - T / 1750.0;
+        c[5] = 0.50;
+        c[6] = -0.45;
+    }
+}
+
+// nh3_sjs for this thread's K frequencies (FP64; S = double or Dual)
+template <int K, class S, class Sh>
+__device__ void nh3_sjs(const DevBlock &B, const LayerT<S> &L, const double (&f)[K], S (&out)[K], Sh &sh) {
+    const double T0 = 296.0;
+    const double J[7] = {1.690, 0.750, 0.6, 1.350, 0.300, 0.200, -0.45};
+    S Sp[7];
+    sjs_spilker(L.T, L.h2, Sp);
+    const S tf = pow(T0 / L.T, 2.0 / 3.0);
+    const S eT = -((1.0 / L.T) - (1.0 / T0)) * kHck;
+    S Ag[K], Bg[K], Az[K], Bz[K], del[K], sum[K];
+    double x[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        S c[7];
+        if (f[k] <= 26.0) {
+            for (int m = 0; m < 7; ++m) c[m] = Sp[m];
+        } else if (f[k] >= 34.0) {
+            for (int m = 0; m < 7; ++m) c[m] = J[m];
+        } else {
+            const double flfh = (26.0 - 34.0) / (f[k] - 26.0);
+            for (int m = 0; m < 7; ++m) c[m] = Sp[m] + (Sp[m] - J[m]) / flfh;
+        }
+        Ag[k] = tf * (c[0] * L.h2 + c[1] * L.he);
+        Bg[k] = tf * c[2] * L.nh3;
+        Az[k] = tf * (c[3] * L.h2 + c[4] * L.he);
+        Bz[k] = tf * c[5] * L.nh3;
+        del[k] = c[6] * L.nh3;
+        x[k] = f[k] * f[k];
+        sum[k] = 0.0;
+    }
+    for (int l0 = 0; l0 < B.n; l0 += kThreads) {
+        const int i = l0 + threadIdx.x;
+        if (i < B.n) {
+            const double f0 = B.col[0][i];
+            const S w = B.col[1][i] * exp(eT * B.col[2][i]) / (f0 * f0);
+            sh.sjs.f0[threadIdx.x] = f0;
+            sh.sjs.w[threadIdx.x] = val(w);
+            if constexpr (std::is_same<S, Dual>::value) {
+                for (int m = 0; m < kNd; ++m) sh.sjs.dw[m][threadIdx.x] = w.d[m];
+            }
+            sh.sjs.g0[threadIdx.x] = B.col[3][i];
+        }
+        __syncthreads();
+        const int cnt = min(kThreads, B.n - l0);
+        for (int j = 0; j < cnt; ++j) {
+            const double f0 = sh.sjs.f0[j], g0 = sh.sjs.g0[j];
+            S w;
+            if constexpr (std::is_same<S, Dual>::value)
+                w = Dual(sh.sjs.w[j], sh.sjs.dw[0][j], sh.sjs.dw[1][j], sh.sjs.dw[2][j]);
+            else
+                w = sh.sjs.w[j];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                S gam = Ag[k] + Bg[k] * g0;
+                S zet = Az[k] + Bz[k] * g0;
+                S fd = f0 + del[k];
+                S Dd = fd * fd + gam * gam - zet * zet;
+                S num = (gam - zet) * x[k] + (gam + zet) * Dd;
+                S d = x[k] - Dd;
+                S den = d * d + 4.0 * x[k] * gam * gam;
+                sum[k] = sum[k] + w * num / den;
+            }
+        }
+        __syncthreads();
+    }
+    const S pref = kGeisa * (L.nh3 / T0) * pow(T0 / L.T, 3.5) * (kGHz * 2.0 / kPi) * (1.0 + L.P / 1.0E5);
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = pref * x[k] * sum[k];
+}
+
+// h2_jj_ddb layer constant: alpha_h2 = K x (h2_jj_ddb.py:7-38)
+template <class S>
+__device__ inline S h2_cia(const LayerT<S> &L, int state) {
+    const S th = 273.0 / L.T;
+    S pre;
+    if (state == 1) {
+        pre = fmin1(pow(L.T / 55.0, 2.7));
+        pre = fmin1(pre * pow(L.T / 120.0, 0.55));
+    } else {
+        pre = fmin1(pow(L.T / 40.0, 2.5));
+    }
+    return 3.9522E-14 * L.h2 * pre *
+           (L.h2 * pow(th, 3.12) + 1.382 * L.he * pow(th, 2.24) + 9.322 * L.ch4 * pow(th, 3.34));
+}
+
+// h2o_bk continuum: alpha = K x (h2o_bk.py:102-125)
+template <class S>
+__device__ inline S h2o_continuum(const LayerT<S> &L) {
+    const S Th = 300.0 / L.T;
+    const double Cf_he = ((1.0 / 0.001) * (1.0 / 0.001)) * 1.03562010226e-10;
+    const double Cf_h2 = ((1.0 / 0.001) * (1.0 / 0.001)) * 5.07722009423e-11;
+    const S Cs1 = 3.1e-07 * pow(Th, 12.0);
+    const S Th3 = Th * Th * Th;
+    const S w = L.h2o / 0.001;
+    return (4.342945 * (Cf_he * L.he * L.h2o * Th3 + Cf_h2 * L.h2 * L.h2o * Th3) + 4.342945 * Cs1 * (w * w)) /
+           kDbPerInvCm;
+}
+
+struct Tile {
+    int layer, f_begin, f_end;
+    bool hi;
+};
+
+__device__ inline Tile tile_of(const AlphaArgs &A) {
+    Tile t;
+    t.layer = blockIdx.y;
+    t.hi = static_cast<int>(blockIdx.x) >= A.tiles_low;
+    t.f_begin = t.hi ? A.nf_low + (blockIdx.x - A.tiles_low) * A.tile_width : blockIdx.x * A.tile_width;
+    t.f_end = t.hi ? A.nf : A.nf_low;
+    return t;
+}
+
+__device__ inline LayerT<double> layer_of(const AlphaArgs &A, int layer) {
+    LayerT<double> L;
+    L.P = A.P[layer];
+    L.T = A.T[layer];
+    const double *mx = A.mix;
+    const int nl = A.nl;
+    L.h2 = L.P * mx[RBX_X_H2 * nl + layer];
+    L.he = L.P * mx[RBX_X_HE * nl + layer];
+    L.ch4 = L.P * mx[RBX_X_CH4 * nl + layer];
+    L.nh3 = L.P * mx[RBX_X_NH3 * nl + layer];
+    L.h2o = L.P * mx[RBX_X_H2O * nl + layer];
+    L.h2s = L.P * mx[RBX_X_H2S * nl + layer];
+    L.ph3 = L.P * mx[RBX_X_PH3 * nl + layer];
+    return L;
+}
+
+// nh3_hs_sjs pressure blend (nh3_hs_sjs.py:6-26); sel 1 blend, 2 hs only, 3 sjs only
+template <class S>
+__device__ inline S nh3_select(int sel, double P, const S &hs, const S &sj) {
+    if (sel == 2) return hs;
+    if (sel == 3) return sj;
+    if (P < 400.0) return hs;
+    if (P > 2000.0) return sj;
+    const double W = (P - 400.0) / (2000.0 - 400.0);
+    return W * sj + (1.0 - W) * hs;
+}
+
+template <int K, bool F32>
+__global__ __launch_bounds__(kThreads) void alpha_kernel(AlphaArgs A) {
+    __shared__ SharedFwd sh;
+    const Tile tl = tile_of(A);
+    const int layer = tl.layer;
+    const LayerT<double> L = layer_of(A, layer);
+
+    int col[K];
+    bool ok[K];
+    double f[K], x[K], xn[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        col[k] = tl.f_begin + threadIdx.x + k * kThreads;
+        ok[k] = col[k] < tl.f_end;
+        f[k] = ok[k] ? A.freq[col[k]] : 1.0;
+        const double fn = ok[k] ? (A.freq_nh3 ? A.freq_nh3[col[k]] : f[k]) : 1.0;
+        x[k] = f[k] * f[k];
+        xn[k] = fn * fn;
+    }
+
+    double res[RBX_NUM_CONSTITUENTS][K];
+#pragma unroll
+    for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[c][k] = 0.0;
+    double unused = 0.0;
+
+    if (A.abs.h2) {
+        const double Kh = h2_cia(L, A.abs.h2);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_H2][k] = Kh * x[k];
+    }
+    if (A.abs.h2o) {
+        const DevBlock &B = A.cat.blk[RBX_LINES_H2O];
+        H2oPrep<double> prep(L);
+        const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+        double acc[K] = {};
+        double base = 0.0;
+        line_sum<K, F32>(B, prep, s, x, acc, sh, base);
+        base = block_reduce_sum(base, sh.red);
+        const double cont = h2o_continuum(L);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_H2O][k] = x[k] * (acc[k] / s + base + cont);
+    }
+    if (A.abs.h2s) {
+        const DevBlock &B = A.cat.blk[RBX_LINES_H2S];
+        H2sPrep<double> prep(L);
+        const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+        double acc[K] = {};
+        line_sum<K, F32>(B, prep, s, x, acc, sh, unused);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_H2S][k] = x[k] * (acc[k] / s);
+    }
+    if (A.abs.nh3) {
+        const int sel = A.abs.nh3;
+        const bool use_hs = sel == 2 || (sel == 1 && L.P <= 2000.0);
+        const bool use_sjs = sel == 3 || (sel == 1 && L.P >= 400.0);
+        double hs[K] = {}, sj[K] = {};
+        if (use_hs) {
+            double acc[K] = {};
+            {
+                const DevBlock &B = A.cat.blk[RBX_LINES_NH3_INV];
+                InvPrep<double> prep(L, tl.hi ? 1 : 0);
+                const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+                double a2[K] = {};
+                line_sum<K, F32>(B, prep, s, xn, a2, sh, unused);
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s;
+            }
+            {
+                const DevBlock &B = A.cat.blk[RBX_LINES_NH3_ROT];
+                RotPrep<double> prep(L);
+                const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+                double a2[K] = {};
+                line_sum<K, F32>(B, prep, s, xn, a2, sh, unused);
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s;
+            }
+            {
+                const DevBlock &B = A.cat.blk[RBX_LINES_NH3_V2];
+                V2Prep<double> prep(L);
+                const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+                double a2[K] = {};
+                line_sum<K, F32>(B, prep, s, xn, a2, sh, unused);
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s;
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                hs[k] = xn[k] * acc[k];
+                if (hs[k] < 0.0) hs[k] = 1.0E-8;   // nh3_hs.py:337-338
+            }
+        }
+        if (use_sjs) nh3_sjs<K>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_NH3][k] = nh3_select(sel, L.P, hs[k], sj[k]);
+    }
+    if (A.abs.ph3) {
+        const DevBlock &B = A.cat.blk[RBX_LINES_PH3];
+        Ph3Prep<double> prep(L);
+        const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+        double acc[K] = {};
+        line_sum<K, F32>(B, prep, s, x, acc, sh, unused);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_PH3][k] = x[k] * (acc[k] / s);
+    }
+
+    // total_layer_alpha (alpha.py:151-192): scale-sum in sorted constituent order
+    const int nl = A.nl;
+    double sc[RBX_NUM_CONSTITUENTS];
+#pragma unroll
+    for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c) sc[c] = A.scale ? A.scale[c * nl + layer] : 1.0;
+    const int en[RBX_NUM_CONSTITUENTS] = {A.abs.h2, A.abs.h2o, A.abs.h2s, A.abs.nh3, A.abs.ph3};
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!ok[k]) continue;
+        double tot = 0.0;
+#pragma unroll
+        for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
+            if (en[c]) tot += res[c][k] * sc[c];
+        const size_t o = static_cast<size_t>(layer) * A.nf + col[k];
+        if (F32)
+            static_cast<float *>(A.alpha)[o] = static_cast<float>(tot);
+        else
+            static_cast<double *>(A.alpha)[o] = tot;
+        if (A.by_c) {
+#pragma unroll
+            for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
+                A.by_c[(static_cast<size_t>(c) * nl + layer) * A.nf + col[k]] = res[c][k];
+        }
+    }
+}
+
+// ---- tangent kernel: dalpha[3][nl][nf] = d alpha / d(T, q_NH3, q_H2O) ----
+template <int K>
+__global__ __launch_bounds__(kThreads) void alpha_tangent_kernel(AlphaArgs A) {
+    __shared__ SharedTan sh;
+    const Tile tl = tile_of(A);
+    const int layer = tl.layer;
+    const LayerT<double> L0 = layer_of(A, layer);
+    const int nl = A.nl;
+    LayerT<Dual> L;
+    L.P = L0.P;
+    L.T = Dual(L0.T, 1.0, 0.0, 0.0);
+    L.h2 = L0.h2;
+    L.he = L0.he;
+    L.ch4 = L0.ch4;
+    L.nh3 = Dual(L0.nh3, 0.0, L0.P, 0.0);   // d(P q)/dq = P
+    L.h2o = Dual(L0.h2o, 0.0, 0.0, L0.P);
+    L.h2s = L0.h2s;
+    L.ph3 = L0.ph3;
+
+    int col[K];
+    bool ok[K];
+    double f[K], x[K], xn[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        col[k] = tl.f_begin + threadIdx.x + k * kThreads;
+        ok[k] = col[k] < tl.f_end;
+        f[k] = ok[k] ? A.freq[col[k]] : 1.0;
+        const double fn = ok[k] ? (A.freq_nh3 ? A.freq_nh3[col[k]] : f[k]) : 1.0;
+        x[k] = f[k] * f[k];
+        xn[k] = fn * fn;
+    }
+    Dual res[RBX_NUM_CONSTITUENTS][K];
+    Dual unused;
+
+    // a line block's value + (T, q_own) tangents, times x
+    auto finish = [&](const double (&acc)[K], const double (&dacc)[2][K], int qslot, const double (&xx)[K],
+                      Dual (&out)[K]) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            Dual r(xx[k] * acc[k]);
+            r.d[0] = xx[k] * dacc[0][k];
+            r.d[qslot] = xx[k] * dacc[1][k];
+            out[k] = out[k] + r;
+        }
+    };
+
+    if (A.abs.h2) {
+        const Dual Kh = h2_cia(L, A.abs.h2);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_H2][k] = Kh * x[k];
+    }
+    if (A.abs.h2o) {
+        const DevBlock &B = A.cat.blk[RBX_LINES_H2O];
+        H2oPrep<Dual> prep(L);
+        double acc[K] = {}, dacc[2][K] = {};
+        Dual base;
+        line_sum_tan<K>(B, prep, 2, x, acc, dacc, sh, base);
+        base = block_reduce_sum(base, sh);
+        const Dual Kc = base + h2o_continuum(L);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_H2O][k] = Kc * x[k];
+        finish(acc, dacc, 2, x, res[RBX_C_H2O]);
+    }
+    if (A.abs.h2s) {
+        const DevBlock &B = A.cat.blk[RBX_LINES_H2S];
+        H2sPrep<Dual> prep(L);
+        double acc[K] = {}, dacc[2][K] = {};
+        line_sum_tan<K>(B, prep, 1, x, acc, dacc, sh, unused);
+        finish(acc, dacc, 1, x, res[RBX_C_H2S]);
+    }
+    if (A.abs.nh3) {
+        const int sel = A.abs.nh3;
+        const bool use_hs = sel == 2 || (sel == 1 && L.P <= 2000.0);
+        const bool use_sjs = sel == 3 || (sel == 1 && L.P >= 400.0);
+        Dual hs[K], sj[K];
+        if (use_hs) {
+            {
+                double acc[K] = {}, dacc[2][K] = {};
+                InvPrep<Dual> prep(L, tl.hi ? 1 : 0);
+                line_sum_tan<K>(A.cat.blk[RBX_LINES_NH3_INV], prep, 1, xn, acc, dacc, sh, unused);
+                finish(acc, dacc, 1, xn, hs);
+            }
+            {
+                double acc[K] = {}, dacc[2][K] = {};
+                RotPrep<Dual> prep(L);
+                line_sum_tan<K>(A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, xn, acc, dacc, sh, unused);
+                finish(acc, dacc, 1, xn, hs);
+            }
+            {
+                double acc[K] = {}, dacc[2][K] = {};
+                V2Prep<Dual> prep(L);
+                line_sum_tan<K>(A.cat.blk[RBX_LINES_NH3_V2], prep, 1, xn, acc, dacc, sh, unused);
+                finish(acc, dacc, 1, xn, hs);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (hs[k].v < 0.0) hs[k] = Dual(1.0E-8);   // nh3_hs.py:337-338
+        }
+        if (use_sjs) nh3_sjs<K>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh);
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[RBX_C_NH3][k] = nh3_select(sel, L.P, hs[k], sj[k]);
+    }
+    if (A.abs.ph3) {
+        const DevBlock &B = A.cat.blk[RBX_LINES_PH3];
+        Ph3Prep<Dual> prep(L);
+        double acc[K] = {}, dacc[2][K] = {};
+        line_sum_tan<K>(B, prep, 1, x, acc, dacc, sh, unused);
+        // ph3 does not depend on q_NH3: its slot-1 tangent is identically zero
+        finish(acc, dacc, 1, x, res[RBX_C_PH3]);
+    }
+
+    double sc[RBX_NUM_CONSTITUENTS];
+#pragma unroll
+    for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c) sc[c] = A.scale ? A.scale[c * nl + layer] : 1.0;
+    const int en[RBX_NUM_CONSTITUENTS] = {A.abs.h2, A.abs.h2o, A.abs.h2s, A.abs.nh3, A.abs.ph3};
+    double *out = static_cast<double *>(A.alpha);
+    const size_t plane = static_cast<size_t>(nl) * A.nf;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!ok[k]) continue;
+        Dual tot;
+#pragma unroll
+        for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
+            if (en[c]) tot = tot + res[c][k] * sc[c];
+        const size_t o = static_cast<size_t>(layer) * A.nf + col[k];
+#pragma unroll
+        for (int m = 0; m < kNd; ++m) out[m * plane + o] = tot.d[m];
+    }
+}
+
+// check that the catalogue holds what the selected absorbers read
+static int check_catalog(const rbx_catalog *cat, const rbx_absorbers &ab, const char *who) {
+    RBX_CHECK_ARG(ab.h2 >= 0 && ab.h2 <= 2, "%s: bad h2 selector %d", who, ab.h2);
+    RBX_CHECK_ARG(ab.h2o >= 0 && ab.h2o <= 1, "%s: bad h2o selector %d", who, ab.h2o);
+    RBX_CHECK_ARG(ab.h2s >= 0 && ab.h2s <= 1, "%s: bad h2s selector %d", who, ab.h2s);
+    RBX_CHECK_ARG(ab.nh3 >= 0 && ab.nh3 <= 3, "%s: bad nh3 selector %d", who, ab.nh3);
+    RBX_CHECK_ARG(ab.ph3 >= 0 && ab.ph3 <= 1, "%s: bad ph3 selector %d", who, ab.ph3);
+    struct Need {
+        bool on;
+        int kind;
+        const char *what;
+    } needs[] = {{ab.h2o != 0, RBX_LINES_H2O, "h2o_bk"},
+                 {ab.h2s != 0, RBX_LINES_H2S, "h2s_ddb"},
+                 {ab.ph3 != 0, RBX_LINES_PH3, "ph3_jh"},
+                 {ab.nh3 == 1 || ab.nh3 == 2, RBX_LINES_NH3_INV, "nh3_hs"},
+                 {ab.nh3 == 1 || ab.nh3 == 3, RBX_LINES_NH3_SJS, "nh3_sjs"}};
+    for (const Need &n : needs) {
+        if (n.on && cat->dev.blk[n.kind].n <= 0) {
+            set_error("%s: absorber %s selected but its line block %d is empty", who, n.what, n.kind);
+            return RBX_ERR_STATE;
+        }
+    }
+    return RBX_OK;
+}
+
+static int fill_args(AlphaArgs &A, const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,
+                     const double *T, const double *mix, int nl, const double *freq, const double *freq_nh3, int nf,
+                     int nf_nh3_low, const double *scale, void *out, double *by_c, const char *who) {
+    RBX_CHECK_ARG(cat && absorbers, "%s: null catalogue or absorber selection", who);
+    RBX_CHECK_ARG(P && T && mix && freq && out, "%s: null device pointer", who);
+    RBX_CHECK_ARG(nl > 0 && nf > 0, "%s: nl=%d nf=%d must be positive", who, nl, nf);
+    RBX_CHECK_ARG(nl <= 65535, "%s: nl=%d exceeds 65535", who, nl);
+    RBX_CHECK_ARG(nf_nh3_low >= 0 && nf_nh3_low <= nf, "%s: nf_nh3_low=%d outside [0, %d]", who, nf_nh3_low, nf);
+    int rc = check_catalog(cat, *absorbers, who);
+    if (rc) return rc;
+    A.cat = cat->dev;
+    A.abs = *absorbers;
+    A.P = P;
+    A.T = T;
+    A.mix = mix;
+    A.nl = nl;
+    A.freq = freq;
+    A.freq_nh3 = freq_nh3;
+    A.nf = nf;
+    A.nf_low = nf_nh3_low;
+    A.scale = scale;
+    A.alpha = out;
+    A.by_c = by_c;
+    return RBX_OK;
+}
+
+static void set_tiles(AlphaArgs &A, int K, dim3 &grid) {
+    A.tile_width = K * kThreads;
+    A.tiles_low = (A.nf_low + A.tile_width - 1) / A.tile_width;
+    const int tiles_hi = (A.nf - A.nf_low + A.tile_width - 1) / A.tile_width;
+    grid = dim3(A.tiles_low + tiles_hi, A.nl);
+}
+
+int alpha_tile_k(int nf) {
+    if (nf >= 4 * kThreads * 2) return 4;
+    if (nf >= 2 * kThreads * 2) return 2;
+    return 1;
+}
+
+template <int K>
+static int launch_alpha(AlphaArgs &A, int precision, hipStream_t st) {
+    dim3 grid;
+    set_tiles(A, K, grid);
+    if (precision == RBX_FP32)
+        hipLaunchKernelGGL((alpha_kernel<K, true>), grid, dim3(kThreads), 0, st, A);
+    else
+        hipLaunchKernelGGL((alpha_kernel<K, false>), grid, dim3(kThreads), 0, st, A);
+    return launch_status("alpha_kernel");
+}
+
+}  // namespace rbx
+
+extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P, const double *T,
+                         const double *mix, int nl, const double *freq, const double *freq_nh3, int nf,
+                         int nf_nh3_low, const double *scale, int precision, void *alpha, double *by_constituent,
+                         void *stream) {
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha: bad precision %d", precision);
+    rbx::AlphaArgs A;
+    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
+                            by_constituent, "rbx_alpha");
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (rbx::alpha_tile_k(nf)) {
+        case 4: return rbx::launch_alpha<4>(A, precision, st);
+        case 2: return rbx::launch_alpha<2>(A, precision, st);
+        default: return rbx::launch_alpha<1>(A, precision, st);
+    }
+}
+
+extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,
+                                 const double *T, const double *mix, int nl, const double *freq,
+                                 const double *freq_nh3, int nf, int nf_nh3_low, const double *scale, double *dalpha,
+                                 void *stream) {
+    rbx::AlphaArgs A;
+    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nl, freq, freq_nh3, nf, nf_nh3_low, scale, dalpha,
+                            nullptr, "rbx_alpha_tangent");
+    if (rc) return rc;
+    dim3 grid;
+    rbx::set_tiles(A, 1, grid);
+    hipLaunchKernelGGL((rbx::alpha_tangent_kernel<1>), grid, dim3(rbx::kThreads), 0,
+                       static_cast<hipStream_t>(stream), A);
+    return rbx::launch_status("alpha_tangent_kernel");
+}

@@ -1,0 +1,133 @@
+// Error handling, version query and the line catalogue of librbx.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "catalog.hpp"
+
+namespace rbx {
+
+static thread_local std::string g_error;
+
+void set_error(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+const std::string &last_error() { return g_error; }
+
+}  // namespace rbx
+
+using rbx::kNumCols;
+
+extern "C" int rbx_abi_version(void) { return RBX_ABI_VERSION; }
+
+extern "C" int rbx_last_error(char *buf, size_t len) {
+    if (buf == nullptr || len == 0) return RBX_ERR_ARG;
+    const std::string &e = rbx::last_error();
+    size_t n = e.size() < len - 1 ? e.size() : len - 1;
+    std::memcpy(buf, e.data(), n);
+    buf[n] = '\0';
+    return RBX_OK;
+}
+
+extern "C" int rbx_device_count(int *count) {
+    RBX_CHECK_ARG(count != nullptr, "rbx_device_count: null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return RBX_OK;
+}
+
+extern "C" int rbx_catalog_create(rbx_catalog **out) {
+    RBX_CHECK_ARG(out != nullptr, "rbx_catalog_create: null out");
+    auto *c = new (std::nothrow) rbx_catalog;
+    if (c == nullptr) {
+        rbx::set_error("rbx_catalog_create: host allocation failed");
+        return RBX_ERR_NOMEM;
+    }
+    std::memset(c, 0, sizeof(*c));
+    *out = c;
+    return RBX_OK;
+}
+
+static void free_block(rbx_catalog *cat, int kind) {
+    if (cat->storage[kind] != nullptr) {
+        (void)hipFree(cat->storage[kind]);
+        cat->storage[kind] = nullptr;
+    }
+    std::memset(&cat->dev.blk[kind], 0, sizeof(rbx::DevBlock));
+}
+
+// Strength factor bounding a block's line weights (see DevBlock::wscale).
+static double block_wscale(int kind, int n, const double *const *col) {
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double f0 = col[0][i];
+        double s;
+        switch (kind) {
+            case RBX_LINES_NH3_ROT:
+            case RBX_LINES_NH3_V2: s = col[1][i]; break;
+            case RBX_LINES_PH3: s = col[1][i] * col[3][i] / (f0 * f0); break;
+            default: s = col[1][i] / (f0 * f0); break;
+        }
+        s = std::fabs(s);
+        if (std::isfinite(s) && s > m) m = s;
+    }
+    return m;
+}
+
+extern "C" int rbx_catalog_set_lines(rbx_catalog *cat, int kind, int n, const double *const *host_columns) {
+    RBX_CHECK_ARG(cat != nullptr, "rbx_catalog_set_lines: null catalogue");
+    RBX_CHECK_ARG(kind >= 0 && kind < RBX_NUM_LINE_KINDS, "rbx_catalog_set_lines: bad kind %d", kind);
+    RBX_CHECK_ARG(n >= 0, "rbx_catalog_set_lines: negative line count %d", n);
+    RBX_CHECK_ARG(n == 0 || host_columns != nullptr, "rbx_catalog_set_lines: null columns");
+    const int ncol = kNumCols[kind];
+    for (int i = 0; n > 0 && i < ncol; ++i)
+        RBX_CHECK_ARG(host_columns[i] != nullptr, "rbx_catalog_set_lines: column %d is null", i);
+    for (int i = 0; i < n; ++i)
+        RBX_CHECK_ARG(host_columns[0][i] > 0.0, "rbx_catalog_set_lines: line %d has centre %g <= 0",
+                      i, host_columns[0][i]);
+    free_block(cat, kind);
+    if (n == 0) return RBX_OK;
+    const size_t bytes = sizeof(double) * static_cast<size_t>(n) * ncol;
+    std::vector<double> packed(static_cast<size_t>(n) * ncol);
+    for (int c = 0; c < ncol; ++c)
+        std::memcpy(packed.data() + static_cast<size_t>(c) * n, host_columns[c], sizeof(double) * n);
+    double *d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) {
+        rbx::set_error("rbx_catalog_set_lines: hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return RBX_ERR_NOMEM;
+    }
+    e = hipMemcpy(d, packed.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        rbx::set_error("rbx_catalog_set_lines: upload failed: %s", hipGetErrorString(e));
+        return RBX_ERR_HIP;
+    }
+    cat->storage[kind] = d;
+    rbx::DevBlock &b = cat->dev.blk[kind];
+    for (int c = 0; c < ncol; ++c) b.col[c] = d + static_cast<size_t>(c) * n;
+    b.n = n;
+    b.wscale = block_wscale(kind, n, host_columns);
+    return RBX_OK;
+}
+
+extern "C" int rbx_catalog_lines(const rbx_catalog *cat, int kind, int *n) {
+    RBX_CHECK_ARG(cat != nullptr && n != nullptr, "rbx_catalog_lines: null argument");
+    RBX_CHECK_ARG(kind >= 0 && kind < RBX_NUM_LINE_KINDS, "rbx_catalog_lines: bad kind %d", kind);
+    *n = cat->dev.blk[kind].n;
+    return RBX_OK;
+}
+
+extern "C" int rbx_catalog_destroy(rbx_catalog *cat) {
+    if (cat == nullptr) return RBX_OK;
+    for (int k = 0; k < RBX_NUM_LINE_KINDS; ++k) free_block(cat, k);
+    delete cat;
+    return RBX_OK;
+}

@@ -1,0 +1,251 @@
+// Brightness integration along ray paths, and its Jacobian.
+//
+// Lane = frequency, wavefront = one ray x 64 consecutive frequencies.  The
+// step loop follows Brightness.single (brightness.py:53-117):
+//   dtau = (a0 + a1) ds/2, tau_{i+1} = tau_i + dtau          (FP64 prefix sum)
+//   W_{i+1} = a1 e^{-tau} (point) or 2 a1 E2(tau) (disc), W_0 = 0
+//   intW += (W_{i+1} + W_i) ds/2, Tb_lyr += (T1 W_{i+1} + T0 W_i) ds/2
+//   Tb = T_cmb if Tb_lyr < T_cmb else Tb_lyr / intW
+// Path point k is layer k (the reference's paths are ingress-only), so alpha
+// for a step is one coalesced row read alpha[k][f0..f0+63]; the ray's ds and
+// T[k] are wave-uniform scalar loads.  Blocks map frequency groups to XCDs
+// (block b -> group b % ngroups) so each XCD's L2 holds a slice of the table.
+#include "common.hpp"
+
+namespace rbx {
+
+constexpr int kWaves = 4;
+constexpr int kLanes = 64;
+
+struct IntArgs {
+    const void *alpha;
+    int f32;
+    const double *T;
+    int nl, nf;
+    const double *hds;
+    int ld;
+    const int *nsteps;
+    int nray;
+    int mode, fast_exp;
+    double tau_cut;
+    double *Tb, *intW;
+    int ngroups;
+};
+
+__device__ inline double load_alpha(const void *p, int f32, size_t i) {
+    return f32 ? static_cast<double>(static_cast<const float *>(p)[i]) : static_cast<const double *>(p)[i];
+}
+
+__global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
+    const int wave = threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
+    const int group = blockIdx.x % A.ngroups;
+    const int ray = __builtin_amdgcn_readfirstlane((blockIdx.x / A.ngroups) * kWaves + wave);
+    if (ray >= A.nray) return;
+    const int j = group * kLanes + lane;
+    const bool ok = j < A.nf;
+    const int jj = ok ? j : A.nf - 1;
+    const int S = min(A.nsteps[ray], A.nl - 1);   // never read past the ray's ds row or the table
+    const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
+    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
+    if (S > 1) {
+        double a0 = load_alpha(A.alpha, A.f32, jj);
+        for (int i = 0; i < S - 1; ++i) {
+            const double hh = h[i];
+            const double a1 = load_alpha(A.alpha, A.f32, static_cast<size_t>(i + 1) * A.nf + jj);
+            const double T1 = A.T[i + 1];
+            tau = tau + (a0 + a1) * hh;
+            double W1;
+            if (A.mode == 1)
+                W1 = 2.0 * a1 * expint_e2(tau);
+            else
+                W1 = a1 * (A.fast_exp ? exp_neg_mixed(tau) : exp(-tau));
+            intW = intW + (W1 + W0) * hh;
+            const double TW1 = T1 * W1;
+            num = num + (TW1 + TW0) * hh;
+            a0 = a1;
+            W0 = W1;
+            TW0 = TW1;
+            if (A.tau_cut > 0.0 && (i & 7) == 7) {
+                if (__all(!ok || tau > A.tau_cut)) break;
+            }
+        }
+    }
+    if (!ok) return;
+    const size_t o = static_cast<size_t>(ray) * A.nf + j;
+    double Tb;
+    if (S < 0 || num < kTcmb)
+        Tb = kTcmb;
+    else
+        Tb = num / intW;
+    A.Tb[o] = Tb;
+    if (A.intW) A.intW[o] = (S < 0) ? 0.0 : intW;
+}
+
+// ---- Jacobian: two forward passes per (ray, freq) ----
+// With c_k = h_{k-1} + h_k (h_{-1} = h_{S-1} = 0), D = intW, N = Tb_lyr:
+//   lambda_k = dTb/dW_k = c_k (T_k - Tb) / D,   sum_k lambda_k W_k = (N - Tb D)/D = 0,
+//   dTb/dT_k|direct = c_k W_k / D,
+//   dTb/dalpha_k = lambda_k e^{-tau_k} + h_{k-1} P_{k-1} + h_k P_k,  P_k = sum_{j<=k} lambda_j W_j,
+// the last form using Lambda_k = sum_{j>k} lambda_j W_j = -P_k.  e^{-tau} is
+// recomputed in the second pass, never stored.
+struct JacArgs {
+    IntArgs I;
+    const double *dalpha;  // [3][nl][nf]
+    float *J;              // [nray][2][nl][nf]
+};
+
+__global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
+    const IntArgs &A = Aj.I;
+    const int wave = threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
+    const int group = blockIdx.x % A.ngroups;
+    const int ray = __builtin_amdgcn_readfirstlane((blockIdx.x / A.ngroups) * kWaves + wave);
+    if (ray >= A.nray) return;
+    const int j = group * kLanes + lane;
+    const bool ok = j < A.nf;
+    const int jj = ok ? j : A.nf - 1;
+    const int S = min(A.nsteps[ray], A.nl - 1);   // never read past the ray's ds row or the table
+    const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
+    const size_t nlnf = static_cast<size_t>(A.nl) * A.nf;
+    float *JT = Aj.J + static_cast<size_t>(ray) * 2 * nlnf;
+    float *Jq = JT + nlnf;
+
+    // pass 1: Tb and intW (same recurrence and cut as integrate_kernel)
+    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
+    int last = 0;   // last path point reached
+    if (S > 1) {
+        double a0 = load_alpha(A.alpha, A.f32, jj);
+        for (int i = 0; i < S - 1; ++i) {
+            const double hh = h[i];
+            const double a1 = load_alpha(A.alpha, A.f32, static_cast<size_t>(i + 1) * A.nf + jj);
+            tau = tau + (a0 + a1) * hh;
+            const double W1 = a1 * (A.fast_exp ? exp_neg_mixed(tau) : exp(-tau));
+            intW = intW + (W1 + W0) * hh;
+            const double TW1 = A.T[i + 1] * W1;
+            num = num + (TW1 + TW0) * hh;
+            a0 = a1;
+            W0 = W1;
+            TW0 = TW1;
+            last = i + 1;
+            if (A.tau_cut > 0.0 && (i & 7) == 7) {
+                if (__all(!ok || tau > A.tau_cut)) break;
+            }
+        }
+    }
+    const bool live = S > 1 && !(num < kTcmb);
+    const double Tb = (S < 0 || num < kTcmb) ? kTcmb : num / intW;
+    if (ok && A.Tb) A.Tb[static_cast<size_t>(ray) * A.nf + j] = Tb;
+
+    // pass 2: gradients along the path; zero elsewhere
+    int k = 0;
+    if (live) {
+        const double invD = 1.0 / intW;
+        double P = 0.0;          // P_{k-1}
+        double hprev = 0.0;      // h_{k-1}
+        double a_k = load_alpha(A.alpha, A.f32, jj);
+        tau = 0.0;
+        for (k = 0; k <= last; ++k) {
+            const double hk = (k < last) ? h[k] : 0.0;   // the cut (or path end) closes the sums
+            double e = 1.0, W = 0.0;
+            if (k > 0) {
+                e = A.fast_exp ? exp_neg_mixed(tau) : exp(-tau);
+                W = a_k * e;
+            }
+            const double ck = hprev + hk;
+            const double lam = ck * (A.T[k] - Tb) * invD;
+            const double Pk = P + lam * W;
+            const double ga = (k > 0 ? lam * e : 0.0) + hprev * P + hk * Pk;
+            const size_t o = static_cast<size_t>(k) * A.nf + jj;
+            const double dT = ck * W * invD + ga * Aj.dalpha[o];
+            const double dq = ga * Aj.dalpha[nlnf + o];
+            if (ok) {
+                JT[static_cast<size_t>(k) * A.nf + j] = static_cast<float>(dT);
+                Jq[static_cast<size_t>(k) * A.nf + j] = static_cast<float>(dq);
+            }
+            if (k < last) {
+                const double a_next = load_alpha(A.alpha, A.f32, static_cast<size_t>(k + 1) * A.nf + jj);
+                tau = tau + (a_k + a_next) * hk;
+                a_k = a_next;
+            }
+            P = Pk;
+            hprev = hk;
+        }
+    }
+    if (ok) {
+        for (; k < A.nl; ++k) {
+            JT[static_cast<size_t>(k) * A.nf + j] = 0.0f;
+            Jq[static_cast<size_t>(k) * A.nf + j] = 0.0f;
+        }
+    }
+}
+
+static int check_int_args(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                          const double *hds, int ld, const int *nsteps, int nray, int precision) {
+    RBX_CHECK_ARG(alpha && T && hds && nsteps, "integration: null device pointer");
+    RBX_CHECK_ARG(nl >= 2 && nf > 0 && nray >= 0, "integration: bad sizes nl=%d nf=%d nray=%d", nl, nf, nray);
+    RBX_CHECK_ARG(ld >= nl - 1, "integration: ld=%d < nl-1=%d", ld, nl - 1);
+    RBX_CHECK_ARG(alpha_precision == RBX_FP64 || alpha_precision == RBX_FP32, "integration: bad alpha precision");
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "integration: bad precision");
+    return RBX_OK;
+}
+
+static void fill_int_args(IntArgs &A, const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                          const double *hds, int ld, const int *nsteps, int nray, int precision, double tau_cut) {
+    A.alpha = alpha;
+    A.f32 = alpha_precision == RBX_FP32;
+    A.T = T;
+    A.nl = nl;
+    A.nf = nf;
+    A.hds = hds;
+    A.ld = ld;
+    A.nsteps = nsteps;
+    A.nray = nray;
+    A.mode = 0;
+    A.fast_exp = precision == RBX_FP32;
+    A.tau_cut = tau_cut;
+    A.Tb = nullptr;
+    A.intW = nullptr;
+    A.ngroups = (nf + kLanes - 1) / kLanes;
+}
+
+}  // namespace rbx
+
+extern "C" int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                             const double *hds, int ld, const int *nsteps, int nray, int mode, int precision,
+                             double tau_cut, double *Tb, double *intW, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    RBX_CHECK_ARG(Tb != nullptr, "rbx_integrate: null Tb");
+    RBX_CHECK_ARG(mode == 0 || mode == 1, "rbx_integrate: bad mode %d", mode);
+    if (nray == 0) return RBX_OK;
+    rbx::IntArgs A;
+    rbx::fill_int_args(A, alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision, tau_cut);
+    A.mode = mode;
+    A.Tb = Tb;
+    A.intW = intW;
+    const long long blocks = static_cast<long long>(A.ngroups) * ((nray + rbx::kWaves - 1) / rbx::kWaves);
+    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate: too many blocks");
+    hipLaunchKernelGGL(rbx::integrate_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes),
+                       0, static_cast<hipStream_t>(stream), A);
+    return rbx::launch_status("integrate_kernel");
+}
+
+extern "C" int rbx_jacobian(const void *alpha, int alpha_precision, const double *dalpha, const double *T, int nl,
+                            int nf, const double *hds, int ld, const int *nsteps, int nray, double tau_cut, float *J,
+                            double *Tb, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, alpha_precision);
+    if (rc) return rc;
+    RBX_CHECK_ARG(dalpha != nullptr && J != nullptr, "rbx_jacobian: null dalpha or J");
+    if (nray == 0) return RBX_OK;
+    rbx::JacArgs Aj;
+    rbx::fill_int_args(Aj.I, alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, alpha_precision, tau_cut);
+    Aj.I.Tb = Tb;
+    Aj.dalpha = dalpha;
+    Aj.J = J;
+    const long long blocks = static_cast<long long>(Aj.I.ngroups) * ((nray + rbx::kWaves - 1) / rbx::kWaves);
+    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_jacobian: too many blocks");
+    hipLaunchKernelGGL(rbx::jacobian_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes),
+                       0, static_cast<hipStream_t>(stream), Aj);
+    return rbx::launch_status("jacobian_kernel");
+}

@@ -1,0 +1,224 @@
+// Ray geometry in FP64: one thread per ray.
+//
+// Follows raypath.compute_ds / findEdge (raypath.py:60-273) and the ellipse
+// branch of Shape (shape.py:223-298) step for step.  The reference converts
+// latitude / longitude to degrees and back and rebuilds sines through
+// arcsin/arccos; here the same quantities come from the vector components
+// (sin lat = y/|v|, cos lat = sqrt(x^2+z^2)/|v|, cos lng = z/sqrt(x^2+z^2), ...),
+// which is the same arithmetic up to rounding, with no transcendental calls in
+// the loop.  Quirks kept: the latitude-0 substitution lat = 1e-6 rad
+// (shape.py:231-233), the next-shell radius taken at the current point's
+// latitude (raypath.py:182), refraction only at the first boundary
+// (raypath.py:158, 257), the numpy arange trial grid and linear root of
+// findEdge (raypath.py:64, 94-97), the ds < 0 truncation (raypath.py:212-216),
+// NaN (not an exception) when a ray misses the next shell (raypath.py:190-209),
+// and the loop end when layer + 1 runs past the last layer (raypath.py:255-264).
+#include "common.hpp"
+
+namespace rbx {
+
+struct V3 {
+    double x, y, z;
+};
+__device__ inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline double norm(V3 a) { return sqrt(dot(a, a)); }
+
+struct Geo {
+    double ratio;   // Rpol/Req for 'ellipse', 1 for 'sphere'
+    double M[9];    // rotate2planet = rotX(rotate) * rotZ(tip), row-major
+};
+
+__device__ inline V3 mat(const double *M, V3 v) {
+    return {M[0] * v.x + M[1] * v.y + M[2] * v.z, M[3] * v.x + M[4] * v.y + M[5] * v.z,
+            M[6] * v.x + M[7] * v.y + M[8] * v.z};
+}
+
+// sines/cosines of (planetocentric latitude, delta longitude) of vector v
+struct LatLng {
+    double slat, clat, slng, clng;
+};
+
+__device__ inline LatLng latlng(V3 v) {
+    LatLng o;
+    const double r = norm(v);
+    const double h = sqrt(v.x * v.x + v.z * v.z);
+    if (v.y == 0.0) {                 // lat == 0 -> 1e-6 rad (shape.py:231-233)
+        o.slat = 9.999999999998333e-07;   // sin(1e-6)
+        o.clat = 0.9999999999995;         // cos(1e-6)
+    } else {
+        o.slat = v.y / r;
+        o.clat = h / r;
+    }
+    if (h > 0.0) {
+        o.slng = v.x / h;
+        o.clng = v.z / h;
+    } else {                           // atan2(0, 0) = 0
+        o.slng = 0.0;
+        o.clng = 1.0;
+    }
+    return o;
+}
+
+// Shape._calcEllipse for equatorial radius a at (lat, lng): |r|, r and normal n
+struct Surf {
+    double rmag;
+    V3 r, n;
+};
+
+__device__ inline Surf ellipse(const Geo &G, double a, const LatLng &q) {
+    const double b = G.ratio * a;
+    V3 nn = {0.0, a * q.slat, b * q.clat};
+    const double inv = 1.0 / sqrt(nn.y * nn.y + nn.z * nn.z);
+    nn.y *= inv;
+    nn.z *= inv;
+    Surf s;
+    s.n = {q.slng * nn.z, nn.y, q.clng * nn.z};                      // rotY(lng, norm)
+    const double rc = a * q.clat;
+    s.r = {q.slng * rc, b * q.slat, q.clng * rc};                    // rotY(lng, r_vec)
+    s.rmag = norm(s.r);
+    return s;
+}
+
+struct RayArgs {
+    Geo G;
+    const double *R, *N;
+    int nl;
+    const double *b;
+    int nray;
+    int units;   // 0 km, 1 half step in cm
+    double *ds;
+    int ld;
+    int *nsteps;
+    int limb;
+};
+
+__global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= A.nray) return;
+    const double b0 = A.b[2 * ray], b1 = A.b[2 * ray + 1];
+    double *out = A.ds + static_cast<size_t>(ray) * A.ld;
+    const Geo &G = A.G;
+    const double rNorm = A.R[0];
+    const double tmp = b0 * b0 + b1 * b1;
+    if (!(tmp < 1.0)) {                                // raypath.py:126-127 (NaN b: off planet too)
+        A.nsteps[ray] = -1;
+        return;
+    }
+    const double mu = sqrt(1.0 - tmp);
+
+    // ---- findEdge: trial zQ = arange(1.01 sqrt(1-b^2), 0, -0.005) ----
+    const double start = sqrt(1.0 - tmp) * 1.01;
+    const int ntrial = static_cast<int>(ceil((0.0 - start) / -0.005));
+    const double z1 = start + -0.005;
+    const double dz = z1 - start;
+    double zprev = 0.0, dprev = 0.0, zhit = 0.0, dhit = 0.0;
+    int hit = -1;
+    for (int i = 0; i < ntrial; ++i) {
+        const double zq = (i == 0) ? start : ((i == 1) ? z1 : start + i * dz);
+        const V3 bv = mat(G.M, V3{b0, b1, zq});
+        const double r1 = norm(bv) * rNorm;
+        const Surf s = ellipse(G, rNorm, latlng(bv));
+        const double dd = r1 - s.rmag;
+        if (r1 < s.rmag) {
+            hit = i;
+            zhit = zq;
+            dhit = dd;
+            break;
+        }
+        zprev = zq;
+        dprev = dd;
+    }
+    if (hit < 0) {
+        A.nsteps[ray] = -1;
+        return;
+    }
+    // np.interp(0, [dhit, dprev], [zhit, zprev]) on the two bracketing trials
+    double zq;
+    if (hit == 0 || dhit == 0.0) {
+        zq = zhit;
+    } else {
+        const double slope = (zprev - zhit) / (dprev - dhit);
+        zq = slope * (0.0 - dhit) + zhit;
+        if (zq != zq) zq = slope * (0.0 - dprev) + zprev;
+    }
+    const V3 edge = rNorm * mat(G.M, V3{b0, b1, zq});
+
+    // ---- compute_ds: start on the top shell at the edge's latitude ----
+    LatLng q = latlng(edge);
+    Surf geo = ellipse(G, rNorm, q);
+    V3 s = mat(G.M, V3{0.0, 0.0, -1.0});
+    V3 r = geo.r;
+    V3 n = geo.n;
+    double c = -dot(s, n);                            // cos(t_inc)
+    double nratio = A.N[0] / A.N[1];
+    double st = nratio * sqrt(1.0 - c * c);          // sin(t_tran)
+    double ct = sqrt(1.0 - st * st);                 // cos(t_tran)
+    const double cm_half = (A.units == 1);
+    int layer = 0;
+    int count = 0;
+    while (true) {
+        s = nratio * s + (nratio * c - ct) * n;      // raypath.py:179-180 (ingress)
+        const double rNow = geo.rmag;
+        const double rNext = ellipse(G, A.R[layer + 1], q).rmag;
+        const double rdots = dot(r, s);
+        const double disc = rdots * rdots + rNext * rNext - rNow * rNow;
+        double ds = -rdots - sqrt(disc);             // NaN when the shell is missed
+        if (ds < 0.0) break;                         // raypath.py:212-216
+        if (A.limb == 1) ds = fabs(rNext - rNow) / mu;
+        out[count] = cm_half ? (ds * 1.0E3 / 1.0E-2) * 0.5 : ds;
+        ++count;
+        const V3 rn = r + ds * s;
+        q = latlng(rn);
+        geo = ellipse(G, A.R[layer + 1], q);
+        r = rn;
+        n = geo.n;
+        ++layer;
+        c = -dot(s, n);
+        if (layer + 1 >= A.nl) break;                // IndexError on nr[layer + 1]
+        nratio = 1.0;
+        st = sqrt(1.0 - c * c);
+        ct = sqrt(1.0 - st * st);
+    }
+    A.nsteps[ray] = count;
+}
+
+}  // namespace rbx
+
+extern "C" int rbx_raypaths(const rbx_planet *planet, const double *R, const double *N, int nl, const double *b,
+                            int nray, int out_units, double *ds, int ld, int *nsteps, void *stream) {
+    RBX_CHECK_ARG(planet && R && N && b && ds && nsteps, "rbx_raypaths: null argument");
+    RBX_CHECK_ARG(nl >= 2, "rbx_raypaths: need at least 2 layers (nl=%d)", nl);
+    RBX_CHECK_ARG(nray >= 0, "rbx_raypaths: negative ray count");
+    RBX_CHECK_ARG(ld >= nl - 1, "rbx_raypaths: ld=%d < nl-1=%d", ld, nl - 1);
+    RBX_CHECK_ARG(out_units == 0 || out_units == 1, "rbx_raypaths: bad out_units %d", out_units);
+    RBX_CHECK_ARG(planet->gtype == 0 || planet->gtype == 1, "rbx_raypaths: bad gtype %d", planet->gtype);
+    RBX_CHECK_ARG(planet->limb == 0 || planet->limb == 1, "rbx_raypaths: bad limb %d", planet->limb);
+    RBX_CHECK_ARG(planet->req > 0.0 && planet->rpol > 0.0, "rbx_raypaths: radii must be positive");
+    if (nray == 0) return RBX_OK;
+    rbx::RayArgs A;
+    // computeAspect (raypath.py:39-44)
+    const double f = 1.0 - planet->rpol / planet->req;
+    const double tip = -planet->orientation[0] * rbx::kPi / 180.0;
+    const double rot = -std::atan(std::tan(planet->orientation[1] * rbx::kPi / 180.0) * (1.0 - f) * (1.0 - f));
+    const double ct = std::cos(tip), st = std::sin(tip), cr = std::cos(rot), sr = std::sin(rot);
+    // rotX(rot) * rotZ(tip)
+    const double M[9] = {ct, -st, 0.0, cr * st, cr * ct, -sr, sr * st, sr * ct, cr};
+    for (int i = 0; i < 9; ++i) A.G.M[i] = M[i];
+    A.G.ratio = planet->gtype == 0 ? planet->rpol / planet->req : 1.0;
+    A.R = R;
+    A.N = N;
+    A.nl = nl;
+    A.b = b;
+    A.nray = nray;
+    A.units = out_units;
+    A.ds = ds;
+    A.ld = ld;
+    A.nsteps = nsteps;
+    A.limb = planet->limb;
+    const int threads = 64;
+    hipLaunchKernelGGL(rbx::raypath_kernel, dim3((nray + threads - 1) / threads), dim3(threads), 0,
+                       static_cast<hipStream_t>(stream), A);
+    return rbx::launch_status("raypath_kernel");
+}

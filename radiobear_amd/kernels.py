@@ -1,0 +1,263 @@
+"""Python entry points of the HIP kernels (torch tensors as device buffers).
+
+Each function checks shapes and dtypes on the host, then calls the C ABI of
+librbx.so on torch's current HIP stream.  Inputs may be numpy arrays or torch
+tensors; outputs are torch tensors on the GPU.  There is no CPU path: without
+a HIP device (or without the library) these functions raise.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+F_SPLIT = 30.0          # nh3_hs switches coefficient sets above 30 GHz (nh3_hs.py:84)
+NH3_MODELS = {'nh3_hs_sjs': 1, 'nh3_hs': 2, 'nh3_sjs': 3}
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError('radiobear_amd needs a HIP device (torch.cuda.is_available() is False)')
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def as_device(x, dtype=torch.float64):
+    """Contiguous device tensor of dtype from numpy / list / tensor."""
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=device(), dtype=dtype)
+    else:
+        t = torch.as_tensor(np.asarray(x), dtype=dtype, device=device())
+    return t.contiguous()
+
+
+# ------------------------------------------------------------------ catalogue
+class Catalog:
+    """Device-resident line lists (rbx_catalog)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().rbx_catalog_create(ctypes.byref(h)))
+        self._h = h
+        self.counts = {}
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_lines(self, kind, data):
+        """Upload block ``kind`` from a dict holding its columns (see _lib.LINE_COLUMNS)."""
+        names = _lib.LINE_COLUMNS[kind]
+        cols = [np.ascontiguousarray(np.asarray(data[n], dtype=np.float64)) for n in names]
+        n = len(cols[0])
+        if any(len(c) != n for c in cols):
+            raise ValueError('line block {}: columns of unequal length'.format(kind))
+        arr = (ctypes.POINTER(ctypes.c_double) * len(cols))(
+            *[c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in cols])
+        check(lib().rbx_catalog_set_lines(self._h, kind, n, arr))
+        self.counts[kind] = n
+
+    def lines(self, kind):
+        n = ctypes.c_int()
+        check(lib().rbx_catalog_lines(self._h, kind, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, '_h', None) is not None and _lib._lib is not None:
+            _lib._lib.rbx_catalog_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def absorbers_struct(constituent_alpha, h2state='e'):
+    """rbx_absorbers from a reference-style {'nh3': 'nh3_hs_sjs', ...} mapping."""
+    ab = _lib.Absorbers()
+    for c, model in constituent_alpha.items():
+        if model is None:
+            continue
+        if c == 'h2':
+            if model != 'h2_jj_ddb':
+                raise ValueError('unsupported h2 absorber {}'.format(model))
+            if h2state not in ('e', 'n'):
+                raise ValueError('INVALID H2STATE {!r}'.format(h2state))
+            ab.h2 = 1 if h2state == 'e' else 2
+        elif c == 'h2o':
+            if model != 'h2o_bk':
+                raise ValueError('unsupported h2o absorber {}'.format(model))
+            ab.h2o = 1
+        elif c == 'h2s':
+            if model != 'h2s_ddb':
+                raise ValueError('unsupported h2s absorber {}'.format(model))
+            ab.h2s = 1
+        elif c == 'nh3':
+            if model not in NH3_MODELS:
+                raise ValueError('unsupported nh3 absorber {}'.format(model))
+            ab.nh3 = NH3_MODELS[model]
+        elif c == 'ph3':
+            if model != 'ph3_jh':
+                raise ValueError('unsupported ph3 absorber {}'.format(model))
+            ab.ph3 = 1
+        else:
+            raise ValueError('unsupported constituent {}'.format(c))
+    return ab
+
+
+def nh3_order(freqs):
+    """Column order of nh3_hs's output: request indices <= 30 GHz, then > 30 GHz."""
+    f = np.asarray(freqs, dtype=np.float64)
+    lo = np.where(f <= F_SPLIT)[0]
+    hi = np.where(f > F_SPLIT)[0]
+    return np.concatenate([lo, hi]), len(lo)
+
+
+def mix_table(gas, C):
+    """(7, nl) mixing ratios in RBX_X_* order from a reference gas table."""
+    return np.stack([np.asarray(gas[C[s]], dtype=np.float64) for s in _lib.SPECIES])
+
+
+def _alpha_inputs(P, T, mix, freqs, scale):
+    f = np.asarray(freqs, dtype=np.float64)
+    order, nf_low = nh3_order(f)
+    P = as_device(P)
+    T = as_device(T)
+    mix = as_device(mix)
+    nl = P.shape[0]
+    if T.shape != (nl,) or mix.shape != (len(_lib.SPECIES), nl):
+        raise ValueError('shape mismatch: P {}, T {}, mix {}'.format(tuple(P.shape), tuple(T.shape),
+                                                                     tuple(mix.shape)))
+    fd = as_device(f)
+    fn = None if np.array_equal(order, np.arange(len(f))) else as_device(f[order])
+    sc = None
+    if scale is not None:
+        sc = as_device(scale)
+        if sc.shape != (len(_lib.CONSTITUENTS), nl):
+            raise ValueError('scale must be ({}, {})'.format(len(_lib.CONSTITUENTS), nl))
+    return P, T, mix, fd, fn, nf_low, sc, nl
+
+
+def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by_constituent=False):
+    """alpha[nl][nf] (cm^-1) on the device; optionally also [5][nl][nf] per constituent."""
+    prec = _lib.PRECISIONS[precision]
+    P, T, mix, fd, fn, nf_low, sc, nl = _alpha_inputs(P, T, mix, freqs, scale)
+    nf = fd.shape[0]
+    out = torch.empty((nl, nf), dtype=torch.float32 if prec == _lib.FP32 else torch.float64, device=device())
+    byc = torch.zeros((len(_lib.CONSTITUENTS), nl, nf), dtype=torch.float64, device=device()) \
+        if by_constituent else None
+    check(lib().rbx_alpha(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), nl, _ptr(fd),
+                          _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(out), _ptr(byc), _stream()))
+    return (out, byc) if by_constituent else out
+
+
+def alpha_tangent(catalog, absorbers, P, T, mix, freqs, scale=None):
+    """d alpha / d(T, q_NH3, q_H2O): (3, nl, nf) float64 on the device."""
+    P, T, mix, fd, fn, nf_low, sc, nl = _alpha_inputs(P, T, mix, freqs, scale)
+    nf = fd.shape[0]
+    out = torch.empty((3, nl, nf), dtype=torch.float64, device=device())
+    check(lib().rbx_alpha_tangent(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), nl,
+                                  _ptr(fd), _ptr(fn), nf, nf_low, _ptr(sc), _ptr(out), _stream()))
+    return out
+
+
+def planet_desc(req, rpol, orientation=(0.0, 0.0), gtype='ellipse', limb='shape'):
+    gt = {'ellipse': 0, 'sphere': 1, 'circle': 1}
+    lm = {'shape': 0, 'sec': 1}
+    if gtype not in gt:
+        raise ValueError('{} not a supported planet shape (ellipse, sphere, circle)'.format(gtype))
+    if limb not in lm:
+        raise ValueError('unsupported limb {}'.format(limb))
+    d = _lib.PlanetDesc()
+    d.req = float(req)
+    d.rpol = float(rpol)
+    d.orientation[0] = float(orientation[0])
+    d.orientation[1] = float(orientation[1])
+    d.gtype = gt[gtype]
+    d.limb = lm[limb]
+    return d
+
+
+def raypaths(planet, R, N, b, units='half_cm'):
+    """Ray paths for impact parameters b (nray, 2): (ds (nray, nl-1) float64, nsteps (nray,) int32)."""
+    R = as_device(R)
+    N = as_device(N)
+    b = as_device(np.asarray(b, dtype=np.float64).reshape(-1, 2))
+    nl = R.shape[0]
+    nray = b.shape[0]
+    ld = max(nl - 1, 1)
+    ds = torch.empty((nray, ld), dtype=torch.float64, device=device())
+    ns = torch.empty((nray,), dtype=torch.int32, device=device())
+    check(lib().rbx_raypaths(ctypes.byref(planet), _ptr(R), _ptr(N), nl, _ptr(b), nray,
+                             1 if units == 'half_cm' else 0, _ptr(ds), ld, _ptr(ns), _stream()))
+    return ds, ns
+
+
+def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False):
+    """Tb (nray, nf) float64 from an alpha table (nl, nf) and ray paths."""
+    aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
+    prec = aprec if precision is None else _lib.PRECISIONS[precision]
+    a = alpha_table.contiguous()
+    T = as_device(T)
+    nl, nf = a.shape
+    nray, ld = hds.shape
+    Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
+    iw = torch.empty((nray, nf), dtype=torch.float64, device=device()) if with_intW else None
+    check(lib().rbx_integrate(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds.contiguous()), ld,
+                              _ptr(nsteps.contiguous()), nray, 1 if disc else 0, prec, float(tau_cut),
+                              _ptr(Tb), _ptr(iw), _stream()))
+    return (Tb, iw) if with_intW else Tb
+
+
+def jacobian(alpha_table, dalpha, T, hds, nsteps, tau_cut=0.0):
+    """J (nray, 2, nl, nf) float32: dTb/dT and dTb/dq_NH3 per layer; and Tb (nray, nf)."""
+    aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
+    a = alpha_table.contiguous()
+    T = as_device(T)
+    nl, nf = a.shape
+    nray, ld = hds.shape
+    J = torch.empty((nray, 2, nl, nf), dtype=torch.float32, device=device())
+    Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
+    check(lib().rbx_jacobian(_ptr(a), aprec, _ptr(dalpha.contiguous()), _ptr(T), nl, nf,
+                             _ptr(hds.contiguous()), ld, _ptr(nsteps.contiguous()), nray, float(tau_cut),
+                             _ptr(J), _ptr(Tb), _stream()))
+    return J, Tb
+
+
+# ------------------------------------------------------------- Jupiter lines
+def jupiter_catalog(raw, truncate_strength=None, truncate_freq=None, nh3_inversion=None, h2o=None):
+    """Catalogue with the Jupiter line lists and the config's truncation.
+
+    raw: lines.raw_catalog(); truncate_strength/freq: per-constituent dicts as
+    config.truncate_* (defaults: h2s and ph3 at I0 > 1e-22); nh3_inversion /
+    h2o: optional replacement lists (synthetic workloads).
+    """
+    from . import lines as L
+    ts = {'h2s': 1e-22, 'ph3': 1e-22} if truncate_strength is None else truncate_strength
+    tf = truncate_freq or {}
+    cat = Catalog()
+    amm = dict(raw['ammonia'])
+    if nh3_inversion is not None:
+        for k in ('fo', 'Io', 'Eo', 'gammaNH3o'):
+            amm[k] = nh3_inversion[k]
+    cat.set_lines(_lib.LINES_NH3_INV, amm)
+    cat.set_lines(_lib.LINES_NH3_ROT, amm)
+    cat.set_lines(_lib.LINES_NH3_V2, amm)
+    cat.set_lines(_lib.LINES_NH3_SJS, raw['nh3'])
+    cat.set_lines(_lib.LINES_H2S, L.truncate(raw['h2s'], ('f0', 'I0', 'E', 'GH2S'),
+                                             ts.get('h2s'), tf.get('h2s')))
+    cat.set_lines(_lib.LINES_PH3, L.truncate_ph3(raw['ph3jh'], raw['PH3WGT'], ts.get('ph3'), tf.get('ph3')))
+    h2o_lines = L.h2o_bk_lines() if h2o is None else h2o
+    cat.set_lines(_lib.LINES_H2O, L.truncate_h2o(h2o_lines, ts.get('h2o'), tf.get('h2o')))
+    return cat

@@ -24,6 +24,49 @@ def raw_catalog():
     return {name: load(name) for name in CATALOGUES}
 
 
+def truncate(data, keys, strength=None, freq=None):
+    """Load-time line cut of h2s_ddb (constituents/h2s/h2s_ddb.py:23-38):
+    keep I0 > strength, then f0 < freq, on the columns ``keys``."""
+    out = {k: np.asarray(data[k], dtype=np.float64) for k in keys}
+    if strength is not None:
+        keep = out['I0'] > strength
+        out = {k: v[keep] for k, v in out.items()}
+    if freq is not None:
+        keep = out['f0'] < freq
+        out = {k: v[keep] for k, v in out.items()}
+    return out
+
+
+def truncate_ph3(lines, weights, strength=None, freq=None):
+    """Load-time cut of ph3_jh (constituents/ph3/ph3_jh.py:28-60): lines and
+    their weights are cut together; a cut that leaves them unequal raises."""
+    d = {k: np.asarray(lines[k], dtype=np.float64) for k in ('f0', 'I0', 'E')}
+    w = {k: np.asarray(weights[k], dtype=np.float64) for k in ('WgtI0', 'WgtFGB', 'WgtSB')}
+    if len(w['WgtI0']) != len(d['f0']):
+        raise ValueError("PH3 wgt vector wrong length.")
+    keep = np.ones(len(d['f0']), dtype=bool)
+    if strength is not None:
+        keep &= d['I0'] > strength
+    if freq is not None:
+        keep &= d['f0'] < freq
+    out = {k: v[keep] for k, v in d.items()}
+    out.update({k: v[keep] for k, v in w.items()})
+    return out
+
+
+def truncate_h2o(data, strength=None, freq=None):
+    """Load-time cut of h2o_bk (constituents/h2o/h2o_bk.py:51-62); a zero
+    strength or frequency means no cut, as there."""
+    d = {k: np.asarray(v, dtype=np.float64) for k, v in data.items()}
+    if strength:
+        keep = d['I_o'] > strength
+        d = {k: v[keep] for k, v in d.items()}
+    if freq:
+        keep = d['f_o'] < freq
+        d = {k: v[keep] for k, v in d.items()}
+    return d
+
+
 def h2o_bk_lines():
     """Karpowicz/Rosenkranz H2O lines in the h2o_bk schema, widths in GHz/bar."""
     f_o = [22.2351, 183.3101, 321.2256, 325.1529, 380.1974, 439.1508, 443.0183,

@@ -1,0 +1,87 @@
+"""CPU checks of the C-ABI library and the host-side logic (no GPU compute)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import absorbers as OA
+from radiobear_amd import _lib, kernels, lines
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'include', 'rbx.h')
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r'^int\s+(rbx_\w+)\s*\(', text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = declared()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTS)
+    assert lib.rbx_abi_version() == _lib.ABI_VERSION
+
+
+def test_argument_errors_are_reported_without_gpu_work():
+    lib = _lib.lib()
+    pl = kernels.planet_desc(71492.0, 66854.0)
+    rc = lib.rbx_raypaths(ctypes.byref(pl), None, None, 1, None, 1, 0, None, 0, None, None)
+    assert rc == -1
+    assert 'null' in _lib.last_error() or 'layers' in _lib.last_error()
+    with pytest.raises(_lib.RbxError) as e:
+        _lib.check(lib.rbx_integrate(None, 0, None, 10, 5, None, 9, None, 1, 0, 0, 0.0, None, None, None))
+    assert e.value.status == -1
+    h = ctypes.c_void_p()
+    assert lib.rbx_catalog_create(ctypes.byref(h)) == 0
+    assert lib.rbx_catalog_set_lines(h, 99, 0, None) == -1
+    assert 'bad kind' in _lib.last_error()
+    assert lib.rbx_catalog_set_lines(h, 0, 0, None) == 0          # empty block: no device work
+    n = ctypes.c_int(-5)
+    assert lib.rbx_catalog_lines(h, 0, ctypes.byref(n)) == 0 and n.value == 0
+    ab = kernels.absorbers_struct({'nh3': 'nh3_hs'})
+    dummy = ctypes.c_void_p(8)
+    rc = lib.rbx_alpha(h, ctypes.byref(ab), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy, None, None)
+    assert rc == -4 and 'nh3_hs' in _lib.last_error()             # empty inversion block
+    assert lib.rbx_catalog_destroy(h) == 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, '_lib', None)
+    monkeypatch.setattr(_lib, 'LIB_PATH', str(tmp_path / 'nope.so'))
+    with pytest.raises(_lib.RbxLibraryError):
+        _lib.lib()
+
+
+def test_nh3_order_and_absorber_selection():
+    order, nlow = kernels.nh3_order([40.0, 1.0, 30.0, 31.0, 2.0])
+    assert list(order) == [1, 2, 4, 0, 3] and nlow == 3
+    ab = kernels.absorbers_struct({'h2': 'h2_jj_ddb', 'nh3': 'nh3_hs_sjs', 'ph3': None}, h2state='n')
+    assert (ab.h2, ab.h2o, ab.h2s, ab.nh3, ab.ph3) == (2, 0, 0, 1, 0)
+    with pytest.raises(ValueError):
+        kernels.absorbers_struct({'h2': 'h2_jj_ddb'}, h2state='x')
+    with pytest.raises(ValueError):
+        kernels.absorbers_struct({'nh3': 'nh3_dbs'})
+    with pytest.raises(ValueError):
+        kernels.planet_desc(1.0, 1.0, gtype='gravity')
+
+
+def test_truncation_matches_oracle(raw_lines):
+    a = lines.truncate(raw_lines['h2s'], ('f0', 'I0', 'E', 'GH2S'), 1e-22, 900.0)
+    b = OA.truncate_ftable(raw_lines['h2s'], 1e-22, 900.0)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    a = lines.truncate_ph3(raw_lines['ph3jh'], raw_lines['PH3WGT'], 1e-22, 1200.0)
+    b = OA.truncate_ph3(raw_lines['ph3jh'], raw_lines['PH3WGT'], 1e-22, 1200.0)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    a = lines.truncate_h2o(lines.h2o_bk_lines(), 1e-12, 600.0)
+    b = OA.truncate_h2o(OA.h2o_bk_lines(), 1e-12, 600.0)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    assert len(lines.truncate(raw_lines['h2s'], ('f0', 'I0', 'E', 'GH2S'), 1e-22)['f0']) == 121
+    assert len(lines.truncate_ph3(raw_lines['ph3jh'], raw_lines['PH3WGT'], 1e-22)['f0']) == 33
