@@ -1,0 +1,353 @@
+"""Benchmark: batched forward model (absorption + ray paths + integration) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3_large]
+
+One step = the whole hot path over one batch of synthetic input already resident
+in HBM: rbx_alpha (line sums), rbx_raypaths (FP64 geometry), rbx_integrate
+(FP64 tau scan), captured once in a HIP graph and replayed.  Prints ONE JSON
+line.  ``value`` is on-planet ray x frequency Tb values per second in the
+FP32-line-sum mode; the plain FP64 mode, the Jacobian workload (rows/s), the
+per-kernel times, the roofline of the dominant kernel and the CPU baseline
+(the FP64 oracle on a bounded sample) are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+PEAK = {'fp32': 157.3, 'fp64': 78.6, 'hbm': 8000.0}   # TFLOP/s, TFLOP/s, GB/s (MI355X)
+# algorithmic FLOP per line x layer x frequency (division = 1 FLOP)
+FLOP_BR, FLOP_GROSS, FLOP_VVW = 9, 7, 12
+FLOP_RT_STEP = 11         # 10 FLOP + 1 exp per ray x freq x step
+CLOSED_FORM_FLOP = 20     # h2 CIA / h2o continuum per layer x frequency
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=20)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--workload', default='c3_large')
+    p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-secondary', action='store_true')
+    p.add_argument('--quiet', action='store_true')
+    p.add_argument('--copy-calibration', action='store_true',
+                   help='run only a plain device copy of 1 GiB (calibrates FETCH_SIZE / WRITE_SIZE)')
+    return p.parse_args()
+
+
+def copy_calibration():
+    """A plain copy kernel of known size: 2^27 float64 = 1 GiB read + 1 GiB written."""
+    import torch
+    a = torch.ones(2 ** 27, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes': a.numel() * 8, 'launches': 3}))
+
+
+def log(args, *a):
+    if not args.quiet:
+        print(*a, file=sys.stderr, flush=True)
+
+
+class Problem:
+    """Device-resident inputs and outputs of one workload."""
+
+    def __init__(self, name):
+        import torch
+        from radiobear_amd import kernels as K, lines, synth
+        self.K = K
+        w = synth.workload(name)
+        self.w = w
+        atm = w['atm']
+        self.atm = atm
+        raw = lines.raw_catalog()
+        self.cat = K.jupiter_catalog(raw, nh3_inversion=w['nh3_inv'], h2o=w['h2o'])
+        self.ab = K.absorbers_struct({'h2': 'h2_jj_ddb', 'h2o': 'h2o_bk', 'h2s': 'h2s_ddb',
+                                      'nh3': 'nh3_hs_sjs', 'ph3': 'ph3_jh'}, 'e')
+        dev = K.device()
+        self.freqs = np.asarray(w['freqs'], dtype=np.float64)
+        self.nf = len(self.freqs)
+        self.nl = atm.nl
+        self.rays = w['rays']
+        self.nray = len(self.rays)
+        C = atm.C
+        self.P = K.as_device(atm.P)
+        self.T = K.as_device(atm.T)
+        self.mix = K.as_device(K.mix_table(atm.gas, C))
+        self.R = K.as_device(atm.R)
+        self.N = K.as_device(atm.N)
+        self.b = K.as_device(self.rays)
+        order, self.nf_low = K.nh3_order(self.freqs)
+        assert np.array_equal(order, np.arange(self.nf))      # ascending request
+        self.f = K.as_device(self.freqs)
+        self.planet = K.planet_desc(atm.Req, atm.Rpol)
+        self.alpha32 = torch.empty((self.nl, self.nf), dtype=torch.float32, device=dev)
+        self.alpha64 = torch.empty((self.nl, self.nf), dtype=torch.float64, device=dev)
+        self.ld = self.nl - 1
+        self.hds = torch.empty((self.nray, self.ld), dtype=torch.float64, device=dev)
+        self.ns = torch.empty((self.nray,), dtype=torch.int32, device=dev)
+        self.Tb = torch.empty((self.nray, self.nf), dtype=torch.float64, device=dev)
+        self.Tb64 = torch.empty((self.nray, self.nf), dtype=torch.float64, device=dev)
+        self.nlines = {k: self.cat.lines(k) for k in range(7)}
+
+    # the three launches of one step, on torch's current stream
+    def launch_alpha(self, prec):
+        import ctypes
+        from radiobear_amd import _lib
+        K = self.K
+        out = self.alpha32 if prec == _lib.FP32 else self.alpha64
+        _lib.check(_lib.lib().rbx_alpha(self.cat.handle, ctypes.byref(self.ab), K._ptr(self.P), K._ptr(self.T),
+                                        K._ptr(self.mix), self.nl, K._ptr(self.f), None, self.nf, self.nf_low,
+                                        None, prec, K._ptr(out), None, K._stream()))
+
+    def launch_paths(self):
+        import ctypes
+        from radiobear_amd import _lib
+        K = self.K
+        _lib.check(_lib.lib().rbx_raypaths(ctypes.byref(self.planet), K._ptr(self.R), K._ptr(self.N), self.nl,
+                                           K._ptr(self.b), self.nray, 1, K._ptr(self.hds), self.ld,
+                                           K._ptr(self.ns), K._stream()))
+
+    def launch_integrate(self, prec):
+        from radiobear_amd import _lib
+        K = self.K
+        a = self.alpha32 if prec == _lib.FP32 else self.alpha64
+        Tb = self.Tb if prec == _lib.FP32 else self.Tb64
+        tau_cut = 50.0 if prec == _lib.FP32 else 0.0
+        _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
+                                            self.ld, K._ptr(self.ns), self.nray, 0, prec, tau_cut, K._ptr(Tb), None,
+                                            K._stream()))
+
+    def launch_integrate_nocut(self, prec):
+        from radiobear_amd import _lib
+        K = self.K
+        a = self.alpha32 if prec == _lib.FP32 else self.alpha64
+        _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
+                                            self.ld, K._ptr(self.ns), self.nray, 0, prec, 0.0, K._ptr(self.Tb), None,
+                                            K._stream()))
+
+    def step(self, prec):
+        self.launch_alpha(prec)
+        self.launch_paths()
+        self.launch_integrate(prec)
+
+    def alpha_flop(self):
+        n = self.nlines
+        per_lf = (FLOP_BR * (n[0] + n[4] + n[5]) + FLOP_GROSS * (n[1] + n[2]) + FLOP_VVW * n[6]
+                  + 2 * CLOSED_FORM_FLOP)
+        return float(per_lf) * self.nl * self.nf
+
+
+def timed_graph(prob, prec, steps, warmup):
+    import torch
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        prob.step(prec)                       # eager warm-up (also the reference result)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        prob.step(prec)
+    for _ in range(warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, g
+
+
+def kernel_times(prob, prec, steps):
+    """Average duration of each launch, HIP events on the launch stream."""
+    import torch
+    names = ('alpha_kernel', 'raypath_kernel', 'integrate_kernel')
+    acc = {n: 0.0 for n in names}
+    for _ in range(steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        prob.launch_alpha(prec)
+        ev[1].record()
+        prob.launch_paths()
+        ev[2].record()
+        prob.launch_integrate(prec)
+        ev[3].record()
+        torch.cuda.synchronize()
+        for i, n in enumerate(names):
+            acc[n] += ev[i].elapsed_time(ev[i + 1])
+    return {n: acc[n] / steps for n in names}
+
+
+def cpu_baseline(prob, args):
+    """The FP64 oracle on a bounded sample, extrapolated to the whole workload."""
+    from oracle import geometry, layers, pipeline, rt
+    from radiobear_amd import lines
+    w = prob.w
+    atm = prob.atm
+    raw = lines.raw_catalog()
+    jl = pipeline.jupiter_lines(raw)
+    inj = dict(jl['nh3_hs'])
+    for k in ('fo', 'Io', 'Eo', 'gammaNH3o'):
+        inj[k] = w['nh3_inv'][k]
+    jl['nh3_hs'] = inj
+    if w['h2o'] is not None:
+        jl['h2o'] = w['h2o']
+    aset = layers.jupiter_absorbers(jl)
+    C = atm.C
+    sample_layers = [0, atm.nl // 2]
+    t0 = time.perf_counter()
+    for l in sample_layers:
+        aset.absorb(list(prob.freqs), atm.T[l], atm.P[l], atm.gas[:, l], C)
+    t_layer = (time.perf_counter() - t0) / len(sample_layers)
+    # geometry + integration on sample rays (on planet), with a cheap stand-in alpha table
+    rng = np.random.default_rng(0)
+    onp = [i for i in range(prob.nray) if np.hypot(*prob.rays[i]) < 0.99]
+    pick = rng.choice(onp, size=min(3, len(onp)), replace=False)
+    alpha_tab = np.full((prob.nf, atm.nl), 1e-7)
+    t0 = time.perf_counter()
+    for i in pick:
+        ray = geometry.compute_ds(atm.R, atm.N, prob.rays[i], atm.Req, atm.Rpol)
+        rt.single(alpha_tab, atm.T, ray.ds, ray.layer4ds)
+    t_ray = (time.perf_counter() - t0) / len(pick)
+    total = t_layer * atm.nl + t_ray * prob.on_planet
+    sample = ('{} layers of alpha at full lines x freqs ({:.2f} s/layer) and {} rays of geometry + '
+              'integration at full freqs x steps ({:.2f} s/ray), extrapolated to {} layers and {} '
+              'on-planet rays').format(len(sample_layers), t_layer, len(pick), t_ray, atm.nl, prob.on_planet)
+    return {'value': prob.on_planet * prob.nf / total, 'unit': 'ray*freq Tb/s', 'cores': 1, 'kind': 'port',
+            'sample': sample, 'seconds_extrapolated': total}
+
+
+def jacobian_rate(args):
+    """Workload 4: forward + adjoint Jacobian rows/s (dTb/dT and dTb/dq_NH3 per layer)."""
+    import torch
+    from radiobear_amd import kernels as K, lines, synth
+    w = synth.workload('c4_jacobian')
+    atm = w['atm']
+    cat = K.jupiter_catalog(lines.raw_catalog(), nh3_inversion=w['nh3_inv'])
+    ab = K.absorbers_struct({'h2': 'h2_jj_ddb', 'h2o': 'h2o_bk', 'h2s': 'h2s_ddb', 'nh3': 'nh3_hs_sjs',
+                             'ph3': 'ph3_jh'}, 'e')
+    P, T, mix = K.as_device(atm.P), K.as_device(atm.T), K.as_device(K.mix_table(atm.gas, atm.C))
+    pl = K.planet_desc(atm.Req, atm.Rpol)
+
+    def run():
+        tab = K.alpha(cat, ab, P, T, mix, w['freqs'], 'fp32')
+        dal = K.alpha_tangent(cat, ab, P, T, mix, w['freqs'])
+        hds, ns = K.raypaths(pl, atm.R, atm.N, w['rays'])
+        return K.jacobian(tab, dal, T, hds, ns, tau_cut=50.0)
+
+    run()
+    torch.cuda.synchronize()
+    n = 5
+    t0 = time.perf_counter()
+    for _ in range(n):
+        J, Tb = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / n
+    rows = len(w['rays']) * len(w['freqs'])
+    return {'workload': 'c4_jacobian', 'rows_per_s': rows / dt, 'ms_per_step': dt * 1e3, 'rows': rows,
+            'entries_per_row': 2 * atm.nl}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path '
+              '(every GPU box here has one GPU); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from radiobear_amd import _lib
+    if not torch.cuda.is_available():
+        print('no HIP device', file=sys.stderr)
+        sys.exit(1)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    log(args, 'building inputs for', args.workload)
+    prob = Problem(args.workload)
+    FP32, FP64 = _lib.FP32, _lib.FP64
+
+    dt32, _ = timed_graph(prob, FP32, args.steps, args.warmup)
+    log(args, 'fp32 graph step {:.3f} ms'.format(dt32 * 1e3))
+    ns = prob.ns.cpu().numpy()
+    prob.on_planet = int(np.sum(ns >= 0))
+    kt32 = kernel_times(prob, FP32, max(3, min(args.steps, 10)))
+    log(args, 'fp32 kernels', kt32)
+    res = {}
+    if not args.no_secondary:
+        dt64, _ = timed_graph(prob, FP64, max(3, args.steps // 4), 1)
+        kt64 = kernel_times(prob, FP64, 3)
+        log(args, 'fp64 graph step {:.3f} ms'.format(dt64 * 1e3), kt64)
+        Tb32 = prob.Tb.cpu().numpy()
+        Tb64 = prob.Tb64.cpu().numpy()
+        ok = ~np.isnan(Tb64)
+        a32 = prob.alpha32.double().cpu().numpy()
+        a64 = prob.alpha64.cpu().numpy()
+        res['fp64_plain'] = {'value': prob.on_planet * prob.nf / dt64, 'ms_per_step': dt64 * 1e3,
+                             'kernel_ms': kt64,
+                             'fp32_vs_fp64_max_abs_dTb_K': float(np.max(np.abs(Tb32[ok] - Tb64[ok]))),
+                             'fp32_vs_fp64_max_rel_dalpha': float(np.max(np.abs(a32 - a64) / np.abs(a64))),
+                             'nan_rays': int(np.sum(np.isnan(Tb64).any(axis=1)))}
+        # the tau cut (e^-50) changes Tb below FP64 resolution: show it costs nothing in accuracy
+        prob.launch_integrate(FP32)
+        cut = prob.Tb.clone()
+        prob.launch_integrate_nocut(FP32)
+        torch.cuda.synchronize()
+        d = (cut - prob.Tb).abs()
+        res['tau_cut_max_abs_dTb_K'] = float(torch.nan_to_num(d, nan=0.0).max())
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(3):
+            prob.launch_integrate_nocut(FP32)
+        ev[1].record()
+        torch.cuda.synchronize()
+        res['integrate_fp32_without_tau_cut_ms'] = ev[0].elapsed_time(ev[1]) / 3
+        res['speedup_fp32_vs_fp64'] = dt64 / dt32
+        res['jacobian'] = jacobian_rate(args)
+        log(args, 'jacobian', res['jacobian'])
+
+    # roofline of the dominant kernel (by duration)
+    dom = max(kt32, key=kt32.get)
+    if dom == 'alpha_kernel':
+        flop = prob.alpha_flop()
+        achieved = flop / (kt32[dom] * 1e-3) / 1e12
+        roof = {'kernel': dom, 'bound': 'valu', 'achieved': achieved, 'peak': PEAK['fp32'], 'unit': 'TFLOP/s',
+                'frac': achieved / PEAK['fp32'], 'traffic': None,
+                'algorithmic_flop_per_launch': flop}
+    else:
+        steps = float(np.sum(np.maximum(ns - 1, 0)))
+        flop = steps * prob.nf * FLOP_RT_STEP
+        achieved = flop / (kt32[dom] * 1e-3) / 1e12
+        roof = {'kernel': dom, 'bound': 'valu', 'achieved': achieved, 'peak': PEAK['fp64'], 'unit': 'TFLOP/s',
+                'frac': achieved / PEAK['fp64'], 'traffic': None, 'algorithmic_flop_per_launch': flop}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(args, 'cpu baseline (oracle sample)...')
+        cpu = cpu_baseline(prob, args)
+    value = prob.on_planet * prob.nf / dt32
+    line = {
+        'metric': 'ray*freq Tb/s (FP32 batched forward; Jacobian rows/s and FP64 plain alongside)',
+        'value': value, 'unit': 'ray*freq Tb/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': dt32 * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic (seeded generator)',
+        'config': {'workload': args.workload, 'layers': prob.nl, 'freqs': prob.nf, 'rays': prob.nray,
+                   'on_planet_rays': prob.on_planet, 'lines': {k: v for k, v in prob.nlines.items()},
+                   'graph': 'hipGraph replay of alpha + raypaths + integrate', 'parallelism': 'single GPU',
+                   'precision': 'FP32 line sums (FP64 outer accumulation), FP64 detuning, geometry and tau; '
+                                'tau cut at 50'},
+        'kernel_ms': kt32, 'roofline': roof, 'cpu_baseline': cpu,
+    }
+    line.update(res)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -84,6 +84,10 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RbxLibraryError('librbx.so not built at {} (run python -m radiobear_amd.build)'.format(LIB_PATH))
+    # torch ships its own HIP runtime with the same soname (libamdhip64.so.7);
+    # loading torch first makes librbx bind to that one runtime, so device
+    # pointers and streams from torch are valid in the library.
+    import torch  # noqa: F401
     try:
         handle = ctypes.CDLL(LIB_PATH)
     except OSError as e:
