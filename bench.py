@@ -328,6 +328,13 @@ def main():
         achieved = flop / (kt32[dom] * 1e-3) / 1e12
         roof = {'kernel': dom, 'bound': 'valu', 'achieved': achieved, 'peak': PEAK['fp64'], 'unit': 'TFLOP/s',
                 'frac': achieved / PEAK['fp64'], 'traffic': None, 'algorithmic_flop_per_launch': flop}
+    tr_path = os.path.join(REPO, 'profiles', '{}_fp32_traffic.json'.format(args.workload))
+    if os.path.exists(tr_path):
+        with open(tr_path) as fp:
+            tr = json.load(fp)['bytes_per_launch'].get(dom)
+        if tr:
+            roof['traffic'] = tr['read'] + tr['write']
+            roof['traffic_source'] = 'rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)'
     cpu = None
     if not args.no_cpu_baseline:
         log(args, 'cpu baseline (oracle sample)...')
