@@ -29,6 +29,8 @@
  *                       (radiobear/raypath.py:39-273, radiobear/shape.py:223-298)
  *   rbx_integrate       Brightness.single's tau / W / Tb loop and final spectrum
  *                       (radiobear/brightness.py:30-126)
+ *   rbx_integrate_profiles  the .tau / .W / .Tb_lyr attributes Brightness.single keeps
+ *                       (radiobear/brightness.py:53-120)
  *   rbx_jacobian        (no reference counterpart: dTb/dT and dTb/dq per layer)
  */
 #ifndef RBX_H
@@ -175,6 +177,14 @@ int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int n
                   const double *hds, int ld, const int *nsteps, int nray,
                   int mode, int precision, double tau_cut, double *Tb, double *intW,
                   void *stream);
+
+/* ---- per-point profiles (Brightness.single's .tau, .W, .Tb_lyr, brightness.py:53-120) ----
+ * tau, W, Tb_lyr  [nray][nl][nf] float64 at path points 0..nsteps-1 (0 beyond);
+ * same arguments and arithmetic as rbx_integrate, no tau cut.
+ */
+int rbx_integrate_profiles(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                           const double *hds, int ld, const int *nsteps, int nray, int mode,
+                           int precision, double *tau, double *W, double *Tb_lyr, void *stream);
 
 /* ---- Jacobian rows ----
  * dalpha          [3][nl][nf] from rbx_alpha_tangent

@@ -28,7 +28,7 @@ SPECIES = ('H2', 'HE', 'CH4', 'NH3', 'H2O', 'H2S', 'PH3')  # RBX_X_* rows of mix
 
 EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
-           'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_jacobian')
+           'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles', 'rbx_jacobian')
 
 
 class RbxLibraryError(RuntimeError):
@@ -71,6 +71,7 @@ _SIGS = {
                                _VP, _VP, _VP]),
     'rbx_raypaths': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _I, _VP, _I, _VP, _VP]),
     'rbx_integrate': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _VP]),
+    'rbx_integrate_profiles': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
     'rbx_jacobian': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP]),
 }
 

@@ -1,0 +1,205 @@
+"""Alpha: per-layer absorption on the GPU behind the reference's interface.
+
+Mirrors radiobear/alpha.py:13-305.  ``get_layers`` computes the whole
+(layer x frequency) table in one rbx_alpha launch instead of a Python loop
+over layers and constituent modules; the result stays on the device
+(``self.table``, [nl][nf]) for the ray integration, and ``self.layers`` gives
+the reference's (nf, nl) numpy view on demand.  Scales (float, per-layer list
+or per-constituent dict of lists, alpha.py:235-259), and saving / reading the
+per-constituent absorption to memory or file (alpha.py:110-149) behave as in
+the reference.
+"""
+import os
+from argparse import Namespace
+
+import numpy as np
+import torch
+
+from . import _lib, kernels, lines, utils
+
+
+class Alpha:
+    """Absorption of all layers of an atmosphere at a set of frequencies (GHz)."""
+
+    def __init__(self, idnum=0, config=None, log=None, load_formal=True, verbose=True, precision='fp64',
+                 **kwargs):
+        from . import config as pcfg
+        if config is None or isinstance(config, str):
+            config = pcfg.planetConfig('jupiter', configFile=config)
+            config.update_config(**kwargs)
+        self.config = config
+        self.verbose = verbose
+        self.idnum = idnum
+        self.precision = precision
+        self.alphafile = os.path.join(self.config.scratch_directory, 'alpha{:04d}.npz'.format(idnum))
+        self.saved_fields = ['ordered_constituents', 'alpha_data', 'freqs', 'P']
+        self.memory = Namespace()
+        self.catalog = None
+        self.line_overrides = {}
+        self.reset_layers()
+        if load_formal:
+            self.setup_formalisms()
+
+    # ------------------------------------------------------------ formalisms
+    def setup_formalisms(self):
+        """Absorber selection and truncation from the config (alpha.py:49-102)."""
+        self.constituent = {}
+        self.truncate_strength = {}
+        self.truncate_freq = {}
+        for c, absorber in self.config.constituent_alpha.items():
+            if absorber is None:
+                continue
+            kernels.absorbers_struct({c: absorber}, self.config.h2state if c == 'h2' else 'e')
+            self.constituent[c] = absorber
+            self.truncate_freq[c] = None
+            self.truncate_strength[c] = None
+            meth = self.config.truncate_method.get(c) if isinstance(self.config.truncate_method, dict) else None
+            if meth is not None:
+                for tm in meth.split(','):
+                    getattr(self, tm)[c] = getattr(self.config, tm)[c]
+        self.ordered_constituents = sorted(self.constituent)
+        self.other_dict = {c: {} for c in self.ordered_constituents}
+        if 'h2' in self.other_dict:
+            self.other_dict['h2'] = {'h2state': self.config.h2state, 'h2newset': self.config.h2newset}
+        self.absorbers = kernels.absorbers_struct(self.constituent, self.config.h2state)
+        self.catalog = None
+
+    def set_lines(self, nh3_inversion=None, h2o=None):
+        """Replace the NH3 inversion block and/or the H2O list (synthetic workloads)."""
+        if nh3_inversion is not None:
+            self.line_overrides['nh3_inversion'] = nh3_inversion
+        if h2o is not None:
+            self.line_overrides['h2o'] = h2o
+        self.catalog = None
+
+    def _catalog(self):
+        if self.catalog is None:
+            self.catalog = kernels.jupiter_catalog(lines.raw_catalog(), self.truncate_strength, self.truncate_freq,
+                                                   **self.line_overrides)
+        return self.catalog
+
+    def reset_layers(self):
+        self.P = None
+        self.freqs = None
+        self.table = None
+        self._layers_host = None
+
+    @property
+    def layers(self):
+        """(nf, nl) float64 numpy array as the reference's self.layers (alpha.py:300)."""
+        if self.table is None:
+            return None
+        if self._layers_host is None:
+            self._layers_host = self.table.double().cpu().numpy().T
+        return self._layers_host
+
+    # ---------------------------------------------------------------- scales
+    def get_layer_scale(self, scale, N):
+        """Per-layer scale list (alpha.py:235-259)."""
+        layer_scale = []
+        if isinstance(scale, dict):
+            for k, v in scale.items():
+                if k not in self.ordered_constituents:
+                    raise ValueError("{} not found as constituent for alpha".format(k))
+                if len(v) != N:
+                    raise ValueError("Incorrect scale for {}:  N {} vs {}".format(k, len(v), N))
+            for lyr in range(N):
+                layer_scale.append({k: v[lyr] for k, v in scale.items()})
+        elif isinstance(scale, (list, np.ndarray)):
+            if len(scale) != N:
+                raise ValueError("Incorrect number of scale layers.")
+            layer_scale = scale
+        else:
+            val = float(scale) if utils.isanynum(scale) else 1.0
+            layer_scale = [val] * N
+        return layer_scale
+
+    def scale_table(self, scale, N):
+        """(5, nl) per-constituent scale factors in RBX_C_* order for rbx_alpha."""
+        ls = self.get_layer_scale(scale, N)
+        out = np.ones((len(_lib.CONSTITUENTS), N))
+        for lyr, s in enumerate(ls):
+            if isinstance(s, dict):
+                for k, v in s.items():
+                    out[_lib.CONSTITUENTS.index(k), lyr] = v
+            else:
+                out[:, lyr] = float(s)
+        return out
+
+    # ------------------------------------------------------------ save / read
+    def save_alpha_data(self, save_type):
+        if save_type == 'file':
+            np.savez(self.alphafile, ordered_constituents=self.ordered_constituents, alpha_data=self.tosave,
+                     freqs=self.freqs, P=self.P)
+        elif save_type == 'memory':
+            self.memory.ordered_constituents = self.ordered_constituents
+            self.memory.alpha_data = self.tosave
+            self.memory.freqs = self.freqs
+            self.memory.P = self.P
+
+    def read_alpha_data(self, save_type):
+        if save_type == 'file':
+            with np.load(self.alphafile, allow_pickle=False) as d:
+                for sf in self.saved_fields:
+                    setattr(self, sf, d[sf])
+            self.ordered_constituents = [str(x) for x in self.ordered_constituents]
+        elif save_type == 'memory':
+            for sf in self.saved_fields:
+                setattr(self, sf, getattr(self.memory, sf))
+
+    # ----------------------------------------------------------------- main
+    def get_layers(self, freqs, atm, scale=False, get_alpha='calc', save_alpha='none'):
+        """Absorption of every layer (alpha.py:261-305); table on the device."""
+        self.reset_layers()
+        self.freqs = list(freqs)
+        C = atm.config.C
+        self.P = atm.gas[C['P']]
+        nl = atm.gas.shape[1]
+        get_memfil = get_alpha in ('memory', 'file')
+        save_memfil = save_alpha in ('memory', 'file')
+        sc = self.scale_table(scale, nl)
+        if get_memfil:
+            self.read_alpha_data(get_alpha)
+            data = np.asarray(self.alpha_data, dtype=np.float64)          # (nl, nf, nc)
+            cols = [_lib.CONSTITUENTS.index(c) for c in self.ordered_constituents]
+            s = sc[cols].T[:, None, :]                                     # (nl, 1, nc)
+            dev = kernels.device()
+            tot = (torch.as_tensor(data, device=dev) * torch.as_tensor(s, device=dev)).sum(dim=2)
+            self.table = tot.to(torch.float32 if self.precision == 'fp32' else torch.float64).contiguous()
+            if save_memfil:
+                self.tosave = data * s
+                self.save_alpha_data(save_alpha)
+            return
+        P, T, mix = atm.gas[C['P']], atm.gas[C['T']], kernels.mix_table(atm.gas, C)
+        res = kernels.alpha(self._catalog(), self.absorbers, P, T, mix, self.freqs, self.precision,
+                            scale=sc, by_constituent=save_memfil)
+        if save_memfil:
+            self.table, byc = res
+            cols = [_lib.CONSTITUENTS.index(c) for c in self.ordered_constituents]
+            absorb = byc.cpu().numpy()[cols]                               # (nc, nl, nf), unscaled
+            self.tosave = np.transpose(absorb, (1, 2, 0)) * sc[cols].T[:, None, :]
+            self.save_alpha_data(save_alpha)
+        else:
+            self.table = res
+
+    def get_single_layer(self, freqs, layer, atm, lscale, units='invcm'):
+        """Total absorption of one layer (alpha.py:218-233), as a numpy array over freqs."""
+        one = _OneLayer(atm, layer)
+        saved = (self.table, self._layers_host, self.freqs, self.P)
+        try:
+            scale = lscale if not isinstance(lscale, dict) else {k: [v] for k, v in lscale.items()}
+            self.get_layers(freqs, one, scale=scale)
+            out = self.layers[:, 0].copy()
+        finally:
+            self.table, self._layers_host, self.freqs, self.P = saved
+        if units == 'dBperkm':
+            out *= 434294.5
+        return out
+
+
+class _OneLayer:
+    """A one-layer view of an atmosphere (for single-layer calls)."""
+
+    def __init__(self, atm, layer):
+        self.config = atm.config
+        self.gas = np.ascontiguousarray(atm.gas[:, layer:layer + 1])

@@ -1,0 +1,21 @@
+"""h2_jj_ddb plug-in: constituents/h2/h2_jj_ddb.py:7-38 (H2-H2/He/CH4 collision-induced absorption).
+
+alpha(freq, T, P, X, P_dict, other_dict, **kwargs) -> len(freq) values
+(dB/km, or cm^-1 with units='invcm'), computed by the rbx_alpha kernel.
+"""
+import numpy as np
+
+from . import _Module
+
+_module = _Module('h2', 'h2_jj_ddb', ('H2', 'HE', 'CH4'))
+data = None   # assign a line list before the first call to replace the shipped one
+
+
+def alpha(freq, T, P, X, P_dict, other_dict, **kwargs):
+    if data is not None and _module.data is not data:
+        _module.data = data
+        _module.reset()
+    a = _module(freq, T, P, X, P_dict, other_dict, **kwargs)
+    if np.isscalar(a):
+        return a
+    return list(np.ravel(a))

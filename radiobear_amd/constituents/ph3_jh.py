@@ -1,0 +1,21 @@
+"""ph3_jh plug-in: constituents/ph3/ph3_jh.py:64-108 (Ben-Reuven with per-line weights).
+
+alpha(freq, T, P, X, P_dict, other_dict, **kwargs) -> len(freq) values
+(dB/km, or cm^-1 with units='invcm'), computed by the rbx_alpha kernel.
+"""
+import numpy as np
+
+from . import _Module
+
+_module = _Module('ph3', 'ph3_jh', ('PH3', 'H2', 'HE'))
+data = None   # assign a line list before the first call to replace the shipped one
+
+
+def alpha(freq, T, P, X, P_dict, other_dict, **kwargs):
+    if data is not None and _module.data is not data:
+        _module.data = data
+        _module.reset()
+    a = _module(freq, T, P, X, P_dict, other_dict, **kwargs)
+    if np.isscalar(a):
+        return a
+    return np.asarray(a)

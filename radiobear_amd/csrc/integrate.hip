@@ -180,6 +180,62 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
     }
 }
 
+// ---- per-point profiles of one ray set: tau, W and Tb_lyr at every path point ----
+// (Brightness.single keeps these as .tau / .W / .Tb_lyr, brightness.py:55-57, 94-104, 118-120)
+struct ProfArgs {
+    IntArgs I;
+    double *tau, *W, *Tbl;   // [nray][nl][nf]
+};
+
+__global__ __launch_bounds__(kWaves * kLanes) void profile_kernel(ProfArgs Ap) {
+    const IntArgs &A = Ap.I;
+    const int wave = threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
+    const int group = blockIdx.x % A.ngroups;
+    const int ray = __builtin_amdgcn_readfirstlane((blockIdx.x / A.ngroups) * kWaves + wave);
+    if (ray >= A.nray) return;
+    const int j = group * kLanes + lane;
+    if (j >= A.nf) return;
+    const int S = min(A.nsteps[ray], A.nl - 1);
+    const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
+    const size_t base = static_cast<size_t>(ray) * A.nl * A.nf + j;
+    double tau = 0.0, W0 = 0.0, TW0 = 0.0, num = 0.0;
+    int k = 0;
+    if (S >= 1) {
+        Ap.tau[base] = 0.0;
+        Ap.W[base] = 0.0;
+        Ap.Tbl[base] = 0.0;
+        k = 1;
+        double a0 = load_alpha(A.alpha, A.f32, j);
+        for (int i = 0; i < S - 1; ++i, ++k) {
+            const double hh = h[i];
+            const double a1 = load_alpha(A.alpha, A.f32, static_cast<size_t>(i + 1) * A.nf + j);
+            tau = tau + (a0 + a1) * hh;
+            double W1;
+            if (A.mode == 1)
+                W1 = 2.0 * a1 * expint_e2(tau);
+            else
+                W1 = a1 * (A.fast_exp ? exp_neg_mixed(tau) : exp(-tau));
+            const double TW1 = A.T[i + 1] * W1;
+            num = num + (TW1 + TW0) * hh;
+            const size_t o = base + static_cast<size_t>(i + 1) * A.nf;
+            Ap.tau[o] = tau;
+            Ap.W[o] = W1;
+            Ap.Tbl[o] = num;
+            a0 = a1;
+            W0 = W1;
+            TW0 = TW1;
+        }
+    }
+    for (; k < A.nl; ++k) {   // points the ray does not reach
+        const size_t o = base + static_cast<size_t>(k) * A.nf;
+        Ap.tau[o] = 0.0;
+        Ap.W[o] = 0.0;
+        Ap.Tbl[o] = 0.0;
+    }
+    (void)W0;
+}
+
 static int check_int_args(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
                           const double *hds, int ld, const int *nsteps, int nray, int precision) {
     RBX_CHECK_ARG(alpha && T && hds && nsteps, "integration: null device pointer");
@@ -248,4 +304,25 @@ extern "C" int rbx_jacobian(const void *alpha, int alpha_precision, const double
     hipLaunchKernelGGL(rbx::jacobian_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes),
                        0, static_cast<hipStream_t>(stream), Aj);
     return rbx::launch_status("jacobian_kernel");
+}
+
+extern "C" int rbx_integrate_profiles(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                                      const double *hds, int ld, const int *nsteps, int nray, int mode,
+                                      int precision, double *tau, double *W, double *Tb_lyr, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    RBX_CHECK_ARG(tau && W && Tb_lyr, "rbx_integrate_profiles: null output");
+    RBX_CHECK_ARG(mode == 0 || mode == 1, "rbx_integrate_profiles: bad mode %d", mode);
+    if (nray == 0) return RBX_OK;
+    rbx::ProfArgs Ap;
+    rbx::fill_int_args(Ap.I, alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision, 0.0);
+    Ap.I.mode = mode;
+    Ap.tau = tau;
+    Ap.W = W;
+    Ap.Tbl = Tb_lyr;
+    const long long blocks = static_cast<long long>(Ap.I.ngroups) * ((nray + rbx::kWaves - 1) / rbx::kWaves);
+    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate_profiles: too many blocks");
+    hipLaunchKernelGGL(rbx::profile_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes), 0,
+                       static_cast<hipStream_t>(stream), Ap);
+    return rbx::launch_status("profile_kernel");
 }

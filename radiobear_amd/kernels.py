@@ -220,6 +220,21 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
     return (Tb, iw) if with_intW else Tb
 
 
+def integrate_profiles(alpha_table, T, hds, nsteps, disc=False, precision=None):
+    """tau, W, Tb_lyr at every path point: three (nray, nl, nf) float64 tensors."""
+    aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
+    prec = aprec if precision is None else _lib.PRECISIONS[precision]
+    a = alpha_table.contiguous()
+    T = as_device(T)
+    nl, nf = a.shape
+    nray, ld = hds.shape
+    out = [torch.empty((nray, nl, nf), dtype=torch.float64, device=device()) for _ in range(3)]
+    check(lib().rbx_integrate_profiles(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds.contiguous()), ld,
+                                       _ptr(nsteps.contiguous()), nray, 1 if disc else 0, prec,
+                                       *[_ptr(t) for t in out], _stream()))
+    return tuple(out)
+
+
 def jacobian(alpha_table, dalpha, T, hds, nsteps, tau_cut=0.0):
     """J (nray, 2, nl, nf) float32: dTb/dT and dTb/dq_NH3 per layer; and Tb (nray, nf)."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
