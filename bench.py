@@ -37,6 +37,7 @@ def parse():
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-secondary', action='store_true')
     p.add_argument('--quiet', action='store_true')
+    p.add_argument('--no-tune', action='store_true', help='use the built-in launch shapes (no autotuning)')
     p.add_argument('--copy-calibration', action='store_true',
                    help='run only a plain device copy of 1 GiB (calibrates FETCH_SIZE / WRITE_SIZE)')
     return p.parse_args()
@@ -146,6 +147,23 @@ class Problem:
         per_lf = (FLOP_BR * (n[0] + n[4] + n[5]) + FLOP_GROSS * (n[1] + n[2]) + FLOP_VVW * n[6]
                   + 2 * CLOSED_FORM_FLOP)
         return float(per_lf) * self.nl * self.nf
+
+
+def autotune(prob, prec, args):
+    """Pick the launch shapes of rbx_alpha / rbx_integrate for this shape (outside the timed region)."""
+    import torch
+    from radiobear_amd import autotune as at
+    if args.no_tune:
+        at.reset()
+        return None
+    key = '{}:{}x{}x{}:p{}'.format(args.workload, prob.nl, prob.nf, prob.nray, prec)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        prob.launch_paths()
+        ka = at.tune('alpha_k', key, lambda: prob.launch_alpha(prec))
+        ki = at.tune('int_unroll', key, lambda: prob.launch_integrate(prec))
+    torch.cuda.synchronize()
+    return {'alpha_k': ka['best'], 'int_unroll': ki['best'], 'alpha_ms': ka['ms'], 'integrate_ms': ki['ms']}
 
 
 def timed_graph(prob, prec, steps, warmup):
@@ -275,6 +293,8 @@ def main():
     prob = Problem(args.workload)
     FP32, FP64 = _lib.FP32, _lib.FP64
 
+    tune32 = autotune(prob, FP32, args)
+    log(args, 'fp32 launch shapes', tune32)
     dt32, _ = timed_graph(prob, FP32, args.steps, args.warmup)
     log(args, 'fp32 graph step {:.3f} ms'.format(dt32 * 1e3))
     ns = prob.ns.cpu().numpy()
@@ -283,6 +303,7 @@ def main():
     log(args, 'fp32 kernels', kt32)
     res = {}
     if not args.no_secondary:
+        tune64 = autotune(prob, FP64, args)
         dt64, _ = timed_graph(prob, FP64, max(3, args.steps // 4), 1)
         kt64 = kernel_times(prob, FP64, 3)
         log(args, 'fp64 graph step {:.3f} ms'.format(dt64 * 1e3), kt64)
@@ -292,11 +313,12 @@ def main():
         a32 = prob.alpha32.double().cpu().numpy()
         a64 = prob.alpha64.cpu().numpy()
         res['fp64_plain'] = {'value': prob.on_planet * prob.nf / dt64, 'ms_per_step': dt64 * 1e3,
-                             'kernel_ms': kt64,
+                             'kernel_ms': kt64, 'launch_shapes': tune64,
                              'fp32_vs_fp64_max_abs_dTb_K': float(np.max(np.abs(Tb32[ok] - Tb64[ok]))),
                              'fp32_vs_fp64_max_rel_dalpha': float(np.max(np.abs(a32 - a64) / np.abs(a64))),
                              'nan_rays': int(np.sum(np.isnan(Tb64).any(axis=1)))}
         # the tau cut (e^-50) changes Tb below FP64 resolution: show it costs nothing in accuracy
+        autotune(prob, FP32, args)
         prob.launch_integrate(FP32)
         cut = prob.Tb.clone()
         prob.launch_integrate_nocut(FP32)
@@ -350,7 +372,7 @@ def main():
                    'graph': 'hipGraph replay of alpha + raypaths + integrate', 'parallelism': 'single GPU',
                    'precision': 'FP32 line sums (FP64 outer accumulation), FP64 detuning, geometry and tau; '
                                 'tau cut at 50'},
-        'kernel_ms': kt32, 'roofline': roof, 'cpu_baseline': cpu,
+        'kernel_ms': kt32, 'launch_shapes': tune32, 'roofline': roof, 'cpu_baseline': cpu,
     }
     line.update(res)
     print(json.dumps(line), flush=True)

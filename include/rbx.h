@@ -114,6 +114,13 @@ int rbx_last_error(char *buf, size_t len);
 /* Number of HIP devices visible to the library (host call, no GPU work). */
 int rbx_device_count(int *count);
 
+/* Launch-parameter overrides (no reference counterpart; used by the autotuner):
+ * applies to every later call in the process; value 0 restores the built-in choice. */
+#define RBX_TUNE_ALPHA_K    0   /* frequencies per lane of rbx_alpha: 1, 2 or 4 */
+#define RBX_TUNE_INT_UNROLL 1   /* path steps per load batch of rbx_integrate: 4 or 8 */
+#define RBX_NUM_TUNE_KEYS   2
+int rbx_tune(int key, int value);
+
 /* ---- line catalogue (device-resident, owned by the library) ---- */
 int rbx_catalog_create(rbx_catalog **out);
 /* Replace block `kind` with n lines; host_columns[i] is a HOST array of n

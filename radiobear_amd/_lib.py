@@ -26,7 +26,10 @@ LINE_COLUMNS = {
 CONSTITUENTS = ('h2', 'h2o', 'h2s', 'nh3', 'ph3')          # RBX_C_* order (sorted names)
 SPECIES = ('H2', 'HE', 'CH4', 'NH3', 'H2O', 'H2S', 'PH3')  # RBX_X_* rows of mix
 
-EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_catalog_create',
+TUNE_ALPHA_K = 0
+TUNE_INT_UNROLL = 1
+
+EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_tune', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
            'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles', 'rbx_jacobian')
 
@@ -61,6 +64,7 @@ _SIGS = {
     'rbx_abi_version': (_I, []),
     'rbx_last_error': (_I, [ctypes.c_char_p, ctypes.c_size_t]),
     'rbx_device_count': (_I, [ctypes.POINTER(_I)]),
+    'rbx_tune': (_I, [_I, _I]),
     'rbx_catalog_create': (_I, [ctypes.POINTER(_VP)]),
     'rbx_catalog_set_lines': (_I, [_VP, _I, _I, ctypes.POINTER(ctypes.POINTER(_D))]),
     'rbx_catalog_lines': (_I, [_VP, _I, ctypes.POINTER(_I)]),

@@ -45,7 +45,7 @@
 namespace rbx {
 
 constexpr int kThreads = 256;
-constexpr int kSub = 16;   // FP32 partial-sum length
+constexpr int kSub = 8;    // FP32 partial-sum length (lines unrolled per step)
 
 // physical constants as the plug-ins spell them
 constexpr double kHsHcKb = 19.858252418E-24 / 1.38E-23;    // nh3_hs.py:75-76
@@ -86,8 +86,9 @@ struct AlphaArgs {
     const double *scale;
     void *alpha;       // forward: alpha [nl][nf]; tangent: dalpha [3][nl][nf] (double)
     double *by_c;
-    int tiles_low;
-    int tile_width;
+    int tiles_low;     // tiles covering the columns < nf_low
+    int tiles_total;   // forward kernel: tiles of both regimes
+    int tile_width;    // columns per tile
 };
 
 // ---- per-layer coefficient sets: each derives (D, a, c, g, G) of one line ----
@@ -224,13 +225,13 @@ struct H2oPrep {   // h2o_bk lines, VVW with base (h2o_bk.py:83-99, 130-189)
     }
 };
 
-// ---- LDS staging ----
+// ---- LDS staging: up to two coefficient sets per line (the nh3_hs <= / > 30 GHz sets) ----
 struct Staged64 {
-    double D[kThreads], a[kThreads], c[kThreads], g[kThreads], G[kThreads];
+    double D[2][kThreads], a[2][kThreads], c[2][kThreads], g[2][kThreads], G[2][kThreads];
 };
 struct Staged32 {
-    double D[kThreads];
-    float4 q[kThreads];  // a, c, g, G
+    double D[2][kThreads];
+    float4 q[2][kThreads];  // a, c, g, G
 };
 struct StagedTan {       // value and two tangent slots (d/dT, d/dq_own) of D, a, c, g, G
     double v[5][kThreads];
@@ -254,61 +255,88 @@ union SharedTan {
     double red[kNd + 1][kThreads];
 };
 
-// ---- forward line sum over one block for K frequencies ----
+// ---- forward line sum over one block for K frequencies per lane ----
+// prep[0..1]: coefficient sets (two only for the nh3_hs inversion lines, whose
+// <= / > 30 GHz constants differ); `need` is the block's bitmask of sets to
+// stage, `mine` the set this wavefront sums (-1: the wavefront has no columns).
+// Staged lines are padded to a multiple of kSub with zero terms so the sum
+// loop has a fixed trip count and unrolls.
 template <int K, bool F32, class Prep>
-__device__ void line_sum(const DevBlock &B, const Prep &prep, double scale, const double (&x)[K],
-                         double (&acc)[K], SharedFwd &sh, double &base) {
+__device__ void line_sum(const DevBlock &B, const Prep *prep, int need, int mine, const double *scale,
+                         const double (&x)[K], double (&acc)[K], SharedFwd &sh, double &base) {
     float xf[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) xf[k] = static_cast<float>(x[k]);
     for (int l0 = 0; l0 < B.n; l0 += kThreads) {
         const int i = l0 + threadIdx.x;
-        if (i < B.n) {
-            const CoefT<double> q = prep(B, i, base);
-            if constexpr (F32) {
-                sh.s32.D[threadIdx.x] = q.D;
-                sh.s32.q[threadIdx.x] = make_float4(static_cast<float>(q.a * scale), static_cast<float>(q.c * scale),
-                                                    static_cast<float>(q.g), static_cast<float>(q.G));
-            } else {
-                sh.s64.D[threadIdx.x] = q.D;
-                sh.s64.a[threadIdx.x] = q.a;
-                sh.s64.c[threadIdx.x] = q.c;
-                sh.s64.g[threadIdx.x] = q.g;
-                sh.s64.G[threadIdx.x] = q.G;
+        const int cnt = min(kThreads, B.n - l0);
+        const int cntp = (cnt + kSub - 1) / kSub * kSub;
+        if (static_cast<int>(threadIdx.x) < cntp) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!(need & (1 << s))) continue;
+                CoefT<double> q = {0.0, 0.0, 0.0, 0.0, 1.0};   // padding: term = 0 / ((x)^2 + 1)
+                if (i < B.n) q = prep[s](B, i, base);
+                if constexpr (F32) {
+                    sh.s32.D[s][threadIdx.x] = q.D;
+                    sh.s32.q[s][threadIdx.x] =
+                        make_float4(static_cast<float>(q.a * scale[s]), static_cast<float>(q.c * scale[s]),
+                                    static_cast<float>(q.g), static_cast<float>(q.G));
+                } else {
+                    sh.s64.D[s][threadIdx.x] = q.D;
+                    sh.s64.a[s][threadIdx.x] = q.a;
+                    sh.s64.c[s][threadIdx.x] = q.c;
+                    sh.s64.g[s][threadIdx.x] = q.g;
+                    sh.s64.G[s][threadIdx.x] = q.G;
+                }
             }
         }
         __syncthreads();
-        const int cnt = min(kThreads, B.n - l0);
-        if constexpr (F32) {
-            for (int j0 = 0; j0 < cnt; j0 += kSub) {
-                const int j1 = min(j0 + kSub, cnt);
-                float p[K];
+        if (mine >= 0) {
+            if constexpr (F32) {
+                const double *sD = sh.s32.D[mine];
+                const float4 *sq = sh.s32.q[mine];
+                for (int j0 = 0; j0 < cntp; j0 += kSub) {
+                    float p[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) p[k] = 0.0f;
-                for (int j = j0; j < j1; ++j) {
-                    const double D = sh.s32.D[j];
-                    const float4 q = sh.s32.q[j];
+                    for (int k = 0; k < K; ++k) p[k] = 0.0f;
+                    // two lines over one denominator: n1/d1 + n2/d2 = (n1 d2 + n2 d1)/(d1 d2),
+                    // one reciprocal per pair (denominators stay within 1e-17..1e27 here)
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        float d = static_cast<float>(x[k] - D);
-                        float num = fmaf(q.x, xf[k], q.y);
-                        float den = fmaf(d, d, fmaf(q.z, xf[k], q.w));
-                        p[k] = fmaf(num, __builtin_amdgcn_rcpf(den), p[k]);
+                    for (int u = 0; u < kSub; u += 2) {
+                        const double D1 = sD[j0 + u], D2 = sD[j0 + u + 1];
+                        const float4 q1 = sq[j0 + u], q2 = sq[j0 + u + 1];
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            const float d1 = static_cast<float>(x[k] - D1);
+                            const float d2 = static_cast<float>(x[k] - D2);
+                            const float n1 = fmaf(q1.x, xf[k], q1.y);
+                            const float n2 = fmaf(q2.x, xf[k], q2.y);
+                            const float e1 = fmaf(d1, d1, fmaf(q1.z, xf[k], q1.w));
+                            const float e2 = fmaf(d2, d2, fmaf(q2.z, xf[k], q2.w));
+                            const float num = fmaf(n2, e1, n1 * e2);
+                            p[k] = fmaf(num, __builtin_amdgcn_rcpf(e1 * e2), p[k]);
+                        }
                     }
+#pragma unroll
+                    for (int k = 0; k < K; ++k) acc[k] += static_cast<double>(p[k]);
                 }
+            } else {
+                const double *sD = sh.s64.D[mine], *sa = sh.s64.a[mine], *sc = sh.s64.c[mine];
+                const double *sg = sh.s64.g[mine], *sG = sh.s64.G[mine];
+                for (int j0 = 0; j0 < cntp; j0 += 4) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += static_cast<double>(p[k]);
-            }
-        } else {
-            for (int j = 0; j < cnt; ++j) {
-                const double D = sh.s64.D[j], a = sh.s64.a[j], c = sh.s64.c[j];
-                const double g = sh.s64.g[j], G = sh.s64.G[j];
+                    for (int u = 0; u < 4; ++u) {
+                        const int j = j0 + u;
+                        const double D = sD[j], a = sa[j], c = sc[j], g = sg[j], G = sG[j];
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    double d = x[k] - D;
-                    double num = fma(a, x[k], c);
-                    double den = fma(d, d, fma(g, x[k], G));
-                    acc[k] += num / den;
+                        for (int k = 0; k < K; ++k) {
+                            double d = x[k] - D;
+                            double num = fma(a, x[k], c);
+                            double den = fma(d, d, fma(g, x[k], G));
+                            acc[k] += num / den;
+                        }
+                    }
                 }
             }
         }
@@ -544,58 +572,88 @@ __device__ inline S nh3_select(int sel, double P, const S &hs, const S &sj) {
     return W * sj + (1.0 - W) * hs;
 }
 
+// Forward kernel tiling: a wavefront owns a tile of 64 K columns (lane + k*64)
+// inside one nh3_hs regime (columns < nf_low use the <= 30 GHz set); the 4
+// wavefronts of a block take consecutive tiles, so a block may span the switch.
+__device__ inline int wave_regime(const AlphaArgs &A, int t) {
+    return t < A.tiles_low ? 0 : (t < A.tiles_total ? 1 : -1);
+}
+
 template <int K, bool F32>
-__global__ __launch_bounds__(kThreads) void alpha_kernel(AlphaArgs A) {
+__global__ __launch_bounds__(kThreads, 4) void alpha_kernel(AlphaArgs A) {
     __shared__ SharedFwd sh;
-    const Tile tl = tile_of(A);
-    const int layer = tl.layer;
+    const int layer = blockIdx.y;
     const LayerT<double> L = layer_of(A, layer);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int t = blockIdx.x * (kThreads / 64) + wave;
+    const int regime = wave_regime(A, t);                 // -1: no columns for this wavefront
+    int need_inv = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const int r = wave_regime(A, blockIdx.x * (kThreads / 64) + w);
+        if (r >= 0) need_inv |= 1 << r;
+    }
+    const int mine = regime >= 0 ? 0 : -1;
+    const int f_begin = regime == 0 ? t * A.tile_width : A.nf_low + (t - A.tiles_low) * A.tile_width;
+    const int f_end = regime == 0 ? A.nf_low : A.nf;
 
     int col[K];
     bool ok[K];
     double f[K], x[K], xn[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        col[k] = tl.f_begin + threadIdx.x + k * kThreads;
-        ok[k] = col[k] < tl.f_end;
+        col[k] = f_begin + lane + k * 64;
+        ok[k] = regime >= 0 && col[k] < f_end;
         f[k] = ok[k] ? A.freq[col[k]] : 1.0;
         const double fn = ok[k] ? (A.freq_nh3 ? A.freq_nh3[col[k]] : f[k]) : 1.0;
         x[k] = f[k] * f[k];
         xn[k] = fn * fn;
     }
 
-    double res[RBX_NUM_CONSTITUENTS][K];
+    // total_layer_alpha (alpha.py:151-192): scale-sum in sorted constituent order,
+    // accumulated as each constituent completes
+    const int nl = A.nl;
+    double tot[K];
 #pragma unroll
-    for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
+    for (int k = 0; k < K; ++k) tot[k] = 0.0;
+    auto emit = [&](int c, const double (&v)[K]) {
+        const double sc = A.scale ? A.scale[c * nl + layer] : 1.0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) res[c][k] = 0.0;
+        for (int k = 0; k < K; ++k) {
+            tot[k] += v[k] * sc;
+            if (A.by_c && ok[k]) A.by_c[(static_cast<size_t>(c) * nl + layer) * A.nf + col[k]] = v[k];
+        }
+    };
     double unused = 0.0;
 
     if (A.abs.h2) {
         const double Kh = h2_cia(L, A.abs.h2);
+        double v[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_H2][k] = Kh * x[k];
+        for (int k = 0; k < K; ++k) v[k] = Kh * x[k];
+        emit(RBX_C_H2, v);
     }
     if (A.abs.h2o) {
         const DevBlock &B = A.cat.blk[RBX_LINES_H2O];
         H2oPrep<double> prep(L);
-        const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+        const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
         double acc[K] = {};
         double base = 0.0;
-        line_sum<K, F32>(B, prep, s, x, acc, sh, base);
+        line_sum<K, F32>(B, &prep, 1, mine, s, x, acc, sh, base);
         base = block_reduce_sum(base, sh.red);
         const double cont = h2o_continuum(L);
 #pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_H2O][k] = x[k] * (acc[k] / s + base + cont);
+        for (int k = 0; k < K; ++k) acc[k] = x[k] * (acc[k] / s[0] + base + cont);
+        emit(RBX_C_H2O, acc);
     }
     if (A.abs.h2s) {
         const DevBlock &B = A.cat.blk[RBX_LINES_H2S];
         H2sPrep<double> prep(L);
-        const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+        const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
         double acc[K] = {};
-        line_sum<K, F32>(B, prep, s, x, acc, sh, unused);
+        line_sum<K, F32>(B, &prep, 1, mine, s, x, acc, sh, unused);
 #pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_H2S][k] = x[k] * (acc[k] / s);
+        for (int k = 0; k < K; ++k) acc[k] = x[k] * (acc[k] / s[0]);
+        emit(RBX_C_H2S, acc);
     }
     if (A.abs.nh3) {
         const int sel = A.abs.nh3;
@@ -606,30 +664,32 @@ __global__ __launch_bounds__(kThreads) void alpha_kernel(AlphaArgs A) {
             double acc[K] = {};
             {
                 const DevBlock &B = A.cat.blk[RBX_LINES_NH3_INV];
-                InvPrep<double> prep(L, tl.hi ? 1 : 0);
-                const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+                const InvPrep<double> prep[2] = {InvPrep<double>(L, 0), InvPrep<double>(L, 1)};
+                const double s[2] = {F32 ? pow2_scale(prep[0].pref * B.wscale) : 1.0,
+                                     F32 ? pow2_scale(prep[1].pref * B.wscale) : 1.0};
                 double a2[K] = {};
-                line_sum<K, F32>(B, prep, s, xn, a2, sh, unused);
+                line_sum<K, F32>(B, prep, need_inv, regime, s, xn, a2, sh, unused);
+                const double sm = s[regime > 0 ? 1 : 0];
 #pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s;
+                for (int k = 0; k < K; ++k) acc[k] += a2[k] / sm;
             }
             {
                 const DevBlock &B = A.cat.blk[RBX_LINES_NH3_ROT];
                 RotPrep<double> prep(L);
-                const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+                const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
                 double a2[K] = {};
-                line_sum<K, F32>(B, prep, s, xn, a2, sh, unused);
+                line_sum<K, F32>(B, &prep, 1, mine, s, xn, a2, sh, unused);
 #pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s;
+                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
             }
             {
                 const DevBlock &B = A.cat.blk[RBX_LINES_NH3_V2];
                 V2Prep<double> prep(L);
-                const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+                const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
                 double a2[K] = {};
-                line_sum<K, F32>(B, prep, s, xn, a2, sh, unused);
+                line_sum<K, F32>(B, &prep, 1, mine, s, xn, a2, sh, unused);
 #pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s;
+                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -639,41 +699,27 @@ __global__ __launch_bounds__(kThreads) void alpha_kernel(AlphaArgs A) {
         }
         if (use_sjs) nh3_sjs<K>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh);
 #pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_NH3][k] = nh3_select(sel, L.P, hs[k], sj[k]);
+        for (int k = 0; k < K; ++k) hs[k] = nh3_select(sel, L.P, hs[k], sj[k]);
+        emit(RBX_C_NH3, hs);
     }
     if (A.abs.ph3) {
         const DevBlock &B = A.cat.blk[RBX_LINES_PH3];
         Ph3Prep<double> prep(L);
-        const double s = F32 ? pow2_scale(prep.pref * B.wscale) : 1.0;
+        const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
         double acc[K] = {};
-        line_sum<K, F32>(B, prep, s, x, acc, sh, unused);
+        line_sum<K, F32>(B, &prep, 1, mine, s, x, acc, sh, unused);
 #pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_PH3][k] = x[k] * (acc[k] / s);
+        for (int k = 0; k < K; ++k) acc[k] = x[k] * (acc[k] / s[0]);
+        emit(RBX_C_PH3, acc);
     }
-
-    // total_layer_alpha (alpha.py:151-192): scale-sum in sorted constituent order
-    const int nl = A.nl;
-    double sc[RBX_NUM_CONSTITUENTS];
-#pragma unroll
-    for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c) sc[c] = A.scale ? A.scale[c * nl + layer] : 1.0;
-    const int en[RBX_NUM_CONSTITUENTS] = {A.abs.h2, A.abs.h2o, A.abs.h2s, A.abs.nh3, A.abs.ph3};
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!ok[k]) continue;
-        double tot = 0.0;
-#pragma unroll
-        for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
-            if (en[c]) tot += res[c][k] * sc[c];
         const size_t o = static_cast<size_t>(layer) * A.nf + col[k];
         if (F32)
-            static_cast<float *>(A.alpha)[o] = static_cast<float>(tot);
+            static_cast<float *>(A.alpha)[o] = static_cast<float>(tot[k]);
         else
-            static_cast<double *>(A.alpha)[o] = tot;
-        if (A.by_c) {
-#pragma unroll
-            for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
-                A.by_c[(static_cast<size_t>(c) * nl + layer) * A.nf + col[k]] = res[c][k];
-        }
+            static_cast<double *>(A.alpha)[o] = tot[k];
     }
 }
 
@@ -865,16 +911,20 @@ static void set_tiles(AlphaArgs &A, int K, dim3 &grid) {
     grid = dim3(A.tiles_low + tiles_hi, A.nl);
 }
 
+// frequencies per lane of the forward kernel: rbx_tune(RBX_TUNE_ALPHA_K) or a size rule
 int alpha_tile_k(int nf) {
-    if (nf >= 4 * kThreads * 2) return 4;
-    if (nf >= 2 * kThreads * 2) return 2;
-    return 1;
+    const int k = tune_get(RBX_TUNE_ALPHA_K);
+    if (k == 1 || k == 2 || k == 4) return k;
+    return nf >= 256 ? 2 : 1;
 }
 
 template <int K>
 static int launch_alpha(AlphaArgs &A, int precision, hipStream_t st) {
-    dim3 grid;
-    set_tiles(A, K, grid);
+    A.tile_width = 64 * K;   // one wavefront tile
+    A.tiles_low = (A.nf_low + A.tile_width - 1) / A.tile_width;
+    A.tiles_total = A.tiles_low + (A.nf - A.nf_low + A.tile_width - 1) / A.tile_width;
+    const int waves = kThreads / 64;
+    dim3 grid((A.tiles_total + waves - 1) / waves, A.nl);
     if (precision == RBX_FP32)
         hipLaunchKernelGGL((alpha_kernel<K, true>), grid, dim3(kThreads), 0, st, A);
     else

@@ -1,4 +1,5 @@
 // Error handling, version query and the line catalogue of librbx.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -20,7 +21,18 @@ void set_error(const char *fmt, ...) {
 
 const std::string &last_error() { return g_error; }
 
+static std::atomic<int> g_tune[RBX_NUM_TUNE_KEYS];
+
+int tune_get(int key) { return (key >= 0 && key < RBX_NUM_TUNE_KEYS) ? g_tune[key].load() : 0; }
+
 }  // namespace rbx
+
+extern "C" int rbx_tune(int key, int value) {
+    RBX_CHECK_ARG(key >= 0 && key < RBX_NUM_TUNE_KEYS, "rbx_tune: bad key %d", key);
+    RBX_CHECK_ARG(value >= 0, "rbx_tune: negative value %d", value);
+    rbx::g_tune[key].store(value);
+    return RBX_OK;
+}
 
 using rbx::kNumCols;
 

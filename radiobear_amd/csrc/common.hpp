@@ -32,6 +32,9 @@ const std::string &last_error();
         }                                                                            \
     } while (0)
 
+// launch-parameter overrides set through rbx_tune (0 = built-in choice)
+int tune_get(int key);
+
 inline int launch_status(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -49,12 +52,14 @@ constexpr double kTcmb = 2.725;  // utils.py:77
 // e^{-tau} for tau >= 0 (or NaN): FP64 range reduction t = -tau*log2(e) = n + r,
 // |r| <= 1/2, then 2^r with the hardware FP32 exp2 and an exact FP64 scaling.
 // Relative error ~1e-7, independent of tau.
+// Branch-free: n is clamped at -1100 (the result underflows to 0 there), and a
+// NaN tau gives a NaN r, so NaN propagates as with exp().
 __device__ __forceinline__ double exp_neg_mixed(double tau) {
-    double t = -tau * kLog2e;
-    if (!(t > -1100.0)) return (t != t) ? t : 0.0;
+    const double t = -tau * kLog2e;
     double n = __builtin_rint(t);
-    float r = static_cast<float>(t - n);
-    float e = __builtin_amdgcn_exp2f(r);
+    n = n < -1100.0 ? -1100.0 : n;
+    const float r = static_cast<float>(t - n);
+    const float e = __builtin_amdgcn_exp2f(r);
     return __builtin_ldexp(static_cast<double>(e), static_cast<int>(n));
 }
 

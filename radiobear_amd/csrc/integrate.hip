@@ -36,6 +36,29 @@ __device__ inline double load_alpha(const void *p, int f32, size_t i) {
     return f32 ? static_cast<double>(static_cast<const float *>(p)[i]) : static_cast<const double *>(p)[i];
 }
 
+// One trapezoid step of brightness.py:93-101 (DISC: W = 2 a1 E2(tau)).
+template <int MODE, bool FAST>
+struct Step {
+    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0, a0 = 0.0;
+    __device__ __forceinline__ void operator()(double a1, double hh, double T1) {
+        tau = tau + (a0 + a1) * hh;
+        double W1;
+        if constexpr (MODE == 1)
+            W1 = 2.0 * a1 * expint_e2(tau);
+        else
+            W1 = a1 * (FAST ? exp_neg_mixed(tau) : exp(-tau));
+        intW = intW + (W1 + W0) * hh;
+        const double TW1 = T1 * W1;
+        num = num + (TW1 + TW0) * hh;
+        a0 = a1;
+        W0 = W1;
+        TW0 = TW1;
+    }
+};
+
+// Path steps are taken U at a time: the U rows of alpha, ds and T of a batch are
+// all requested before the first is used, so one memory latency covers U steps.
+template <class AT, int MODE, bool FAST, int U>
 __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     const int wave = threadIdx.x / kLanes;
     const int lane = threadIdx.x % kLanes;
@@ -47,30 +70,32 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     const int jj = ok ? j : A.nf - 1;
     const int S = min(A.nsteps[ray], A.nl - 1);   // never read past the ray's ds row or the table
     const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
-    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
+    const AT *al = static_cast<const AT *>(A.alpha) + jj;
+    const size_t nf = A.nf;
+    Step<MODE, FAST> st;
     if (S > 1) {
-        double a0 = load_alpha(A.alpha, A.f32, jj);
-        for (int i = 0; i < S - 1; ++i) {
-            const double hh = h[i];
-            const double a1 = load_alpha(A.alpha, A.f32, static_cast<size_t>(i + 1) * A.nf + jj);
-            const double T1 = A.T[i + 1];
-            tau = tau + (a0 + a1) * hh;
-            double W1;
-            if (A.mode == 1)
-                W1 = 2.0 * a1 * expint_e2(tau);
-            else
-                W1 = a1 * (A.fast_exp ? exp_neg_mixed(tau) : exp(-tau));
-            intW = intW + (W1 + W0) * hh;
-            const double TW1 = T1 * W1;
-            num = num + (TW1 + TW0) * hh;
-            a0 = a1;
-            W0 = W1;
-            TW0 = TW1;
-            if (A.tau_cut > 0.0 && (i & 7) == 7) {
-                if (__all(!ok || tau > A.tau_cut)) break;
+        const int n = S - 1;
+        st.a0 = static_cast<double>(al[0]);
+        int i = 0;
+        for (; i + U <= n; i += U) {
+            AT ab[U];
+            double hb[U], Tb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ab[u] = al[(i + u + 1) * nf];
+                hb[u] = h[i + u];
+                Tb[u] = A.T[i + u + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) st(static_cast<double>(ab[u]), hb[u], Tb[u]);
+            if (A.tau_cut > 0.0 && __all(!ok || st.tau > A.tau_cut)) {
+                i = n;
+                break;
             }
         }
+        for (; i < n; ++i) st(static_cast<double>(al[(i + 1) * nf]), h[i], A.T[i + 1]);
     }
+    const double intW = st.intW, num = st.num;
     if (!ok) return;
     const size_t o = static_cast<size_t>(ray) * A.nf + j;
     double Tb;
@@ -282,8 +307,20 @@ extern "C" int rbx_integrate(const void *alpha, int alpha_precision, const doubl
     A.intW = intW;
     const long long blocks = static_cast<long long>(A.ngroups) * ((nray + rbx::kWaves - 1) / rbx::kWaves);
     RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate: too many blocks");
-    hipLaunchKernelGGL(rbx::integrate_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes),
-                       0, static_cast<hipStream_t>(stream), A);
+    const dim3 grid(static_cast<unsigned>(blocks)), block(rbx::kWaves * rbx::kLanes);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool u4 = rbx::tune_get(RBX_TUNE_INT_UNROLL) == 4;
+#define RBX_INT_LAUNCH(AT, MODE, FAST, U) hipLaunchKernelGGL((rbx::integrate_kernel<AT, MODE, FAST, U>), grid, block, 0, st, A)
+    if (mode == 1) {
+        if (A.f32) RBX_INT_LAUNCH(float, 1, false, 4); else RBX_INT_LAUNCH(double, 1, false, 4);
+    } else if (A.f32) {
+        if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(float, 0, true, 4); else RBX_INT_LAUNCH(float, 0, true, 8); }
+        else { if (u4) RBX_INT_LAUNCH(float, 0, false, 4); else RBX_INT_LAUNCH(float, 0, false, 8); }
+    } else {
+        if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(double, 0, true, 4); else RBX_INT_LAUNCH(double, 0, true, 8); }
+        else { if (u4) RBX_INT_LAUNCH(double, 0, false, 4); else RBX_INT_LAUNCH(double, 0, false, 8); }
+    }
+#undef RBX_INT_LAUNCH
     return rbx::launch_status("integrate_kernel");
 }
 

@@ -1,0 +1,89 @@
+"""Every launch shape the autotuner can pick gives the same results."""
+import numpy as np
+import pytest
+
+from oracle import layers
+from radiobear_amd import synth
+
+pytestmark = pytest.mark.gpu
+C = synth.GAS_ROWS
+JUPITER_ALPHA = {'h2': 'h2_jj_ddb', 'h2o': 'h2o_bk', 'h2s': 'h2s_ddb', 'nh3': 'nh3_hs_sjs', 'ph3': 'ph3_jh'}
+
+
+@pytest.fixture(scope='module')
+def K():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no HIP device')
+    from radiobear_amd import kernels
+    return kernels
+
+
+@pytest.fixture
+def at(K):
+    from radiobear_amd import autotune
+    yield autotune
+    autotune.reset()
+
+
+@pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
+def test_alpha_every_tile_width(K, at, raw_lines, jupiter_lines, precision, tol):
+    """1, 2 and 4 frequencies per lane: identical tables (the per-frequency line order does
+    not depend on the tile width), all matching the oracle.  330 freqs straddle the
+    30 GHz regime switch inside a tile; 41 layers leave a partial last tile."""
+    atm = synth.jupiter_like(41)
+    h2o = synth.h2o_lines(300, 5)
+    cat = K.jupiter_catalog(raw_lines, h2o=h2o)
+    lines = dict(jupiter_lines)
+    lines['h2o'] = h2o
+    f = np.logspace(np.log10(1.0), np.log10(200.0), 330)
+    ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(lines))
+    out = {}
+    for k in at.CANDIDATES['alpha_k']:
+        at.set_knob('alpha_k', k)
+        out[k] = K.alpha(cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), f,
+                         precision).double().cpu().numpy()
+        got = out[k].T
+        assert np.max(np.abs(got - ref) / np.abs(ref)) < tol, k
+    for k in out:
+        assert np.array_equal(out[k], out[1]), k
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_integrate_every_unroll(K, at, precision):
+    """4 and 8 path steps per load batch: the same Tb to rounding (the compiler may contract
+    multiply-adds differently in the two loop bodies) without the tau cut, and with the cut
+    (checked once per batch, so the two stop at different steps) within e^-50 of it."""
+    import torch
+    atm = synth.jupiter_like(120)
+    rng = np.random.default_rng(3)
+    nf = 70
+    tab = rng.uniform(1e-9, 2e-6, size=(atm.nl, nf))
+    dt = torch.float64 if precision == 'fp64' else torch.float32
+    tab = torch.as_tensor(tab, device='cuda').to(dt)
+    pl = K.planet_desc(atm.Req, atm.Rpol)
+    rays = synth.random_disc_rays(300, 11)
+    hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
+    res = {}
+    for u in at.CANDIDATES['int_unroll']:
+        at.set_knob('int_unroll', u)
+        a = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=0.0).cpu().numpy()
+        b = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=50.0).cpu().numpy()
+        res[u] = (a, b)
+    base = res[at.CANDIDATES['int_unroll'][0]][0]
+    ok = ~np.isnan(base)
+    assert ok.any()
+    for u, (a, b) in res.items():
+        assert np.array_equal(np.isnan(a), ~ok)
+        scale = np.max(np.abs(base[ok]))
+        assert np.max(np.abs(a[ok] - base[ok])) < 1e-13 * scale, u
+        assert np.max(np.abs(b[ok] - base[ok])) < 1e-12 * scale, u
+
+
+def test_tune_picks_a_candidate_and_caches(K, at, tmp_path):
+    calls = []
+    rec = at.tune('int_unroll', 'test-shape', lambda: calls.append(1), reps=1, path=str(tmp_path / 't.json'))
+    assert rec['best'] in at.CANDIDATES['int_unroll']
+    n = len(calls)
+    again = at.tune('int_unroll', 'test-shape', lambda: calls.append(1), reps=1, path=str(tmp_path / 't.json'))
+    assert again == rec and len(calls) == n
