@@ -38,6 +38,8 @@ def parse():
     p.add_argument('--no-secondary', action='store_true')
     p.add_argument('--quiet', action='store_true')
     p.add_argument('--no-tune', action='store_true', help='use the built-in launch shapes (no autotuning)')
+    p.add_argument('--tune-cache', default=None,
+                   help='JSON file of tuned launch shapes: read if it holds this shape, else written')
     p.add_argument('--copy-calibration', action='store_true',
                    help='run only a plain device copy of 1 GiB (calibrates FETCH_SIZE / WRITE_SIZE)')
     return p.parse_args()
@@ -100,6 +102,7 @@ class Problem:
         self.Tb = torch.empty((self.nray, self.nf), dtype=torch.float64, device=dev)
         self.Tb64 = torch.empty((self.nray, self.nf), dtype=torch.float64, device=dev)
         self.nlines = {k: self.cat.lines(k) for k in range(7)}
+        self.ws = None
 
     # the three launches of one step, on torch's current stream
     def launch_alpha(self, prec):
@@ -107,9 +110,15 @@ class Problem:
         from radiobear_amd import _lib
         K = self.K
         out = self.alpha32 if prec == _lib.FP32 else self.alpha64
+        self.ws, nbytes = K.alpha_workspace(self.cat, self.ab, self.nl, self.nf, self.nf_low, prec, reuse=self.ws)
         _lib.check(_lib.lib().rbx_alpha(self.cat.handle, ctypes.byref(self.ab), K._ptr(self.P), K._ptr(self.T),
                                         K._ptr(self.mix), self.nl, K._ptr(self.f), None, self.nf, self.nf_low,
-                                        None, prec, K._ptr(out), None, K._stream()))
+                                        None, prec, K._ptr(out), None, K._ptr(self.ws), nbytes, K._stream()))
+
+    def alpha_parts(self, prec):
+        """Parts per layer of the line sums under the current launch settings."""
+        _, nbytes = self.K.alpha_workspace(self.cat, self.ab, self.nl, self.nf, self.nf_low, prec, reuse=self.ws)
+        return nbytes // 8 // (4 * self.nl * self.nf + self.nl)
 
     def launch_paths(self):
         import ctypes
@@ -160,10 +169,12 @@ def autotune(prob, prec, args):
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         prob.launch_paths()
-        ka = at.tune('alpha_k', key, lambda: prob.launch_alpha(prec))
-        ki = at.tune('int_unroll', key, lambda: prob.launch_integrate(prec))
+        ka = at.tune('alpha_k', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
+        ks = at.tune('alpha_split', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
+        ki = at.tune('int_unroll', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
     torch.cuda.synchronize()
-    return {'alpha_k': ka['best'], 'int_unroll': ki['best'], 'alpha_ms': ka['ms'], 'integrate_ms': ki['ms']}
+    return {'alpha_k': ka['best'], 'alpha_split': ks['best'], 'alpha_parts': prob.alpha_parts(prec),
+            'int_unroll': ki['best'], 'alpha_k_ms': ka['ms'], 'alpha_split_ms': ks['ms'], 'integrate_ms': ki['ms']}
 
 
 def timed_graph(prob, prec, steps, warmup):
@@ -188,7 +199,7 @@ def timed_graph(prob, prec, steps, warmup):
 def kernel_times(prob, prec, steps):
     """Average duration of each launch, HIP events on the launch stream."""
     import torch
-    names = ('alpha_kernel', 'raypath_kernel', 'integrate_kernel')
+    names = ('rbx_alpha', 'rbx_raypaths', 'rbx_integrate')
     acc = {n: 0.0 for n in names}
     for _ in range(steps):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -338,7 +349,7 @@ def main():
 
     # roofline of the dominant kernel (by duration)
     dom = max(kt32, key=kt32.get)
-    if dom == 'alpha_kernel':
+    if dom == 'rbx_alpha':
         flop = prob.alpha_flop()
         achieved = flop / (kt32[dom] * 1e-3) / 1e12
         roof = {'kernel': dom, 'bound': 'valu', 'achieved': achieved, 'peak': PEAK['fp32'], 'unit': 'TFLOP/s',

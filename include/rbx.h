@@ -12,7 +12,8 @@
  *   - `stream` is a hipStream_t passed as void* (NULL = default stream); all
  *     work is enqueued asynchronously on it.
  *   - The library allocates device memory only inside an rbx_catalog handle,
- *     which it frees in rbx_catalog_destroy.
+ *     which it frees in rbx_catalog_destroy; scratch memory a call needs (the
+ *     rbx_alpha workspace) is passed in by the caller.
  *
  * Reference interfaces replaced (RadioBEAR 2.0.1, paths relative to the
  * reference repository):
@@ -42,7 +43,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 1
+#define RBX_ABI_VERSION 2
 
 /* status codes */
 #define RBX_OK          0
@@ -116,9 +117,10 @@ int rbx_device_count(int *count);
 
 /* Launch-parameter overrides (no reference counterpart; used by the autotuner):
  * applies to every later call in the process; value 0 restores the built-in choice. */
-#define RBX_TUNE_ALPHA_K    0   /* frequencies per lane of rbx_alpha: 1, 2 or 4 */
-#define RBX_TUNE_INT_UNROLL 1   /* path steps per load batch of rbx_integrate: 4 or 8 */
-#define RBX_NUM_TUNE_KEYS   2
+#define RBX_TUNE_ALPHA_K     0   /* frequencies per lane of rbx_alpha: 1, 2 or 4 */
+#define RBX_TUNE_INT_UNROLL  1   /* path steps per load batch of rbx_integrate: 4 or 8 */
+#define RBX_TUNE_ALPHA_SPLIT 2   /* parts each layer's line sums are split into: 1..16 */
+#define RBX_NUM_TUNE_KEYS    3
 int rbx_tune(int key, int value);
 
 /* ---- line catalogue (device-resident, owned by the library) ---- */
@@ -142,13 +144,21 @@ int rbx_catalog_destroy(rbx_catalog *cat);
  *                 (total_layer_alpha, alpha.py:151-192) or NULL for 1.0
  * alpha           [nl][nf] float64 (RBX_FP64) or float32 (RBX_FP32), layer-major
  * by_constituent  [RBX_NUM_CONSTITUENTS][nl][nf] float64 unscaled per-constituent
- *                 alpha, or NULL (zero rows for constituents that are not loaded)
+ *                 alpha, or NULL (rows of constituents that are not loaded are not written)
+ * workspace       device scratch of at least rbx_alpha_workspace_size bytes (8-byte
+ *                 aligned) for the partial line sums; it must not be used by another
+ *                 call that runs at the same time (a stream-ordered allocation is right)
  */
 int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
               const double *P, const double *T, const double *mix, int nl,
               const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
               const double *scale, int precision, void *alpha, double *by_constituent,
-              void *stream);
+              void *workspace, size_t workspace_bytes, void *stream);
+
+/* Workspace bytes rbx_alpha needs for this shape (host call; depends on the
+ * rbx_tune settings in force at the rbx_alpha call). */
+int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
+                             int nf_nh3_low, int precision, size_t *bytes);
 
 /* ---- absorption tangents for the adjoint ----
  * dalpha          [3][nl][nf] float64: d alpha / d T, d alpha / d q_NH3,

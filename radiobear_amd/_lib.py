@@ -7,7 +7,7 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'librbx.so')
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 FP64 = 0
 FP32 = 1
@@ -28,10 +28,11 @@ SPECIES = ('H2', 'HE', 'CH4', 'NH3', 'H2O', 'H2S', 'PH3')  # RBX_X_* rows of mix
 
 TUNE_ALPHA_K = 0
 TUNE_INT_UNROLL = 1
+TUNE_ALPHA_SPLIT = 2
 
 EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_tune', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
-           'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles', 'rbx_jacobian')
+           'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles', 'rbx_jacobian')
 
 
 class RbxLibraryError(RuntimeError):
@@ -70,7 +71,9 @@ _SIGS = {
     'rbx_catalog_lines': (_I, [_VP, _I, ctypes.POINTER(_I)]),
     'rbx_catalog_destroy': (_I, [_VP]),
     'rbx_alpha': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
-                       _VP, _I, _VP, _VP, _VP]),
+                       _VP, _I, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
+    'rbx_alpha_workspace_size': (_I, [_VP, ctypes.POINTER(Absorbers), _I, _I, _I, _I,
+                                      ctypes.POINTER(ctypes.c_size_t)]),
     'rbx_alpha_tangent': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
                                _VP, _VP, _VP]),
     'rbx_raypaths': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _I, _VP, _I, _VP, _VP]),

@@ -10,6 +10,9 @@ Knobs:
                   amortise the LDS reads of each line over more frequencies,
                   narrower tiles waste fewer lanes and give more blocks for
                   the tail of the grid;
+* ``alpha_split`` parts each layer's line sums are split into (0 = the
+                  library's model of the grid tail, or 1, 2, 4, 8): more parts
+                  give more, shorter blocks, so the last wave of blocks idles less;
 * ``int_unroll``  path steps per load batch of rbx_integrate (4, 8).
 """
 import json
@@ -19,8 +22,8 @@ import torch
 
 from . import _lib
 
-CANDIDATES = {'alpha_k': (1, 2, 4), 'int_unroll': (4, 8)}
-_KEYS = {'alpha_k': _lib.TUNE_ALPHA_K, 'int_unroll': _lib.TUNE_INT_UNROLL}
+CANDIDATES = {'alpha_k': (1, 2, 4), 'alpha_split': (0, 1, 2, 4, 8), 'int_unroll': (4, 8)}
+_KEYS = {'alpha_k': _lib.TUNE_ALPHA_K, 'alpha_split': _lib.TUNE_ALPHA_SPLIT, 'int_unroll': _lib.TUNE_INT_UNROLL}
 _cache = {}
 
 

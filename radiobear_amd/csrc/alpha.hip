@@ -37,6 +37,9 @@
 // The tangent kernel runs the same coefficient code on dual numbers
 // (value, d/dT, d/dq_NH3, d/dq_H2O) and differentiates the canonical term in
 // the line sum: d term = (dN - term dQ) / Q.
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
 #include <type_traits>
 
 #include "catalog.hpp"
@@ -46,6 +49,7 @@ namespace rbx {
 
 constexpr int kThreads = 256;
 constexpr int kSub = 8;    // FP32 partial-sum length (lines unrolled per step)
+constexpr int kMaxSplit = 16;  // most parts a layer's line sums are split into
 
 // physical constants as the plug-ins spell them
 constexpr double kHsHcKb = 19.858252418E-24 / 1.38E-23;    // nh3_hs.py:75-76
@@ -89,6 +93,8 @@ struct AlphaArgs {
     int tiles_low;     // tiles covering the columns < nf_low
     int tiles_total;   // forward kernel: tiles of both regimes
     int tile_width;    // columns per tile
+    double *ws;        // forward: partial line sums [kSlots][nsplit][nl][nf]
+    double *ws_base;   // forward: partial h2o base terms [nsplit][nl]
 };
 
 // ---- per-layer coefficient sets: each derives (D, a, c, g, G) of one line ----
@@ -255,7 +261,7 @@ union SharedTan {
     double red[kNd + 1][kThreads];
 };
 
-// ---- forward line sum over one block for K frequencies per lane ----
+// ---- forward line sum over lines [lb, le) of one block for K frequencies per lane ----
 // prep[0..1]: coefficient sets (two only for the nh3_hs inversion lines, whose
 // <= / > 30 GHz constants differ); `need` is the block's bitmask of sets to
 // stage, `mine` the set this wavefront sums (-1: the wavefront has no columns).
@@ -263,20 +269,20 @@ union SharedTan {
 // loop has a fixed trip count and unrolls.
 template <int K, bool F32, class Prep>
 __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, int mine, const double *scale,
-                         const double (&x)[K], double (&acc)[K], SharedFwd &sh, double &base) {
+                         const double (&x)[K], double (&acc)[K], SharedFwd &sh, double &base, int lb, int le) {
     float xf[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) xf[k] = static_cast<float>(x[k]);
-    for (int l0 = 0; l0 < B.n; l0 += kThreads) {
+    for (int l0 = lb; l0 < le; l0 += kThreads) {
         const int i = l0 + threadIdx.x;
-        const int cnt = min(kThreads, B.n - l0);
+        const int cnt = min(kThreads, le - l0);
         const int cntp = (cnt + kSub - 1) / kSub * kSub;
         if (static_cast<int>(threadIdx.x) < cntp) {
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 if (!(need & (1 << s))) continue;
                 CoefT<double> q = {0.0, 0.0, 0.0, 0.0, 1.0};   // padding: term = 0 / ((x)^2 + 1)
-                if (i < B.n) q = prep[s](B, i, base);
+                if (i < le) q = prep[s](B, i, base);
                 if constexpr (F32) {
                     sh.s32.D[s][threadIdx.x] = q.D;
                     sh.s32.q[s][threadIdx.x] =
@@ -579,10 +585,31 @@ __device__ inline int wave_regime(const AlphaArgs &A, int t) {
     return t < A.tiles_low ? 0 : (t < A.tiles_total ? 1 : -1);
 }
 
+// ---- forward path: split line sums, then a per-column finish ----
+//
+// A layer's lines are visited as one sequence of staging rounds (kThreads lines
+// each) over the segments h2o, h2s, nh3 inversion, nh3 rotational, nh3 v2, ph3.
+// alpha_lines_kernel block (tile group, layer, part) sums the rounds
+// [part R / nsplit, (part + 1) R / nsplit) of that sequence into four partial
+// sums per column (h2o, h2s, nh3_hs, ph3 slots, each already divided by its
+// FP32 scale) and writes them to the workspace; splitting the lines lets the
+// grid be cut into enough equal blocks to keep every CU busy to the end.
+// alpha_finish_kernel then adds the parts in order, applies the closed forms
+// (H2 CIA, H2O continuum), the nh3_hs negative clamp, nh3_sjs and the
+// pressure blend, and scale-sums the constituents (alpha.py:151-192).
+constexpr int kSlots = 4;
+enum { kSlotH2o = 0, kSlotH2s = 1, kSlotNh3 = 2, kSlotPh3 = 3 };
+
+__device__ __host__ inline int rounds_of(int n) { return (n + kThreads - 1) / kThreads; }
+
+// nh3_hs_sjs (nh3_hs_sjs.py:6-26): which parts a layer needs; sel 1 blend, 2 hs, 3 sjs
+__device__ inline bool nh3_uses_hs(int sel, double P) { return sel == 2 || (sel == 1 && P <= 2000.0); }
+__device__ inline bool nh3_uses_sjs(int sel, double P) { return sel == 3 || (sel == 1 && P >= 400.0); }
+
 template <int K, bool F32>
-__global__ __launch_bounds__(kThreads, 4) void alpha_kernel(AlphaArgs A) {
+__global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
     __shared__ SharedFwd sh;
-    const int layer = blockIdx.y;
+    const int layer = blockIdx.y, part = blockIdx.z, nsplit = gridDim.z;
     const LayerT<double> L = layer_of(A, layer);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int t = blockIdx.x * (kThreads / 64) + wave;
@@ -598,129 +625,157 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_kernel(AlphaArgs A) {
 
     int col[K];
     bool ok[K];
-    double f[K], x[K], xn[K];
+    double x[K], xn[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         col[k] = f_begin + lane + k * 64;
         ok[k] = regime >= 0 && col[k] < f_end;
-        f[k] = ok[k] ? A.freq[col[k]] : 1.0;
-        const double fn = ok[k] ? (A.freq_nh3 ? A.freq_nh3[col[k]] : f[k]) : 1.0;
-        x[k] = f[k] * f[k];
+        const double f = ok[k] ? A.freq[col[k]] : 1.0;
+        const double fn = ok[k] ? (A.freq_nh3 ? A.freq_nh3[col[k]] : f) : 1.0;
+        x[k] = f * f;
         xn[k] = fn * fn;
     }
 
-    // total_layer_alpha (alpha.py:151-192): scale-sum in sorted constituent order,
-    // accumulated as each constituent completes
-    const int nl = A.nl;
-    double tot[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) tot[k] = 0.0;
-    auto emit = [&](int c, const double (&v)[K]) {
-        const double sc = A.scale ? A.scale[c * nl + layer] : 1.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            tot[k] += v[k] * sc;
-            if (A.by_c && ok[k]) A.by_c[(static_cast<size_t>(c) * nl + layer) * A.nf + col[k]] = v[k];
-        }
+    const DevBlock *blk = A.cat.blk;
+    const bool hs = A.abs.nh3 && nh3_uses_hs(A.abs.nh3, L.P);
+    const int nr[6] = {A.abs.h2o ? rounds_of(blk[RBX_LINES_H2O].n) : 0,
+                       A.abs.h2s ? rounds_of(blk[RBX_LINES_H2S].n) : 0,
+                       hs ? rounds_of(blk[RBX_LINES_NH3_INV].n) : 0,
+                       hs ? rounds_of(blk[RBX_LINES_NH3_ROT].n) : 0,
+                       hs ? rounds_of(blk[RBX_LINES_NH3_V2].n) : 0,
+                       A.abs.ph3 ? rounds_of(blk[RBX_LINES_PH3].n) : 0};
+    const int R = nr[0] + nr[1] + nr[2] + nr[3] + nr[4] + nr[5];
+    const int r0 = static_cast<int>(static_cast<long long>(part) * R / nsplit);
+    const int r1 = static_cast<int>(static_cast<long long>(part + 1) * R / nsplit);
+    int seg = 0, off = 0, lb = 0, le = 0;
+    // lines [lb, le) of the next segment that fall in this part's rounds
+    auto next = [&](int n) {
+        const int a = max(r0, off), b = min(r1, off + nr[seg]);
+        lb = (a - off) * kThreads;
+        le = min(n, (b - off) * kThreads);
+        off += nr[seg++];
+        return a < b;
     };
-    double unused = 0.0;
 
-    if (A.abs.h2) {
-        const double Kh = h2_cia(L, A.abs.h2);
-        double v[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = Kh * x[k];
-        emit(RBX_C_H2, v);
-    }
-    if (A.abs.h2o) {
-        const DevBlock &B = A.cat.blk[RBX_LINES_H2O];
+    double acc[kSlots][K] = {};
+    double base = 0.0, unused = 0.0;
+    bool did_h2o = false;
+    if (next(blk[RBX_LINES_H2O].n)) {
         H2oPrep<double> prep(L);
-        const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
-        double acc[K] = {};
-        double base = 0.0;
-        line_sum<K, F32>(B, &prep, 1, mine, s, x, acc, sh, base);
-        base = block_reduce_sum(base, sh.red);
-        const double cont = h2o_continuum(L);
+        const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_H2O].wscale) : 1.0, 1.0};
+        double a2[K] = {};
+        line_sum<K, F32>(blk[RBX_LINES_H2O], &prep, 1, mine, s, x, a2, sh, base, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = x[k] * (acc[k] / s[0] + base + cont);
-        emit(RBX_C_H2O, acc);
+        for (int k = 0; k < K; ++k) acc[kSlotH2o][k] = a2[k] / s[0];
+        did_h2o = true;
     }
-    if (A.abs.h2s) {
-        const DevBlock &B = A.cat.blk[RBX_LINES_H2S];
+    if (next(blk[RBX_LINES_H2S].n)) {
         H2sPrep<double> prep(L);
-        const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
-        double acc[K] = {};
-        line_sum<K, F32>(B, &prep, 1, mine, s, x, acc, sh, unused);
+        const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_H2S].wscale) : 1.0, 1.0};
+        double a2[K] = {};
+        line_sum<K, F32>(blk[RBX_LINES_H2S], &prep, 1, mine, s, x, a2, sh, unused, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = x[k] * (acc[k] / s[0]);
-        emit(RBX_C_H2S, acc);
+        for (int k = 0; k < K; ++k) acc[kSlotH2s][k] = a2[k] / s[0];
     }
+    if (next(blk[RBX_LINES_NH3_INV].n)) {
+        const InvPrep<double> prep[2] = {InvPrep<double>(L, 0), InvPrep<double>(L, 1)};
+        const double s[2] = {F32 ? pow2_scale(prep[0].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0,
+                             F32 ? pow2_scale(prep[1].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0};
+        double a2[K] = {};
+        line_sum<K, F32>(blk[RBX_LINES_NH3_INV], prep, need_inv, regime, s, xn, a2, sh, unused, lb, le);
+        const double sm = s[regime > 0 ? 1 : 0];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / sm;
+    }
+    if (next(blk[RBX_LINES_NH3_ROT].n)) {
+        RotPrep<double> prep(L);
+        const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_NH3_ROT].wscale) : 1.0, 1.0};
+        double a2[K] = {};
+        line_sum<K, F32>(blk[RBX_LINES_NH3_ROT], &prep, 1, mine, s, xn, a2, sh, unused, lb, le);
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[0];
+    }
+    if (next(blk[RBX_LINES_NH3_V2].n)) {
+        V2Prep<double> prep(L);
+        const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_NH3_V2].wscale) : 1.0, 1.0};
+        double a2[K] = {};
+        line_sum<K, F32>(blk[RBX_LINES_NH3_V2], &prep, 1, mine, s, xn, a2, sh, unused, lb, le);
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[0];
+    }
+    if (next(blk[RBX_LINES_PH3].n)) {
+        Ph3Prep<double> prep(L);
+        const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_PH3].wscale) : 1.0, 1.0};
+        double a2[K] = {};
+        line_sum<K, F32>(blk[RBX_LINES_PH3], &prep, 1, mine, s, x, a2, sh, unused, lb, le);
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[kSlotPh3][k] = a2[k] / s[0];
+    }
+    if (did_h2o) base = block_reduce_sum(base, sh.red);   // did_h2o is the same for the whole block
+
+    const int en[kSlots] = {A.abs.h2o, A.abs.h2s, A.abs.nh3, A.abs.ph3};
+    const size_t plane = static_cast<size_t>(A.nl) * A.nf;
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        if (!en[q]) continue;
+        double *w = A.ws + (static_cast<size_t>(q) * nsplit + part) * plane + static_cast<size_t>(layer) * A.nf;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (ok[k]) w[col[k]] = acc[q][k];
+    }
+    if (A.abs.h2o && blockIdx.x == 0 && threadIdx.x == 0) A.ws_base[static_cast<size_t>(part) * A.nl + layer] = base;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int nsplit) {
+    __shared__ SharedFwd sh;
+    const int layer = blockIdx.y;
+    const int nl = A.nl;
+    const LayerT<double> L = layer_of(A, layer);
+    const int col = blockIdx.x * kThreads + threadIdx.x;
+    const bool ok = col < A.nf;
+    const int cc = ok ? col : A.nf - 1;
+    const double f[1] = {A.freq[cc]};
+    const double fn = A.freq_nh3 ? A.freq_nh3[cc] : f[0];
+    const double x = f[0] * f[0], xn = fn * fn;
+    const size_t plane = static_cast<size_t>(nl) * A.nf;
+    const size_t o = static_cast<size_t>(layer) * A.nf + cc;
+    auto parts = [&](int q) {
+        const double *w = A.ws + static_cast<size_t>(q) * nsplit * plane + o;
+        double s = 0.0;
+        for (int p = 0; p < nsplit; ++p) s += w[static_cast<size_t>(p) * plane];
+        return s;
+    };
+
+    double tot = 0.0;
+    auto emit = [&](int c, double v) {
+        const double sc = A.scale ? A.scale[c * nl + layer] : 1.0;
+        tot += v * sc;
+        if (A.by_c && ok) A.by_c[(static_cast<size_t>(c) * nl + layer) * A.nf + col] = v;
+    };
+    if (A.abs.h2) emit(RBX_C_H2, h2_cia(L, A.abs.h2) * x);
+    if (A.abs.h2o) {
+        double base = 0.0;
+        for (int p = 0; p < nsplit; ++p) base += A.ws_base[static_cast<size_t>(p) * nl + layer];
+        emit(RBX_C_H2O, x * (parts(kSlotH2o) + base + h2o_continuum(L)));
+    }
+    if (A.abs.h2s) emit(RBX_C_H2S, x * parts(kSlotH2s));
     if (A.abs.nh3) {
         const int sel = A.abs.nh3;
-        const bool use_hs = sel == 2 || (sel == 1 && L.P <= 2000.0);
-        const bool use_sjs = sel == 3 || (sel == 1 && L.P >= 400.0);
-        double hs[K] = {}, sj[K] = {};
-        if (use_hs) {
-            double acc[K] = {};
-            {
-                const DevBlock &B = A.cat.blk[RBX_LINES_NH3_INV];
-                const InvPrep<double> prep[2] = {InvPrep<double>(L, 0), InvPrep<double>(L, 1)};
-                const double s[2] = {F32 ? pow2_scale(prep[0].pref * B.wscale) : 1.0,
-                                     F32 ? pow2_scale(prep[1].pref * B.wscale) : 1.0};
-                double a2[K] = {};
-                line_sum<K, F32>(B, prep, need_inv, regime, s, xn, a2, sh, unused);
-                const double sm = s[regime > 0 ? 1 : 0];
-#pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += a2[k] / sm;
-            }
-            {
-                const DevBlock &B = A.cat.blk[RBX_LINES_NH3_ROT];
-                RotPrep<double> prep(L);
-                const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
-                double a2[K] = {};
-                line_sum<K, F32>(B, &prep, 1, mine, s, xn, a2, sh, unused);
-#pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
-            }
-            {
-                const DevBlock &B = A.cat.blk[RBX_LINES_NH3_V2];
-                V2Prep<double> prep(L);
-                const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
-                double a2[K] = {};
-                line_sum<K, F32>(B, &prep, 1, mine, s, xn, a2, sh, unused);
-#pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
-            }
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                hs[k] = xn[k] * acc[k];
-                if (hs[k] < 0.0) hs[k] = 1.0E-8;   // nh3_hs.py:337-338
-            }
+        double hs = 0.0, sj[1] = {0.0};
+        if (nh3_uses_hs(sel, L.P)) {
+            hs = xn * parts(kSlotNh3);
+            if (hs < 0.0) hs = 1.0E-8;   // nh3_hs.py:337-338
         }
-        if (use_sjs) nh3_sjs<K>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh);
-#pragma unroll
-        for (int k = 0; k < K; ++k) hs[k] = nh3_select(sel, L.P, hs[k], sj[k]);
-        emit(RBX_C_NH3, hs);
+        if (nh3_uses_sjs(sel, L.P)) nh3_sjs<1>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh);   // block-uniform
+        emit(RBX_C_NH3, nh3_select(sel, L.P, hs, sj[0]));
     }
-    if (A.abs.ph3) {
-        const DevBlock &B = A.cat.blk[RBX_LINES_PH3];
-        Ph3Prep<double> prep(L);
-        const double s[2] = {F32 ? pow2_scale(prep.pref * B.wscale) : 1.0, 1.0};
-        double acc[K] = {};
-        line_sum<K, F32>(B, &prep, 1, mine, s, x, acc, sh, unused);
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = x[k] * (acc[k] / s[0]);
-        emit(RBX_C_PH3, acc);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (!ok[k]) continue;
-        const size_t o = static_cast<size_t>(layer) * A.nf + col[k];
-        if (F32)
-            static_cast<float *>(A.alpha)[o] = static_cast<float>(tot[k]);
-        else
-            static_cast<double *>(A.alpha)[o] = tot[k];
-    }
+    if (A.abs.ph3) emit(RBX_C_PH3, x * parts(kSlotPh3));
+    if (!ok) return;
+    if (F32)
+        static_cast<float *>(A.alpha)[o] = static_cast<float>(tot);
+    else
+        static_cast<double *>(A.alpha)[o] = tot;
 }
 
 // ---- tangent kernel: dalpha[3][nl][nf] = d alpha / d(T, q_NH3, q_H2O) ----
@@ -911,43 +966,159 @@ static void set_tiles(AlphaArgs &A, int K, dim3 &grid) {
     grid = dim3(A.tiles_low + tiles_hi, A.nl);
 }
 
-// frequencies per lane of the forward kernel: rbx_tune(RBX_TUNE_ALPHA_K) or a size rule
-int alpha_tile_k(int nf) {
+// frequencies per lane of the forward kernel: rbx_tune(RBX_TUNE_ALPHA_K), else 1
+static int alpha_tile_k() {
     const int k = tune_get(RBX_TUNE_ALPHA_K);
-    if (k == 1 || k == 2 || k == 4) return k;
-    return nf >= 256 ? 2 : 1;
+    return (k == 1 || k == 2 || k == 4) ? k : 1;
 }
 
-template <int K>
-static int launch_alpha(AlphaArgs &A, int precision, hipStream_t st) {
+static void set_wave_tiles(AlphaArgs &A, int K) {
     A.tile_width = 64 * K;   // one wavefront tile
     A.tiles_low = (A.nf_low + A.tile_width - 1) / A.tile_width;
     A.tiles_total = A.tiles_low + (A.nf - A.nf_low + A.tile_width - 1) / A.tile_width;
+}
+
+template <int K>
+static const void *lines_kernel_ptr(int precision) {
+    return precision == RBX_FP32 ? reinterpret_cast<const void *>(&alpha_lines_kernel<K, true>)
+                                 : reinterpret_cast<const void *>(&alpha_lines_kernel<K, false>);
+}
+
+// resident blocks of the line-sum kernel on the current device (cached per device and variant)
+static int resident_blocks(int K, int precision, int &out) {
+    constexpr int kMaxDev = 64;
+    static std::atomic<int> cache[kMaxDev][3][2];
+    int dev = 0;
+    RBX_CHECK_HIP(hipGetDevice(&dev));
+    const int ki = K == 4 ? 2 : K - 1, pi = precision == RBX_FP32 ? 1 : 0;
+    if (dev < kMaxDev && cache[dev][ki][pi].load() > 0) {
+        out = cache[dev][ki][pi].load();
+        return RBX_OK;
+    }
+    const void *fn = K == 4 ? lines_kernel_ptr<4>(precision)
+                            : (K == 2 ? lines_kernel_ptr<2>(precision) : lines_kernel_ptr<1>(precision));
+    int per_cu = 0, cus = 0;
+    RBX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0));
+    RBX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    out = std::max(1, per_cu * cus);
+    if (dev < kMaxDev) cache[dev][ki][pi].store(out);
+    return RBX_OK;
+}
+
+// Parts per layer of the line sums: rbx_tune(RBX_TUNE_ALPHA_SPLIT), else the split
+// that minimises (dispatch rounds of equal blocks) x (staging rounds per block + 0.2,
+// the block's own set-up), i.e. the grid's tail.
+static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
+    const DevBlock *blk = A.cat.blk;
+    const bool hs = A.abs.nh3 == 1 || A.abs.nh3 == 2;
+    const int R = (A.abs.h2o ? rounds_of(blk[RBX_LINES_H2O].n) : 0) +
+                  (A.abs.h2s ? rounds_of(blk[RBX_LINES_H2S].n) : 0) +
+                  (hs ? rounds_of(blk[RBX_LINES_NH3_INV].n) + rounds_of(blk[RBX_LINES_NH3_ROT].n) +
+                            rounds_of(blk[RBX_LINES_NH3_V2].n)
+                      : 0) +
+                  (A.abs.ph3 ? rounds_of(blk[RBX_LINES_PH3].n) : 0);
+    const int forced = tune_get(RBX_TUNE_ALPHA_SPLIT);
+    if (forced > 0) {
+        nsplit = std::min(forced, kMaxSplit);
+        return RBX_OK;
+    }
+    nsplit = 1;
+    if (R <= 1) return RBX_OK;
+    int slots = 0;
+    int rc = resident_blocks(K, precision, slots);
+    if (rc) return rc;
+    const long long items = static_cast<long long>((A.tiles_total + 3) / 4) * A.nl;
+    double best = 0.0;
+    for (int n = 1; n <= std::min(kMaxSplit, R); ++n) {
+        const long long waves = (items * n + slots - 1) / slots;
+        const double cost = static_cast<double>(waves) * ((R + n - 1) / n + 0.2);
+        if (n == 1 || cost < best * 0.999) {
+            best = cost;
+            nsplit = n;
+        }
+    }
+    return RBX_OK;
+}
+
+static size_t alpha_ws_bytes(const AlphaArgs &A, int nsplit) {
+    const size_t plane = static_cast<size_t>(A.nl) * A.nf;
+    return (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl) * sizeof(double);
+}
+
+template <int K>
+static int launch_alpha(AlphaArgs &A, int precision, int nsplit, hipStream_t st) {
     const int waves = kThreads / 64;
-    dim3 grid((A.tiles_total + waves - 1) / waves, A.nl);
+    dim3 grid((A.tiles_total + waves - 1) / waves, A.nl, nsplit);
     if (precision == RBX_FP32)
-        hipLaunchKernelGGL((alpha_kernel<K, true>), grid, dim3(kThreads), 0, st, A);
+        hipLaunchKernelGGL((alpha_lines_kernel<K, true>), grid, dim3(kThreads), 0, st, A);
     else
-        hipLaunchKernelGGL((alpha_kernel<K, false>), grid, dim3(kThreads), 0, st, A);
-    return launch_status("alpha_kernel");
+        hipLaunchKernelGGL((alpha_lines_kernel<K, false>), grid, dim3(kThreads), 0, st, A);
+    int rc = launch_status("alpha_lines_kernel");
+    if (rc) return rc;
+    dim3 g2((A.nf + kThreads - 1) / kThreads, A.nl);
+    if (precision == RBX_FP32)
+        hipLaunchKernelGGL((alpha_finish_kernel<true>), g2, dim3(kThreads), 0, st, A, nsplit);
+    else
+        hipLaunchKernelGGL((alpha_finish_kernel<false>), g2, dim3(kThreads), 0, st, A, nsplit);
+    return launch_status("alpha_finish_kernel");
+}
+
+// workspace of one rbx_alpha call: the tile width and split it will use
+static int plan_alpha(AlphaArgs &A, int precision, int &K, int &nsplit, size_t &bytes) {
+    K = alpha_tile_k();
+    set_wave_tiles(A, K);
+    int rc = alpha_split(A, K, precision, nsplit);
+    if (rc) return rc;
+    bytes = alpha_ws_bytes(A, nsplit);
+    return RBX_OK;
 }
 
 }  // namespace rbx
 
+extern "C" int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
+                                        int nf_nh3_low, int precision, size_t *bytes) {
+    RBX_CHECK_ARG(bytes, "rbx_alpha_workspace_size: null output");
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_workspace_size: bad precision %d",
+                  precision);
+    RBX_CHECK_ARG(cat && absorbers, "rbx_alpha_workspace_size: null catalogue or absorber selection");
+    RBX_CHECK_ARG(nl > 0 && nl <= 65535 && nf > 0, "rbx_alpha_workspace_size: nl=%d nf=%d", nl, nf);
+    RBX_CHECK_ARG(nf_nh3_low >= 0 && nf_nh3_low <= nf, "rbx_alpha_workspace_size: nf_nh3_low=%d", nf_nh3_low);
+    int rc = rbx::check_catalog(cat, *absorbers, "rbx_alpha_workspace_size");
+    if (rc) return rc;
+    rbx::AlphaArgs A{};
+    A.cat = cat->dev;
+    A.abs = *absorbers;
+    A.nl = nl;
+    A.nf = nf;
+    A.nf_low = nf_nh3_low;
+    int K = 1, nsplit = 1;
+    return rbx::plan_alpha(A, precision, K, nsplit, *bytes);
+}
+
 extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P, const double *T,
                          const double *mix, int nl, const double *freq, const double *freq_nh3, int nf,
                          int nf_nh3_low, const double *scale, int precision, void *alpha, double *by_constituent,
-                         void *stream) {
+                         void *workspace, size_t workspace_bytes, void *stream) {
     RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha: bad precision %d", precision);
     rbx::AlphaArgs A;
     int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
                             by_constituent, "rbx_alpha");
     if (rc) return rc;
+    int K = 1, nsplit = 1;
+    size_t need = 0;
+    rc = rbx::plan_alpha(A, precision, K, nsplit, need);
+    if (rc) return rc;
+    RBX_CHECK_ARG(workspace, "rbx_alpha: null workspace (rbx_alpha_workspace_size gives its size)");
+    RBX_CHECK_ARG(workspace_bytes >= need, "rbx_alpha: workspace of %zu bytes, this call needs %zu", workspace_bytes,
+                  need);
+    RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha: workspace not 8-byte aligned");
+    A.ws = static_cast<double *>(workspace);
+    A.ws_base = A.ws + static_cast<size_t>(rbx::kSlots) * nsplit * nl * nf;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    switch (rbx::alpha_tile_k(nf)) {
-        case 4: return rbx::launch_alpha<4>(A, precision, st);
-        case 2: return rbx::launch_alpha<2>(A, precision, st);
-        default: return rbx::launch_alpha<1>(A, precision, st);
+    switch (K) {
+        case 4: return rbx::launch_alpha<4>(A, precision, nsplit, st);
+        case 2: return rbx::launch_alpha<2>(A, precision, nsplit, st);
+        default: return rbx::launch_alpha<1>(A, precision, nsplit, st);
     }
 }
 

@@ -157,9 +157,21 @@ def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by
     out = torch.empty((nl, nf), dtype=torch.float32 if prec == _lib.FP32 else torch.float64, device=device())
     byc = torch.zeros((len(_lib.CONSTITUENTS), nl, nf), dtype=torch.float64, device=device()) \
         if by_constituent else None
+    ws, nbytes = alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec)
     check(lib().rbx_alpha(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), nl, _ptr(fd),
-                          _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(out), _ptr(byc), _stream()))
+                          _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(out), _ptr(byc), _ptr(ws), nbytes, _stream()))
     return (out, byc) if by_constituent else out
+
+
+def alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec, reuse=None):
+    """Device scratch for rbx_alpha (torch's stream-ordered allocator): (tensor, bytes).
+    `reuse` is returned instead of a new tensor when it is large enough."""
+    nb = ctypes.c_size_t(0)
+    check(lib().rbx_alpha_workspace_size(catalog.handle, ctypes.byref(absorbers), nl, nf, nf_low, prec,
+                                         ctypes.byref(nb)))
+    if reuse is not None and reuse.numel() * reuse.element_size() >= nb.value:
+        return reuse, nb.value
+    return torch.empty((max(1, (nb.value + 7) // 8),), dtype=torch.float64, device=device()), nb.value
 
 
 def alpha_tangent(catalog, absorbers, P, T, mix, freqs, scale=None):

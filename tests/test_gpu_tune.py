@@ -49,6 +49,31 @@ def test_alpha_every_tile_width(K, at, raw_lines, jupiter_lines, precision, tol)
         assert np.array_equal(out[k], out[1]), k
 
 
+@pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
+def test_alpha_every_split(K, at, raw_lines, jupiter_lines, precision, tol):
+    """Line sums split into 1..16 parts per layer (part boundaries fall inside the h2o and
+    nh3 inversion blocks and between segments): the parts add up to the unsplit table up to
+    FP64 reassociation, and every split matches the oracle.  Deep layers (to 3000 bar) take
+    the nh3_sjs / blend branches of nh3_hs_sjs in the finish kernel."""
+    atm = synth.jupiter_like(37, pmax=3000.0)
+    h2o = synth.h2o_lines(700, 9)
+    cat = K.jupiter_catalog(raw_lines, h2o=h2o)
+    lines = dict(jupiter_lines)
+    lines['h2o'] = h2o
+    f = np.logspace(np.log10(0.5), np.log10(250.0), 190)
+    ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(lines))
+    out = {}
+    for n in (1, 2, 3, 7, 16):
+        at.set_knob('alpha_split', n)
+        a, byc = K.alpha(cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), f,
+                         precision, by_constituent=True)
+        out[n] = a.double().cpu().numpy()
+        assert np.max(np.abs(out[n].T - ref) / np.abs(ref)) < tol, n
+        assert np.allclose(byc.sum(axis=0).cpu().numpy(), out[n], rtol=1e-6 if precision == 'fp32' else 1e-12)
+    for n in out:
+        assert np.max(np.abs(out[n] - out[1]) / np.abs(out[1])) < (1e-12 if precision == 'fp64' else 1e-6), n
+
+
 @pytest.mark.parametrize('precision', ['fp64', 'fp32'])
 def test_integrate_every_unroll(K, at, precision):
     """4 and 8 path steps per load batch: the same Tb to rounding (the compiler may contract

@@ -45,8 +45,21 @@ def test_argument_errors_are_reported_without_gpu_work():
     assert lib.rbx_catalog_lines(h, 0, ctypes.byref(n)) == 0 and n.value == 0
     ab = kernels.absorbers_struct({'nh3': 'nh3_hs'})
     dummy = ctypes.c_void_p(8)
-    rc = lib.rbx_alpha(h, ctypes.byref(ab), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy, None, None)
+    rc = lib.rbx_alpha(h, ctypes.byref(ab), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy, None,
+                       dummy, 0, None)
     assert rc == -4 and 'nh3_hs' in _lib.last_error()             # empty inversion block
+    # workspace: partial sums [4 slots][parts][nl][nf] + h2o base [parts][nl] doubles
+    nb = ctypes.c_size_t(0)
+    h2 = kernels.absorbers_struct({'h2': 'h2_jj_ddb'})
+    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, ctypes.byref(nb)) == 0
+    assert nb.value == (4 * 10 * 7 + 10) * 8                      # no lines: one part
+    try:
+        assert lib.rbx_tune(_lib.TUNE_ALPHA_SPLIT, 3) == 0
+        assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, ctypes.byref(nb)) == 0
+        assert nb.value == (4 * 3 * 10 * 7 + 3 * 10) * 8
+    finally:
+        lib.rbx_tune(_lib.TUNE_ALPHA_SPLIT, 0)
+    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 0, 7, 0, 1, ctypes.byref(nb)) == -1
     assert lib.rbx_catalog_destroy(h) == 0
 
 
