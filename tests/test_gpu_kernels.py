@@ -258,3 +258,74 @@ def test_jacobian_vs_central_differences(K, raw_lines):
         scale = np.max(np.abs(J[:, 1]), axis=1) + 1e-30
         assert np.max(np.abs(J[:, 1, l, :] - fdq) / scale) < 1e-3, l
     del torch
+
+
+@pytest.mark.parametrize('alpha_precision', ['fp64', 'fp32'])
+def test_jacobian_vs_torch_autograd(K, raw_lines, jupiter_lines, alpha_precision):
+    """GPU Jacobian rows vs float64 torch autograd of the restated forward model
+    (oracle.jacobian), with the oracle's own ray paths: <= 1e-3 of each row's scale."""
+    from oracle import jacobian as OJ
+    atm = synth.jupiter_like(60)
+    cat = K.jupiter_catalog(raw_lines)
+    ab = K.absorbers_struct(JUPITER_ALPHA, 'e')
+    f = np.array([1.5, 8.0, 22.0, 29.0, 31.0, 40.0, 50.0])
+    rays = np.array([[0.0, 0.0], [0.55, 0.2], [0.93, -0.1]])
+    mix = K.mix_table(atm.gas, C)
+    paths = []
+    for b in rays:
+        r = geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol)
+        paths.append((r.ds, r.layer4ds))
+    Tb_ref, J_ref = OJ.jacobian(f, atm.T, atm.P, mix, jupiter_lines, paths)
+    pl = K.planet_desc(atm.Req, atm.Rpol)
+    hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
+    tab = K.alpha(cat, ab, atm.P, atm.T, mix, f, alpha_precision)
+    dal = K.alpha_tangent(cat, ab, atm.P, atm.T, mix, f)
+    J, Tb = K.jacobian(tab, dal, atm.T, hds, ns)
+    J = J.double().cpu().numpy()
+    assert np.max(np.abs(Tb.cpu().numpy() - Tb_ref)) < (1e-8 if alpha_precision == 'fp64' else 1e-3)
+    for slot in (0, 1):
+        scale = np.max(np.abs(J_ref[:, slot]), axis=1, keepdims=True) + 1e-30    # (nray, 1, nf)
+        err = np.max(np.abs(J[:, slot] - J_ref[:, slot]) / scale)
+        assert err < 1e-3, (slot, err)
+
+
+@pytest.mark.parametrize('name', ['c2_image', 'c3_large'])
+def test_workload_full_shape_sampled(K, raw_lines, jupiter_lines, name):
+    """The image and large workloads at their full shapes (all layers, lines, freqs, rays on
+    the GPU), checked on a sample the oracle finishes quickly: 3 layers of alpha at every
+    frequency, and 4 rays integrated over the GPU FP64 table with the oracle's geometry."""
+    import torch
+    w = synth.workload(name)
+    atm = w['atm']
+    cat = K.jupiter_catalog(raw_lines, nh3_inversion=w['nh3_inv'], h2o=w['h2o'])
+    lines = dict(jupiter_lines)
+    inj = dict(lines['nh3_hs'])
+    for k in ('fo', 'Io', 'Eo', 'gammaNH3o'):
+        inj[k] = w['nh3_inv'][k]
+    lines['nh3_hs'] = inj
+    if w['h2o'] is not None:
+        lines['h2o'] = w['h2o']
+    ab = K.absorbers_struct(JUPITER_ALPHA, 'e')
+    mix = K.mix_table(atm.gas, C)
+    f = w['freqs']
+    a64 = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp64')
+    a32 = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp32')
+    sel = [0, atm.nl // 2, atm.nl - 1]
+    ref = oracle_alpha(atm.gas[:, sel], f, lines).T                      # (3, nf)
+    assert rel(a64.cpu().numpy()[sel], ref) < 1e-9
+    assert rel(a32.double().cpu().numpy()[sel], ref) < 1e-5
+    rays = w['rays']
+    pl = K.planet_desc(atm.Req, atm.Rpol)
+    hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
+    Tb = K.integrate(a64, atm.T, hds, ns, precision='fp64').cpu().numpy()
+    Tb32 = K.integrate(a32, atm.T, hds, ns, precision='fp32', tau_cut=50.0).cpu().numpy()
+    tab = a64.cpu().numpy().T                                            # (nf, nl)
+    nsn = ns.cpu().numpy()
+    on = np.flatnonzero(nsn > 0)
+    picks = [on[0], on[len(on) // 3], on[-1], int(np.flatnonzero(nsn < 0)[0])]
+    for i in picks:
+        r = geometry.compute_ds(atm.R, atm.N, rays[i], atm.Req, atm.Rpol)
+        tb = np.array(rt.single(tab, atm.T, r.ds, r.layer4ds))
+        assert np.max(np.abs(Tb[i] - tb)) < 1e-6, i
+        assert np.max(np.abs(Tb32[i] - tb)) < 0.01, i
+    del torch

@@ -238,3 +238,54 @@ def test_total_layer_alpha_scales():
     with pytest.raises(ValueError):
         layers.layer_scale([1.0, 2.0], 3, ['h2'])
     assert layers.layer_scale('nonsense', 2, ['h2']) == [1.0, 1.0]
+
+
+# ---- the float64 autograd restatement (gradient reference of the adjoint kernels) ----
+def _jac_case(jupiter_lines, nl=14):
+    from oracle import jacobian as J
+    atm = synth.jupiter_like(nl)
+    C = synth.GAS_ROWS
+    mix = np.stack([atm.gas[C[k]] for k in ('H2', 'HE', 'CH4', 'NH3', 'H2O', 'H2S', 'PH3')])
+    f = np.array([1.5, 10.0, 22.0, 30.0, 33.0, 45.0])
+    return J, atm, mix, f
+
+
+def test_autograd_forward_matches_oracle(jupiter_lines):
+    """The torch restatement's alpha and Tb equal the numpy oracle's (which the reference
+    fixtures pin), so its autograd derivatives are those of the pinned forward model."""
+    import torch
+    J, atm, mix, f = _jac_case(jupiter_lines)
+    C = synth.GAS_ROWS
+    ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(jupiter_lines))    # (nf, nl)
+    got = J.alpha_layers(f, torch.as_tensor(atm.T), torch.as_tensor(atm.P), torch.as_tensor(mix),
+                         jupiter_lines).numpy()
+    assert np.max(np.abs(got.T - ref) / np.abs(ref)) < 1e-12
+    for b in ([0.0, 0.0], [0.6, 0.2]):
+        ray = geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol)
+        tb_ref = rt.single(ref, atm.T, ray.ds, ray.layer4ds)
+        tb = J.brightness(torch.as_tensor(got), torch.as_tensor(atm.T), ray.ds, ray.layer4ds).numpy()
+        assert np.max(np.abs(tb - np.array(tb_ref))) < 1e-10
+
+
+def test_autograd_jacobian_vs_central_differences_of_oracle(jupiter_lines):
+    """Autograd dTb/dT[l], dTb/dq_NH3[l] vs central FD (h = 1e-5 x) of the numpy oracle."""
+    J, atm, mix, f = _jac_case(jupiter_lines)
+    C = synth.GAS_ROWS
+    aset = layers.jupiter_absorbers(jupiter_lines)
+    ray = geometry.compute_ds(atm.R, atm.N, [0.3, 0.1], atm.Req, atm.Rpol)
+    Tb, Jm = J.jacobian(f, atm.T, atm.P, mix, jupiter_lines, [(ray.ds, ray.layer4ds)])
+
+    def tb(gas):
+        tab = layers.get_layers(list(f), gas, C, aset)
+        return np.array(rt.single(tab, gas[C['T']], ray.ds, ray.layer4ds))
+
+    for l in (2, 7, 12):
+        for slot, row in ((0, C['T']), (1, C['NH3'])):
+            h = 1e-5 * atm.gas[row, l]
+            gp = atm.gas.copy()
+            gm = atm.gas.copy()
+            gp[row, l] += h
+            gm[row, l] -= h
+            fd = (tb(gp) - tb(gm)) / (2 * h)
+            scale = np.max(np.abs(Jm[0, slot]), axis=0) + 1e-30
+            assert np.max(np.abs(Jm[0, slot, l] - fd) / scale) < 1e-6, (l, slot)
