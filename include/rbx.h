@@ -160,14 +160,20 @@ int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
 int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
                              int nf_nh3_low, int precision, size_t *bytes);
 
-/* ---- absorption tangents for the adjoint ----
+/* ---- absorption and its tangents for the adjoint (one fused pass) ----
+ * Same inputs as rbx_alpha.  precision RBX_FP64 (plain FP64) or RBX_FP32
+ * (FP32 line sums as in rbx_alpha; tangents good to ~1e-6 relative).
+ * alpha           [nl][nf] float64 alpha, or NULL
  * dalpha          [3][nl][nf] float64: d alpha / d T, d alpha / d q_NH3,
  *                 d alpha / d q_H2O (q = mixing ratio), same layout as alpha.
+ * nl may be the total layer count of several atmospheres laid end to end
+ * (layers are independent here); see the *_batch entry points below.
  */
 int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers,
                       const double *P, const double *T, const double *mix, int nl,
                       const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
-                      const double *scale, double *dalpha, void *stream);
+                      const double *scale, int precision, double *alpha, double *dalpha,
+                      void *stream);
 
 /* ---- ray geometry (FP64) ----
  * R, N            [nl] layer radii (km) and refractive indices, top layer first
@@ -212,6 +218,31 @@ int rbx_integrate_profiles(const void *alpha, int alpha_precision, const double 
 int rbx_jacobian(const void *alpha, int alpha_precision, const double *dalpha, const double *T,
                  int nl, int nf, const double *hds, int ld, const int *nsteps, int nray,
                  double tau_cut, float *J, double *Tb, void *stream);
+
+/* ---- many atmospheres in one launch (no reference counterpart; the retrieval
+ * batch: Planet.run over a list of perturbed atmospheres, planet_base.py:209-221)
+ * Per-layer arrays (R, N, P, T, mix rows, alpha, dalpha rows) of natm atmospheres
+ * are laid end to end: atmosphere a owns layers [layer_off[a], layer_off[a+1])
+ * of nl_total (layer_off: device int[natm + 1]); rbx_alpha / rbx_alpha_tangent
+ * take nl = nl_total as they are.  ray_atm [nray] (device) gives each ray's
+ * atmosphere; order [nray] (device, or NULL for 0..nray-1) is the order rays are
+ * scheduled in (longest paths first balances uneven atmospheres); results are
+ * written at each ray's own index either way.  A ray whose atmosphere index or
+ * offsets are out of range gets nsteps = -2 and Tb = NaN.
+ */
+int rbx_raypaths_batch(const rbx_planet *planet, const double *R, const double *N, int nl_total,
+                       const int *layer_off, int natm, const double *b, int nray, const int *ray_atm,
+                       int out_units, double *ds, int ld, int *nsteps, void *stream);
+int rbx_integrate_batch(const void *alpha, int alpha_precision, const double *T, int nl_total, int nf,
+                        const int *layer_off, int natm, const double *hds, int ld, const int *nsteps,
+                        const int *ray_atm, const int *order, int nray, int mode, int precision,
+                        double tau_cut, double *Tb, double *intW, void *stream);
+/* J [nray][2][ldj][nf] float32, ldj >= the layer count of every atmosphere (rows
+ * past a ray's atmosphere are 0); precision as rbx_integrate's (e^-tau evaluation). */
+int rbx_jacobian_batch(const void *alpha, int alpha_precision, const double *dalpha, const double *T,
+                       int nl_total, int nf, const int *layer_off, int natm, const double *hds, int ld,
+                       const int *nsteps, const int *ray_atm, const int *order, int nray, int precision,
+                       double tau_cut, float *J, int ldj, double *Tb, void *stream);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,8 @@ TUNE_ALPHA_SPLIT = 2
 
 EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_tune', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
-           'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles', 'rbx_jacobian')
+           'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles',
+           'rbx_jacobian', 'rbx_raypaths_batch', 'rbx_integrate_batch', 'rbx_jacobian_batch')
 
 
 class RbxLibraryError(RuntimeError):
@@ -75,11 +76,17 @@ _SIGS = {
     'rbx_alpha_workspace_size': (_I, [_VP, ctypes.POINTER(Absorbers), _I, _I, _I, _I,
                                       ctypes.POINTER(ctypes.c_size_t)]),
     'rbx_alpha_tangent': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
-                               _VP, _VP, _VP]),
+                               _VP, _I, _VP, _VP, _VP]),
     'rbx_raypaths': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _I, _VP, _I, _VP, _VP]),
     'rbx_integrate': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _VP]),
     'rbx_integrate_profiles': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
     'rbx_jacobian': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP]),
+    'rbx_raypaths_batch': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _VP, _I, _VP, _I, _VP, _I,
+                                _VP, _VP]),
+    'rbx_integrate_batch': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _I, _I, _I, _D, _VP,
+                                 _VP, _VP]),
+    'rbx_jacobian_batch': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _I, _I, _D, _VP,
+                                _I, _VP, _VP]),
 }
 
 _lib = None

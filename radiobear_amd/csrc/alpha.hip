@@ -21,18 +21,20 @@
 //   lines are summed with per-(line, frequency) coefficients; it is only used
 //   below 400 bar (nh3_hs_sjs.py:14-25).
 //
-// Grid: blockIdx.y = layer, blockIdx.x = frequency tile (tiles never straddle
-// the 30 GHz switch of nh3_hs).  Each 256-thread block stages 256 lines at a
-// time in LDS: thread t derives line t's canonical coefficients for this
-// layer (FP64, with exp/pow), then every thread sums all staged lines for its
-// K frequencies, reading the coefficients as LDS broadcasts.
+// Forward path: the line-sum kernel's grid is (layer x group of four 64K-column
+// wavefront tiles, part of the line list); a tile never straddles the 30 GHz
+// switch of nh3_hs.  Each 256-thread block stages 256 lines at a time in LDS:
+// thread t derives line t's canonical coefficients for this layer (FP64, with
+// exp/pow), then every thread sums all staged lines for its K frequencies,
+// reading the coefficients as LDS broadcasts.  A finish kernel adds the parts
+// and the per-layer closed forms.
 //
 // FP64 mode sums in FP64 with IEEE division.  FP32 mode: the detuning x - D
 // is formed in FP64 (cancellation near line centre) and rounded, the rest of
-// the term is FP32 with the hardware reciprocal, partial sums of 16 lines are
-// FP32 and are accumulated in FP64 (compensated, blocked summation).  Line
-// weights are scaled per layer and block by a power of two so FP32 never sees
-// the 1e-39..1e-19 raw intensities.
+// the term is FP32 with the hardware reciprocal (two lines per reciprocal),
+// partial sums of kSub lines are FP32 and are accumulated in FP64 (blocked
+// summation).  Line weights are scaled per layer and block by a power of two
+// so FP32 never sees the 1e-39..1e-19 raw intensities.
 //
 // The tangent kernel runs the same coefficient code on dual numbers
 // (value, d/dT, d/dq_NH3, d/dq_H2O) and differentiates the canonical term in
@@ -93,6 +95,7 @@ struct AlphaArgs {
     int tiles_low;     // tiles covering the columns < nf_low
     int tiles_total;   // forward kernel: tiles of both regimes
     int tile_width;    // columns per tile
+    double *value;     // tangent: alpha [nl][nf] (float64) or null
     double *ws;        // forward: partial line sums [kSlots][nsplit][nl][nf]
     double *ws_base;   // forward: partial h2o base terms [nsplit][nl]
 };
@@ -239,26 +242,18 @@ struct Staged32 {
     double D[2][kThreads];
     float4 q[2][kThreads];  // a, c, g, G
 };
-struct StagedTan {       // value and two tangent slots (d/dT, d/dq_own) of D, a, c, g, G
-    double v[5][kThreads];
-    double d[2][5][kThreads];
+template <int N>
+struct SjsLines {
+    double f0[N], w[N], g0[N];
+    double dw[kNd][N];
 };
-struct StagedSjs {
-    double f0[kThreads], w[kThreads], g0[kThreads];
-    double dw[kNd][kThreads];
-};
+using StagedSjs = SjsLines<kThreads>;
 
 union SharedFwd {
     Staged64 s64;
     Staged32 s32;
     StagedSjs sjs;
     double red[kThreads];
-};
-
-union SharedTan {
-    StagedTan st;
-    StagedSjs sjs;
-    double red[kNd + 1][kThreads];
 };
 
 // ---- forward line sum over lines [lb, le) of one block for K frequencies per lane ----
@@ -350,47 +345,6 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, int mine
     }
 }
 
-// ---- tangent line sum: value and the two tangent slots (T, q_own) ----
-template <int K, class Prep>
-__device__ void line_sum_tan(const DevBlock &B, const Prep &prep, int qslot, const double (&x)[K],
-                             double (&acc)[K], double (&dacc)[2][K], SharedTan &sh, Dual &base) {
-    for (int l0 = 0; l0 < B.n; l0 += kThreads) {
-        const int i = l0 + threadIdx.x;
-        if (i < B.n) {
-            const CoefT<Dual> q = prep(B, i, base);
-            const Dual *c5[5] = {&q.D, &q.a, &q.c, &q.g, &q.G};
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                sh.st.v[m][threadIdx.x] = c5[m]->v;
-                sh.st.d[0][m][threadIdx.x] = c5[m]->d[0];
-                sh.st.d[1][m][threadIdx.x] = c5[m]->d[qslot];
-            }
-        }
-        __syncthreads();
-        const int cnt = min(kThreads, B.n - l0);
-        for (int j = 0; j < cnt; ++j) {
-            const double D = sh.st.v[0][j], a = sh.st.v[1][j], c = sh.st.v[2][j];
-            const double g = sh.st.v[3][j], G = sh.st.v[4][j];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const double d = x[k] - D;
-                const double num = fma(a, x[k], c);
-                const double den = fma(d, d, fma(g, x[k], G));
-                const double r = 1.0 / den;
-                const double t = num * r;
-                acc[k] += t;
-#pragma unroll
-                for (int m = 0; m < 2; ++m) {
-                    const double dN = fma(sh.st.d[m][1][j], x[k], sh.st.d[m][2][j]);
-                    const double dQ = fma(-2.0 * d, sh.st.d[m][0][j], fma(sh.st.d[m][3][j], x[k], sh.st.d[m][4][j]));
-                    dacc[m][k] += (dN - t * dQ) * r;
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // power of two s with |w s| ~ 1 (FP32 range for the line weights)
 __device__ inline double pow2_scale(double w) {
     if (!(w > 0.0) || !isfinite(w)) return 1.0;
@@ -409,12 +363,6 @@ __device__ inline double block_reduce_sum(double v, double *red) {
     return r;
 }
 
-__device__ inline Dual block_reduce_sum(const Dual &v, SharedTan &sh) {
-    Dual r;
-    r.v = block_reduce_sum(v.v, sh.red[0]);
-    for (int m = 0; m < kNd; ++m) r.d[m] = block_reduce_sum(v.d[m], sh.red[m + 1]);
-    return r;
-}
 
 // nh3_sjs Spilker coefficient set (nh3_sjs.py:52-77): GH2, GHe, GNH3, ZH2, ZHe, ZNH3, D
 template <class S>
@@ -439,9 +387,12 @@ __device__ inline void sjs_spilker(const S &T, const S &Ph2, S *c) {
     }
 }
 
-// nh3_sjs for this thread's K frequencies (FP64; S = double or Dual)
-template <int K, class S, class Sh>
-__device__ void nh3_sjs(const DevBlock &B, const LayerT<S> &L, const double (&f)[K], S (&out)[K], Sh &sh) {
+// nh3_sjs for this thread's K frequencies (FP64; S = double or Dual).  WAVE:
+// each wavefront stages and sums its own 64 lines at a time in its own staging
+// buffer `sg` (wavefronts of a block may be on different layers); otherwise the
+// whole block stages kThreads lines per round in `sg`.
+template <int K, bool WAVE = false, class S, class Stage>
+__device__ void nh3_sjs(const DevBlock &B, const LayerT<S> &L, const double (&f)[K], S (&out)[K], Stage &sg) {
     const double T0 = 296.0;
     const double J[7] = {1.690, 0.750, 0.6, 1.350, 0.300, 0.200, -0.45};
     S Sp[7];
@@ -469,27 +420,32 @@ __device__ void nh3_sjs(const DevBlock &B, const LayerT<S> &L, const double (&f)
         x[k] = f[k] * f[k];
         sum[k] = 0.0;
     }
-    for (int l0 = 0; l0 < B.n; l0 += kThreads) {
-        const int i = l0 + threadIdx.x;
+    constexpr int chunk = WAVE ? 64 : kThreads;
+    const int me = WAVE ? threadIdx.x % 64 : threadIdx.x;
+    for (int l0 = 0; l0 < B.n; l0 += chunk) {
+        const int i = l0 + me;
         if (i < B.n) {
             const double f0 = B.col[0][i];
             const S w = B.col[1][i] * exp(eT * B.col[2][i]) / (f0 * f0);
-            sh.sjs.f0[threadIdx.x] = f0;
-            sh.sjs.w[threadIdx.x] = val(w);
+            sg.f0[me] = f0;
+            sg.w[me] = val(w);
             if constexpr (std::is_same<S, Dual>::value) {
-                for (int m = 0; m < kNd; ++m) sh.sjs.dw[m][threadIdx.x] = w.d[m];
+                for (int m = 0; m < kNd; ++m) sg.dw[m][me] = w.d[m];
             }
-            sh.sjs.g0[threadIdx.x] = B.col[3][i];
+            sg.g0[me] = B.col[3][i];
         }
-        __syncthreads();
-        const int cnt = min(kThreads, B.n - l0);
+        if (WAVE)
+            wave_sync();
+        else
+            __syncthreads();
+        const int cnt = min(chunk, B.n - l0);
         for (int j = 0; j < cnt; ++j) {
-            const double f0 = sh.sjs.f0[j], g0 = sh.sjs.g0[j];
+            const double f0 = sg.f0[j], g0 = sg.g0[j];
             S w;
             if constexpr (std::is_same<S, Dual>::value)
-                w = Dual(sh.sjs.w[j], sh.sjs.dw[0][j], sh.sjs.dw[1][j], sh.sjs.dw[2][j]);
+                w = Dual(sg.w[j], sg.dw[0][j], sg.dw[1][j], sg.dw[2][j]);
             else
-                w = sh.sjs.w[j];
+                w = sg.w[j];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 S gam = Ag[k] + Bg[k] * g0;
@@ -502,7 +458,10 @@ __device__ void nh3_sjs(const DevBlock &B, const LayerT<S> &L, const double (&f)
                 sum[k] = sum[k] + w * num / den;
             }
         }
-        __syncthreads();
+        if (WAVE)
+            wave_sync();
+        else
+            __syncthreads();
     }
     const S pref = kGeisa * (L.nh3 / T0) * pow(T0 / L.T, 3.5) * (kGHz * 2.0 / kPi) * (1.0 + L.P / 1.0E5);
 #pragma unroll
@@ -535,20 +494,6 @@ __device__ inline S h2o_continuum(const LayerT<S> &L) {
     const S w = L.h2o / 0.001;
     return (4.342945 * (Cf_he * L.he * L.h2o * Th3 + Cf_h2 * L.h2 * L.h2o * Th3) + 4.342945 * Cs1 * (w * w)) /
            kDbPerInvCm;
-}
-
-struct Tile {
-    int layer, f_begin, f_end;
-    bool hi;
-};
-
-__device__ inline Tile tile_of(const AlphaArgs &A) {
-    Tile t;
-    t.layer = blockIdx.y;
-    t.hi = static_cast<int>(blockIdx.x) >= A.tiles_low;
-    t.f_begin = t.hi ? A.nf_low + (blockIdx.x - A.tiles_low) * A.tile_width : blockIdx.x * A.tile_width;
-    t.f_end = t.hi ? A.nf : A.nf_low;
-    return t;
 }
 
 __device__ inline LayerT<double> layer_of(const AlphaArgs &A, int layer) {
@@ -609,14 +554,16 @@ __device__ inline bool nh3_uses_sjs(int sel, double P) { return sel == 3 || (sel
 template <int K, bool F32>
 __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
     __shared__ SharedFwd sh;
-    const int layer = blockIdx.y, part = blockIdx.z, nsplit = gridDim.z;
+    const int bpl = (A.tiles_total + kThreads / 64 - 1) / (kThreads / 64);   // blocks per layer
+    const int layer = blockIdx.x / bpl, bx = blockIdx.x % bpl;
+    const int part = blockIdx.y, nsplit = gridDim.y;
     const LayerT<double> L = layer_of(A, layer);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int t = blockIdx.x * (kThreads / 64) + wave;
+    const int t = bx * (kThreads / 64) + wave;
     const int regime = wave_regime(A, t);                 // -1: no columns for this wavefront
     int need_inv = 0;
     for (int w = 0; w < kThreads / 64; ++w) {
-        const int r = wave_regime(A, blockIdx.x * (kThreads / 64) + w);
+        const int r = wave_regime(A, bx * (kThreads / 64) + w);
         if (r >= 0) need_inv |= 1 << r;
     }
     const int mine = regime >= 0 ? 0 : -1;
@@ -723,16 +670,17 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         for (int k = 0; k < K; ++k)
             if (ok[k]) w[col[k]] = acc[q][k];
     }
-    if (A.abs.h2o && blockIdx.x == 0 && threadIdx.x == 0) A.ws_base[static_cast<size_t>(part) * A.nl + layer] = base;
+    if (A.abs.h2o && bx == 0 && threadIdx.x == 0) A.ws_base[static_cast<size_t>(part) * A.nl + layer] = base;
 }
 
 template <bool F32>
 __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int nsplit) {
     __shared__ SharedFwd sh;
-    const int layer = blockIdx.y;
+    const int bpl = (A.nf + kThreads - 1) / kThreads;
+    const int layer = blockIdx.x / bpl;
     const int nl = A.nl;
     const LayerT<double> L = layer_of(A, layer);
-    const int col = blockIdx.x * kThreads + threadIdx.x;
+    const int col = (blockIdx.x % bpl) * kThreads + threadIdx.x;
     const bool ok = col < A.nf;
     const int cc = ok ? col : A.nf - 1;
     const double f[1] = {A.freq[cc]};
@@ -767,7 +715,7 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
             hs = xn * parts(kSlotNh3);
             if (hs < 0.0) hs = 1.0E-8;   // nh3_hs.py:337-338
         }
-        if (nh3_uses_sjs(sel, L.P)) nh3_sjs<1>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh);   // block-uniform
+        if (nh3_uses_sjs(sel, L.P)) nh3_sjs<1>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh.sjs);   // block-uniform
         emit(RBX_C_NH3, nh3_select(sel, L.P, hs, sj[0]));
     }
     if (A.abs.ph3) emit(RBX_C_PH3, x * parts(kSlotPh3));
@@ -778,12 +726,165 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
         static_cast<double *>(A.alpha)[o] = tot;
 }
 
-// ---- tangent kernel: dalpha[3][nl][nf] = d alpha / d(T, q_NH3, q_H2O) ----
-template <int K>
-__global__ __launch_bounds__(kThreads) void alpha_tangent_kernel(AlphaArgs A) {
-    __shared__ SharedTan sh;
-    const Tile tl = tile_of(A);
-    const int layer = tl.layer;
+// ---- tangent kernel: alpha and dalpha[3][nl][nf] = d alpha / d(T, q_NH3, q_H2O) ----
+//
+// Work item = (layer, tile of 64 consecutive columns), one per wavefront; the
+// wavefronts of a block are independent (they may be on different layers), so
+// each stages its own lines: lane l derives line l0 + l's coefficients and
+// their (T, q_own) tangents on dual numbers, writes them to the wavefront's LDS
+// buffer, and all 64 lanes sum the 64 staged lines.  A tile may straddle the
+// nh3_hs 30 GHz switch: the inversion lines are then staged with both
+// coefficient sets and each lane reads its own.  Per line and column:
+//   t = N/Q, N = a x + c, Q = (x - D)^2 + g x + G,
+//   dt = (dN - t dQ)/Q, dN = da x + dc, dQ = -2 (x - D) dD + dg x + dG.
+// FP32 mode stores the staged values in FP32 (FP64 D), scales a, c and their
+// tangents by the layer's power of two, forms x - D in FP64 and sums blocks of
+// kSubT lines in FP32 before accumulating in FP64; FP64 mode is plain FP64.
+constexpr int kWaveLines = 64;
+constexpr int kSetPad = 4;   // moves set 1 to other LDS banks than set 0
+constexpr int kSubT = 4;
+
+template <bool F32>
+struct TanLines;
+template <>
+struct TanLines<true> {
+    double D[2][kWaveLines + kSetPad];
+    float4 q[2][kWaveLines + kSetPad];       // a s, c s, g, G
+    float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s, c s, g, G)
+    float dD[2][2][kWaveLines + kSetPad];    // [set][slot] dD
+};
+template <>
+struct TanLines<false> {
+    double v[2][5][kWaveLines + kSetPad];      // [set] D, a, c, g, G
+    double d[2][2][5][kWaveLines + kSetPad];   // [set][slot]
+};
+
+// each wavefront's own staging buffer (wavefronts of a block run independently)
+template <bool F32>
+union WaveTan {
+    TanLines<F32> lines;
+    SjsLines<64> sjs;
+};
+
+template <bool F32>
+struct SharedTan {
+    WaveTan<F32> w[kThreads / 64];
+};
+
+// value + (T, q_own) tangent sums of one line block for this lane's column.
+// prep[0..1]: coefficient sets (two only for the nh3_hs inversion lines);
+// `need` = sets this wavefront's lanes use, `mine` = this lane's set;
+// scale[s] = FP32 power-of-two weight scale of set s (1 in FP64 mode).
+template <bool F32, class Prep>
+__device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int mine, const double *scale,
+                             int qslot, double x, double &acc, double (&dacc)[2], TanLines<F32> &st, Dual &base) {
+    const int lane = threadIdx.x % 64;
+    const float xf = static_cast<float>(x);
+    for (int l0 = 0; l0 < B.n; l0 += kWaveLines) {
+        const int i = l0 + lane;
+        const int cnt = min(kWaveLines, B.n - l0);
+        const int cntp = (cnt + kSubT - 1) / kSubT * kSubT;
+        if (lane < cntp) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!(need & (1 << s))) continue;
+                CoefT<Dual> q = {Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)};   // padding: t = 0
+                if (i < B.n) q = prep[s](B, i, base);
+                const int sl[2] = {0, qslot};
+                if constexpr (F32) {
+                    const double sc = scale[s];
+                    st.D[s][lane] = q.D.v;
+                    st.q[s][lane] = make_float4(static_cast<float>(q.a.v * sc), static_cast<float>(q.c.v * sc),
+                                                static_cast<float>(q.g.v), static_cast<float>(q.G.v));
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        const int d = sl[m];
+                        st.dq[s][m][lane] =
+                            make_float4(static_cast<float>(q.a.d[d] * sc), static_cast<float>(q.c.d[d] * sc),
+                                        static_cast<float>(q.g.d[d]), static_cast<float>(q.G.d[d]));
+                        st.dD[s][m][lane] = static_cast<float>(q.D.d[d]);
+                    }
+                } else {
+                    const Dual *c5[5] = {&q.D, &q.a, &q.c, &q.g, &q.G};
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        st.v[s][c][lane] = c5[c]->v;
+                        st.d[s][0][c][lane] = c5[c]->d[0];
+                        st.d[s][1][c][lane] = c5[c]->d[qslot];
+                    }
+                }
+            }
+        }
+        wave_sync();
+        if constexpr (F32) {
+            for (int j0 = 0; j0 < cntp; j0 += kSubT) {
+                float p = 0.0f, dp[2] = {0.0f, 0.0f};
+#pragma unroll
+                for (int u = 0; u < kSubT; ++u) {
+                    const int j = j0 + u;
+                    const float d = static_cast<float>(x - st.D[mine][j]);
+                    const float4 q = st.q[mine][j];
+                    const float N = fmaf(q.x, xf, q.y);
+                    const float Q = fmaf(d, d, fmaf(q.z, xf, q.w));
+                    const float r = __builtin_amdgcn_rcpf(Q);
+                    const float t = N * r;
+                    p += t;
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        const float4 dq = st.dq[mine][m][j];
+                        const float dN = fmaf(dq.x, xf, dq.y);
+                        const float dQ = fmaf(-2.0f * d, st.dD[mine][m][j], fmaf(dq.z, xf, dq.w));
+                        dp[m] = fmaf(fmaf(-t, dQ, dN), r, dp[m]);
+                    }
+                }
+                acc += static_cast<double>(p);
+                dacc[0] += static_cast<double>(dp[0]);
+                dacc[1] += static_cast<double>(dp[1]);
+            }
+        } else {
+            for (int j = 0; j < cntp; ++j) {
+                const double D = st.v[mine][0][j], a = st.v[mine][1][j], c = st.v[mine][2][j];
+                const double g = st.v[mine][3][j], G = st.v[mine][4][j];
+                const double d = x - D;
+                const double N = fma(a, x, c);
+                const double r = 1.0 / fma(d, d, fma(g, x, G));
+                const double t = N * r;
+                acc += t;
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const double *dd = &st.d[mine][m][0][0];
+                    const int S = kWaveLines + kSetPad;
+                    const double dN = fma(dd[1 * S + j], x, dd[2 * S + j]);
+                    const double dQ = fma(-2.0 * d, dd[j], fma(dd[3 * S + j], x, dd[4 * S + j]));
+                    dacc[m] += (dN - t * dQ) * r;
+                }
+            }
+        }
+        wave_sync();
+    }
+}
+
+__device__ inline Dual wave_sum(const Dual &v) {
+    Dual r;
+    r.v = wave_sum(v.v);
+    for (int m = 0; m < kNd; ++m) r.d[m] = wave_sum(v.d[m]);
+    return r;
+}
+
+template <bool F32>
+__device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan<F32> &sh) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const long long item = static_cast<long long>(blockIdx.x) * (kThreads / 64) + wave;
+    const int tiles = (A.nf + 63) / 64;
+    if (item >= static_cast<long long>(A.nl) * tiles) return;   // the whole wavefront: no block barriers follow
+    const int layer = static_cast<int>(item / tiles);
+    const int col = static_cast<int>(item % tiles) * 64 + lane;
+    const bool ok = col < A.nf;
+    const int cc = ok ? col : A.nf - 1;
+    const int mine = cc < A.nf_low ? 0 : 1;                      // nh3_hs set of this lane's column
+    const int need = (__any(mine == 0) ? 1 : 0) | (__any(mine == 1) ? 2 : 0);
+    TanLines<F32> &st = sh.w[wave].lines;
+
     const LayerT<double> L0 = layer_of(A, layer);
     const int nl = A.nl;
     LayerT<Dual> L;
@@ -796,116 +897,90 @@ __global__ __launch_bounds__(kThreads) void alpha_tangent_kernel(AlphaArgs A) {
     L.h2o = Dual(L0.h2o, 0.0, 0.0, L0.P);
     L.h2s = L0.h2s;
     L.ph3 = L0.ph3;
+    const double f[1] = {A.freq[cc]};
+    const double fn = A.freq_nh3 ? A.freq_nh3[cc] : f[0];
+    const double x = f[0] * f[0], xn = fn * fn;
 
-    int col[K];
-    bool ok[K];
-    double f[K], x[K], xn[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        col[k] = tl.f_begin + threadIdx.x + k * kThreads;
-        ok[k] = col[k] < tl.f_end;
-        f[k] = ok[k] ? A.freq[col[k]] : 1.0;
-        const double fn = ok[k] ? (A.freq_nh3 ? A.freq_nh3[col[k]] : f[k]) : 1.0;
-        x[k] = f[k] * f[k];
-        xn[k] = fn * fn;
-    }
-    Dual res[RBX_NUM_CONSTITUENTS][K];
-    Dual unused;
-
-    // a line block's value + (T, q_own) tangents, times x
-    auto finish = [&](const double (&acc)[K], const double (&dacc)[2][K], int qslot, const double (&xx)[K],
-                      Dual (&out)[K]) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            Dual r(xx[k] * acc[k]);
-            r.d[0] = xx[k] * dacc[0][k];
-            r.d[qslot] = xx[k] * dacc[1][k];
-            out[k] = out[k] + r;
-        }
+    // total_layer_alpha (alpha.py:151-192): constituents scale-summed as each completes
+    Dual tot, unused;
+    auto emit = [&](int c, const Dual &v) { tot = tot + v * (A.scale ? A.scale[c * nl + layer] : 1.0); };
+    // one line block: x (sum / scale) with the value, d/dT and d/dq_own parts, added to out
+    auto lines = [&](const DevBlock &B, const auto *prep, int nset, int qslot, double xx, Dual &out, Dual &base) {
+        double s[2] = {1.0, 1.0};
+        if (F32)
+            for (int k = 0; k < nset; ++k) s[k] = pow2_scale(prep[k].pref.v * B.wscale);
+        double acc = 0.0, dacc[2] = {0.0, 0.0};
+        const int nd = nset == 2 ? need : 1, me = nset == 2 ? mine : 0;
+        line_sum_tan<F32>(B, prep, nd, me, s, qslot, xx, acc, dacc, st, base);
+        Dual r(xx * acc / s[me]);
+        r.d[0] = xx * dacc[0] / s[me];
+        r.d[qslot] = xx * dacc[1] / s[me];
+        out = out + r;
     };
 
-    if (A.abs.h2) {
-        const Dual Kh = h2_cia(L, A.abs.h2);
-#pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_H2][k] = Kh * x[k];
-    }
+    if (A.abs.h2) emit(RBX_C_H2, h2_cia(L, A.abs.h2) * x);
     if (A.abs.h2o) {
-        const DevBlock &B = A.cat.blk[RBX_LINES_H2O];
-        H2oPrep<Dual> prep(L);
-        double acc[K] = {}, dacc[2][K] = {};
-        Dual base;
-        line_sum_tan<K>(B, prep, 2, x, acc, dacc, sh, base);
-        base = block_reduce_sum(base, sh);
-        const Dual Kc = base + h2o_continuum(L);
-#pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_H2O][k] = Kc * x[k];
-        finish(acc, dacc, 2, x, res[RBX_C_H2O]);
+        Dual v, base;
+        {
+            const H2oPrep<Dual> prep[1] = {H2oPrep<Dual>(L)};
+            lines(A.cat.blk[RBX_LINES_H2O], prep, 1, 2, x, v, base);
+        }
+        emit(RBX_C_H2O, v + (wave_sum(base) + h2o_continuum(L)) * x);
     }
     if (A.abs.h2s) {
-        const DevBlock &B = A.cat.blk[RBX_LINES_H2S];
-        H2sPrep<Dual> prep(L);
-        double acc[K] = {}, dacc[2][K] = {};
-        line_sum_tan<K>(B, prep, 1, x, acc, dacc, sh, unused);
-        finish(acc, dacc, 1, x, res[RBX_C_H2S]);
+        Dual v;
+        const H2sPrep<Dual> prep[1] = {H2sPrep<Dual>(L)};
+        lines(A.cat.blk[RBX_LINES_H2S], prep, 1, 1, x, v, unused);
+        emit(RBX_C_H2S, v);
     }
     if (A.abs.nh3) {
         const int sel = A.abs.nh3;
-        const bool use_hs = sel == 2 || (sel == 1 && L.P <= 2000.0);
-        const bool use_sjs = sel == 3 || (sel == 1 && L.P >= 400.0);
-        Dual hs[K], sj[K];
-        if (use_hs) {
+        Dual hs, sj[1];
+        if (nh3_uses_hs(sel, L0.P)) {
             {
-                double acc[K] = {}, dacc[2][K] = {};
-                InvPrep<Dual> prep(L, tl.hi ? 1 : 0);
-                line_sum_tan<K>(A.cat.blk[RBX_LINES_NH3_INV], prep, 1, xn, acc, dacc, sh, unused);
-                finish(acc, dacc, 1, xn, hs);
+                const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, 0), InvPrep<Dual>(L, 1)};
+                lines(A.cat.blk[RBX_LINES_NH3_INV], prep, 2, 1, xn, hs, unused);
             }
             {
-                double acc[K] = {}, dacc[2][K] = {};
-                RotPrep<Dual> prep(L);
-                line_sum_tan<K>(A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, xn, acc, dacc, sh, unused);
-                finish(acc, dacc, 1, xn, hs);
+                const RotPrep<Dual> prep[1] = {RotPrep<Dual>(L)};
+                lines(A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, 1, xn, hs, unused);
             }
             {
-                double acc[K] = {}, dacc[2][K] = {};
-                V2Prep<Dual> prep(L);
-                line_sum_tan<K>(A.cat.blk[RBX_LINES_NH3_V2], prep, 1, xn, acc, dacc, sh, unused);
-                finish(acc, dacc, 1, xn, hs);
+                const V2Prep<Dual> prep[1] = {V2Prep<Dual>(L)};
+                lines(A.cat.blk[RBX_LINES_NH3_V2], prep, 1, 1, xn, hs, unused);
             }
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (hs[k].v < 0.0) hs[k] = Dual(1.0E-8);   // nh3_hs.py:337-338
+            if (hs.v < 0.0) hs = Dual(1.0E-8);   // nh3_hs.py:337-338
         }
-        if (use_sjs) nh3_sjs<K>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh);
-#pragma unroll
-        for (int k = 0; k < K; ++k) res[RBX_C_NH3][k] = nh3_select(sel, L.P, hs[k], sj[k]);
+        if (nh3_uses_sjs(sel, L0.P)) nh3_sjs<1, true>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh.w[wave].sjs);
+        emit(RBX_C_NH3, nh3_select(sel, L0.P, hs, sj[0]));
     }
     if (A.abs.ph3) {
-        const DevBlock &B = A.cat.blk[RBX_LINES_PH3];
-        Ph3Prep<Dual> prep(L);
-        double acc[K] = {}, dacc[2][K] = {};
-        line_sum_tan<K>(B, prep, 1, x, acc, dacc, sh, unused);
         // ph3 does not depend on q_NH3: its slot-1 tangent is identically zero
-        finish(acc, dacc, 1, x, res[RBX_C_PH3]);
+        Dual v;
+        const Ph3Prep<Dual> prep[1] = {Ph3Prep<Dual>(L)};
+        lines(A.cat.blk[RBX_LINES_PH3], prep, 1, 1, x, v, unused);
+        emit(RBX_C_PH3, v);
     }
 
-    double sc[RBX_NUM_CONSTITUENTS];
-#pragma unroll
-    for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c) sc[c] = A.scale ? A.scale[c * nl + layer] : 1.0;
-    const int en[RBX_NUM_CONSTITUENTS] = {A.abs.h2, A.abs.h2o, A.abs.h2s, A.abs.nh3, A.abs.ph3};
-    double *out = static_cast<double *>(A.alpha);
+    if (!ok) return;
     const size_t plane = static_cast<size_t>(nl) * A.nf;
+    const size_t o = static_cast<size_t>(layer) * A.nf + col;
+    double *out = static_cast<double *>(A.alpha);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (!ok[k]) continue;
-        Dual tot;
-#pragma unroll
-        for (int c = 0; c < RBX_NUM_CONSTITUENTS; ++c)
-            if (en[c]) tot = tot + res[c][k] * sc[c];
-        const size_t o = static_cast<size_t>(layer) * A.nf + col[k];
-#pragma unroll
-        for (int m = 0; m < kNd; ++m) out[m * plane + o] = tot.d[m];
-    }
+    for (int m = 0; m < kNd; ++m) out[m * plane + o] = tot.d[m];
+    if (A.value) A.value[o] = tot.v;
+}
+
+// FP32: ask for 3 wavefronts per SIMD (the staged-line buffers take 35 KB per
+// block); FP64's 65 KB of staging allows 2 blocks per CU whatever the registers.
+__global__ __launch_bounds__(kThreads, 3) void alpha_tangent_kernel_f32(AlphaArgs A) {
+    __shared__ SharedTan<true> sh;
+    alpha_tangent_body<true>(A, sh);
+}
+
+__global__ __launch_bounds__(kThreads) void alpha_tangent_kernel_f64(AlphaArgs A) {
+    __shared__ SharedTan<false> sh;
+    alpha_tangent_body<false>(A, sh);
 }
 
 // check that the catalogue holds what the selected absorbers read
@@ -939,7 +1014,8 @@ static int fill_args(AlphaArgs &A, const rbx_catalog *cat, const rbx_absorbers *
     RBX_CHECK_ARG(cat && absorbers, "%s: null catalogue or absorber selection", who);
     RBX_CHECK_ARG(P && T && mix && freq && out, "%s: null device pointer", who);
     RBX_CHECK_ARG(nl > 0 && nf > 0, "%s: nl=%d nf=%d must be positive", who, nl, nf);
-    RBX_CHECK_ARG(nl <= 65535, "%s: nl=%d exceeds 65535", who, nl);
+    RBX_CHECK_ARG(static_cast<long long>(nl) * ((nf + 63) / 64) < (1LL << 31), "%s: nl=%d x nf=%d too large", who,
+                  nl, nf);
     RBX_CHECK_ARG(nf_nh3_low >= 0 && nf_nh3_low <= nf, "%s: nf_nh3_low=%d outside [0, %d]", who, nf_nh3_low, nf);
     int rc = check_catalog(cat, *absorbers, who);
     if (rc) return rc;
@@ -959,12 +1035,6 @@ static int fill_args(AlphaArgs &A, const rbx_catalog *cat, const rbx_absorbers *
     return RBX_OK;
 }
 
-static void set_tiles(AlphaArgs &A, int K, dim3 &grid) {
-    A.tile_width = K * kThreads;
-    A.tiles_low = (A.nf_low + A.tile_width - 1) / A.tile_width;
-    const int tiles_hi = (A.nf - A.nf_low + A.tile_width - 1) / A.tile_width;
-    grid = dim3(A.tiles_low + tiles_hi, A.nl);
-}
 
 // frequencies per lane of the forward kernel: rbx_tune(RBX_TUNE_ALPHA_K), else 1
 static int alpha_tile_k() {
@@ -1048,14 +1118,14 @@ static size_t alpha_ws_bytes(const AlphaArgs &A, int nsplit) {
 template <int K>
 static int launch_alpha(AlphaArgs &A, int precision, int nsplit, hipStream_t st) {
     const int waves = kThreads / 64;
-    dim3 grid((A.tiles_total + waves - 1) / waves, A.nl, nsplit);
+    dim3 grid(static_cast<unsigned>((A.tiles_total + waves - 1) / waves * static_cast<long long>(A.nl)), nsplit);
     if (precision == RBX_FP32)
         hipLaunchKernelGGL((alpha_lines_kernel<K, true>), grid, dim3(kThreads), 0, st, A);
     else
         hipLaunchKernelGGL((alpha_lines_kernel<K, false>), grid, dim3(kThreads), 0, st, A);
     int rc = launch_status("alpha_lines_kernel");
     if (rc) return rc;
-    dim3 g2((A.nf + kThreads - 1) / kThreads, A.nl);
+    dim3 g2(static_cast<unsigned>((A.nf + kThreads - 1) / kThreads * static_cast<long long>(A.nl)));
     if (precision == RBX_FP32)
         hipLaunchKernelGGL((alpha_finish_kernel<true>), g2, dim3(kThreads), 0, st, A, nsplit);
     else
@@ -1081,7 +1151,8 @@ extern "C" int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorb
     RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_workspace_size: bad precision %d",
                   precision);
     RBX_CHECK_ARG(cat && absorbers, "rbx_alpha_workspace_size: null catalogue or absorber selection");
-    RBX_CHECK_ARG(nl > 0 && nl <= 65535 && nf > 0, "rbx_alpha_workspace_size: nl=%d nf=%d", nl, nf);
+    RBX_CHECK_ARG(nl > 0 && nf > 0 && static_cast<long long>(nl) * ((nf + 63) / 64) < (1LL << 31),
+                  "rbx_alpha_workspace_size: nl=%d nf=%d", nl, nf);
     RBX_CHECK_ARG(nf_nh3_low >= 0 && nf_nh3_low <= nf, "rbx_alpha_workspace_size: nf_nh3_low=%d", nf_nh3_low);
     int rc = rbx::check_catalog(cat, *absorbers, "rbx_alpha_workspace_size");
     if (rc) return rc;
@@ -1124,15 +1195,23 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
 
 extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,
                                  const double *T, const double *mix, int nl, const double *freq,
-                                 const double *freq_nh3, int nf, int nf_nh3_low, const double *scale, double *dalpha,
-                                 void *stream) {
+                                 const double *freq_nh3, int nf, int nf_nh3_low, const double *scale, int precision,
+                                 double *alpha, double *dalpha, void *stream) {
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_tangent: bad precision %d", precision);
     rbx::AlphaArgs A;
     int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nl, freq, freq_nh3, nf, nf_nh3_low, scale, dalpha,
                             nullptr, "rbx_alpha_tangent");
     if (rc) return rc;
-    dim3 grid;
-    rbx::set_tiles(A, 1, grid);
-    hipLaunchKernelGGL((rbx::alpha_tangent_kernel<1>), grid, dim3(rbx::kThreads), 0,
-                       static_cast<hipStream_t>(stream), A);
+    A.value = alpha;
+    const long long items = static_cast<long long>(nl) * ((nf + 63) / 64);
+    const long long blocks = (items + rbx::kThreads / 64 - 1) / (rbx::kThreads / 64);
+    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_alpha_tangent: too many layers x frequencies");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (precision == RBX_FP32)
+        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kThreads),
+                           0, st, A);
+    else
+        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kThreads),
+                           0, st, A);
     return rbx::launch_status("alpha_tangent_kernel");
 }

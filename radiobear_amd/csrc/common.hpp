@@ -44,6 +44,21 @@ inline int launch_status(const char *what) {
     return RBX_OK;
 }
 
+// ---- wavefront-level helpers ----
+// LDS written by some lanes of a wavefront and then read by other lanes of the
+// same wavefront: keeps the compiler from moving accesses across this point
+// (the LDS serves one wavefront's requests in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // ---- device math ----
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kLog2e = 1.4426950408889634074;

@@ -30,7 +30,32 @@ struct IntArgs {
     double tau_cut;
     double *Tb, *intW;
     int ngroups;
+    // several atmospheres laid end to end (null: one atmosphere of nl layers):
+    // ray r uses rows [layer_off[a], layer_off[a + 1]) of alpha and T, a = ray_atm[r]
+    const int *ray_atm, *layer_off;
+    int natm;
+    const int *order;   // wavefront w integrates ray order[w] (null: ray w)
 };
+
+// The ray a wavefront integrates (-1: none) and its first layer row and layer
+// count; false when its atmosphere index or offsets are out of range.
+__device__ inline int ray_of(const IntArgs &A, int w) {
+    if (w >= A.nray) return -1;
+    const int r = A.order ? A.order[w] : w;
+    return (r >= 0 && r < A.nray) ? __builtin_amdgcn_readfirstlane(r) : -1;
+}
+
+__device__ inline bool ray_span(const IntArgs &A, int ray, int &base, int &nl) {
+    base = 0;
+    nl = A.nl;
+    if (!A.ray_atm) return true;
+    const int a = A.ray_atm[ray];
+    if (a < 0 || a >= A.natm) return false;
+    base = A.layer_off[a];
+    const int end = A.layer_off[a + 1];
+    nl = end - base;
+    return base >= 0 && end <= A.nl && nl >= 2 && nl - 1 <= A.ld;
+}
 
 __device__ inline double load_alpha(const void *p, int f32, size_t i) {
     return f32 ? static_cast<double>(static_cast<const float *>(p)[i]) : static_cast<const double *>(p)[i];
@@ -63,15 +88,18 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     const int wave = threadIdx.x / kLanes;
     const int lane = threadIdx.x % kLanes;
     const int group = blockIdx.x % A.ngroups;
-    const int ray = __builtin_amdgcn_readfirstlane((blockIdx.x / A.ngroups) * kWaves + wave);
-    if (ray >= A.nray) return;
+    const int ray = ray_of(A, (blockIdx.x / A.ngroups) * kWaves + wave);
+    if (ray < 0) return;
     const int j = group * kLanes + lane;
     const bool ok = j < A.nf;
     const int jj = ok ? j : A.nf - 1;
-    const int S = min(A.nsteps[ray], A.nl - 1);   // never read past the ray's ds row or the table
+    int base, nlr;
+    const bool valid = ray_span(A, ray, base, nlr);
+    const int S = valid ? min(A.nsteps[ray], nlr - 1) : -1;   // never read past the ray's ds row or table
     const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
-    const AT *al = static_cast<const AT *>(A.alpha) + jj;
     const size_t nf = A.nf;
+    const AT *al = static_cast<const AT *>(A.alpha) + static_cast<size_t>(base) * nf + jj;
+    const double *Tl = A.T + base;
     Step<MODE, FAST> st;
     if (S > 1) {
         const int n = S - 1;
@@ -84,7 +112,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
             for (int u = 0; u < U; ++u) {
                 ab[u] = al[(i + u + 1) * nf];
                 hb[u] = h[i + u];
-                Tb[u] = A.T[i + u + 1];
+                Tb[u] = Tl[i + u + 1];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) st(static_cast<double>(ab[u]), hb[u], Tb[u]);
@@ -93,13 +121,15 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
                 break;
             }
         }
-        for (; i < n; ++i) st(static_cast<double>(al[(i + 1) * nf]), h[i], A.T[i + 1]);
+        for (; i < n; ++i) st(static_cast<double>(al[(i + 1) * nf]), h[i], Tl[i + 1]);
     }
     const double intW = st.intW, num = st.num;
     if (!ok) return;
     const size_t o = static_cast<size_t>(ray) * A.nf + j;
     double Tb;
-    if (S < 0 || num < kTcmb)
+    if (!valid)
+        Tb = __builtin_nan("");
+    else if (S < 0 || num < kTcmb)
         Tb = kTcmb;
     else
         Tb = num / intW;
@@ -117,7 +147,8 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
 struct JacArgs {
     IntArgs I;
     const double *dalpha;  // [3][nl][nf]
-    float *J;              // [nray][2][nl][nf]
+    float *J;              // [nray][2][ldj][nf]
+    int ldj;               // layer rows per ray in J (>= the layers of any ray's atmosphere)
 };
 
 __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
@@ -125,29 +156,37 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
     const int wave = threadIdx.x / kLanes;
     const int lane = threadIdx.x % kLanes;
     const int group = blockIdx.x % A.ngroups;
-    const int ray = __builtin_amdgcn_readfirstlane((blockIdx.x / A.ngroups) * kWaves + wave);
-    if (ray >= A.nray) return;
+    const int ray = ray_of(A, (blockIdx.x / A.ngroups) * kWaves + wave);
+    if (ray < 0) return;
     const int j = group * kLanes + lane;
     const bool ok = j < A.nf;
     const int jj = ok ? j : A.nf - 1;
-    const int S = min(A.nsteps[ray], A.nl - 1);   // never read past the ray's ds row or the table
+    int base, nlr;
+    const bool valid = ray_span(A, ray, base, nlr) && nlr <= Aj.ldj;
+    const int S = valid ? min(A.nsteps[ray], nlr - 1) : -1;   // never read past the ray's ds row or table
     const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
-    const size_t nlnf = static_cast<size_t>(A.nl) * A.nf;
-    float *JT = Aj.J + static_cast<size_t>(ray) * 2 * nlnf;
-    float *Jq = JT + nlnf;
+    const size_t nlnf = static_cast<size_t>(A.nl) * A.nf;        // dalpha plane
+    const size_t jrow = static_cast<size_t>(Aj.ldj) * A.nf;      // J plane of one ray
+    float *JT = Aj.J + static_cast<size_t>(ray) * 2 * jrow;
+    float *Jq = JT + jrow;
+    const size_t row0 = static_cast<size_t>(base) * A.nf;
+    const void *alpha = A.f32 ? static_cast<const void *>(static_cast<const float *>(A.alpha) + row0)
+                              : static_cast<const void *>(static_cast<const double *>(A.alpha) + row0);
+    const double *Tl = A.T + base;
+    const double *dal = Aj.dalpha + row0;
 
     // pass 1: Tb and intW (same recurrence and cut as integrate_kernel)
     double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
     int last = 0;   // last path point reached
     if (S > 1) {
-        double a0 = load_alpha(A.alpha, A.f32, jj);
+        double a0 = load_alpha(alpha, A.f32, jj);
         for (int i = 0; i < S - 1; ++i) {
             const double hh = h[i];
-            const double a1 = load_alpha(A.alpha, A.f32, static_cast<size_t>(i + 1) * A.nf + jj);
+            const double a1 = load_alpha(alpha, A.f32, static_cast<size_t>(i + 1) * A.nf + jj);
             tau = tau + (a0 + a1) * hh;
             const double W1 = a1 * (A.fast_exp ? exp_neg_mixed(tau) : exp(-tau));
             intW = intW + (W1 + W0) * hh;
-            const double TW1 = A.T[i + 1] * W1;
+            const double TW1 = Tl[i + 1] * W1;
             num = num + (TW1 + TW0) * hh;
             a0 = a1;
             W0 = W1;
@@ -159,7 +198,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
         }
     }
     const bool live = S > 1 && !(num < kTcmb);
-    const double Tb = (S < 0 || num < kTcmb) ? kTcmb : num / intW;
+    const double Tb = !valid ? __builtin_nan("") : ((S < 0 || num < kTcmb) ? kTcmb : num / intW);
     if (ok && A.Tb) A.Tb[static_cast<size_t>(ray) * A.nf + j] = Tb;
 
     // pass 2: gradients along the path; zero elsewhere
@@ -168,7 +207,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
         const double invD = 1.0 / intW;
         double P = 0.0;          // P_{k-1}
         double hprev = 0.0;      // h_{k-1}
-        double a_k = load_alpha(A.alpha, A.f32, jj);
+        double a_k = load_alpha(alpha, A.f32, jj);
         tau = 0.0;
         for (k = 0; k <= last; ++k) {
             const double hk = (k < last) ? h[k] : 0.0;   // the cut (or path end) closes the sums
@@ -178,18 +217,18 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
                 W = a_k * e;
             }
             const double ck = hprev + hk;
-            const double lam = ck * (A.T[k] - Tb) * invD;
+            const double lam = ck * (Tl[k] - Tb) * invD;
             const double Pk = P + lam * W;
             const double ga = (k > 0 ? lam * e : 0.0) + hprev * P + hk * Pk;
             const size_t o = static_cast<size_t>(k) * A.nf + jj;
-            const double dT = ck * W * invD + ga * Aj.dalpha[o];
-            const double dq = ga * Aj.dalpha[nlnf + o];
+            const double dT = ck * W * invD + ga * dal[o];
+            const double dq = ga * dal[nlnf + o];
             if (ok) {
                 JT[static_cast<size_t>(k) * A.nf + j] = static_cast<float>(dT);
                 Jq[static_cast<size_t>(k) * A.nf + j] = static_cast<float>(dq);
             }
             if (k < last) {
-                const double a_next = load_alpha(A.alpha, A.f32, static_cast<size_t>(k + 1) * A.nf + jj);
+                const double a_next = load_alpha(alpha, A.f32, static_cast<size_t>(k + 1) * A.nf + jj);
                 tau = tau + (a_k + a_next) * hk;
                 a_k = a_next;
             }
@@ -198,7 +237,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
         }
     }
     if (ok) {
-        for (; k < A.nl; ++k) {
+        for (; k < Aj.ldj; ++k) {
             JT[static_cast<size_t>(k) * A.nf + j] = 0.0f;
             Jq[static_cast<size_t>(k) * A.nf + j] = 0.0f;
         }
@@ -265,7 +304,6 @@ static int check_int_args(const void *alpha, int alpha_precision, const double *
                           const double *hds, int ld, const int *nsteps, int nray, int precision) {
     RBX_CHECK_ARG(alpha && T && hds && nsteps, "integration: null device pointer");
     RBX_CHECK_ARG(nl >= 2 && nf > 0 && nray >= 0, "integration: bad sizes nl=%d nf=%d nray=%d", nl, nf, nray);
-    RBX_CHECK_ARG(ld >= nl - 1, "integration: ld=%d < nl-1=%d", ld, nl - 1);
     RBX_CHECK_ARG(alpha_precision == RBX_FP64 || alpha_precision == RBX_FP32, "integration: bad alpha precision");
     RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "integration: bad precision");
     return RBX_OK;
@@ -288,30 +326,22 @@ static void fill_int_args(IntArgs &A, const void *alpha, int alpha_precision, co
     A.Tb = nullptr;
     A.intW = nullptr;
     A.ngroups = (nf + kLanes - 1) / kLanes;
+    A.ray_atm = nullptr;
+    A.layer_off = nullptr;
+    A.natm = 0;
+    A.order = nullptr;
 }
 
 }  // namespace rbx
 
-extern "C" int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
-                             const double *hds, int ld, const int *nsteps, int nray, int mode, int precision,
-                             double tau_cut, double *Tb, double *intW, void *stream) {
-    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
-    if (rc) return rc;
-    RBX_CHECK_ARG(Tb != nullptr, "rbx_integrate: null Tb");
-    RBX_CHECK_ARG(mode == 0 || mode == 1, "rbx_integrate: bad mode %d", mode);
-    if (nray == 0) return RBX_OK;
-    rbx::IntArgs A;
-    rbx::fill_int_args(A, alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision, tau_cut);
-    A.mode = mode;
-    A.Tb = Tb;
-    A.intW = intW;
-    const long long blocks = static_cast<long long>(A.ngroups) * ((nray + rbx::kWaves - 1) / rbx::kWaves);
+static int launch_integrate(const rbx::IntArgs &A, void *stream) {
+    const long long blocks = static_cast<long long>(A.ngroups) * ((A.nray + rbx::kWaves - 1) / rbx::kWaves);
     RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate: too many blocks");
     const dim3 grid(static_cast<unsigned>(blocks)), block(rbx::kWaves * rbx::kLanes);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const bool u4 = rbx::tune_get(RBX_TUNE_INT_UNROLL) == 4;
 #define RBX_INT_LAUNCH(AT, MODE, FAST, U) hipLaunchKernelGGL((rbx::integrate_kernel<AT, MODE, FAST, U>), grid, block, 0, st, A)
-    if (mode == 1) {
+    if (A.mode == 1) {
         if (A.f32) RBX_INT_LAUNCH(float, 1, false, 4); else RBX_INT_LAUNCH(double, 1, false, 4);
     } else if (A.f32) {
         if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(float, 0, true, 4); else RBX_INT_LAUNCH(float, 0, true, 8); }
@@ -324,11 +354,60 @@ extern "C" int rbx_integrate(const void *alpha, int alpha_precision, const doubl
     return rbx::launch_status("integrate_kernel");
 }
 
+extern "C" int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                             const double *hds, int ld, const int *nsteps, int nray, int mode, int precision,
+                             double tau_cut, double *Tb, double *intW, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    RBX_CHECK_ARG(ld >= nl - 1, "rbx_integrate: ld=%d < nl-1=%d", ld, nl - 1);
+    RBX_CHECK_ARG(Tb != nullptr, "rbx_integrate: null Tb");
+    RBX_CHECK_ARG(mode == 0 || mode == 1, "rbx_integrate: bad mode %d", mode);
+    if (nray == 0) return RBX_OK;
+    rbx::IntArgs A;
+    rbx::fill_int_args(A, alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision, tau_cut);
+    A.mode = mode;
+    A.Tb = Tb;
+    A.intW = intW;
+    return launch_integrate(A, stream);
+}
+
+extern "C" int rbx_integrate_batch(const void *alpha, int alpha_precision, const double *T, int nl_total, int nf,
+                                   const int *layer_off, int natm, const double *hds, int ld, const int *nsteps,
+                                   const int *ray_atm, const int *order, int nray, int mode, int precision,
+                                   double tau_cut, double *Tb, double *intW, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    RBX_CHECK_ARG(layer_off && ray_atm && natm > 0, "rbx_integrate_batch: null offsets / ray map or natm=%d", natm);
+    RBX_CHECK_ARG(ld >= 1, "rbx_integrate_batch: ld=%d", ld);
+    RBX_CHECK_ARG(Tb != nullptr, "rbx_integrate_batch: null Tb");
+    RBX_CHECK_ARG(mode == 0 || mode == 1, "rbx_integrate_batch: bad mode %d", mode);
+    if (nray == 0) return RBX_OK;
+    rbx::IntArgs A;
+    rbx::fill_int_args(A, alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision, tau_cut);
+    A.mode = mode;
+    A.Tb = Tb;
+    A.intW = intW;
+    A.ray_atm = ray_atm;
+    A.layer_off = layer_off;
+    A.natm = natm;
+    A.order = order;
+    return launch_integrate(A, stream);
+}
+
+static int launch_jacobian(const rbx::JacArgs &Aj, void *stream) {
+    const long long blocks = static_cast<long long>(Aj.I.ngroups) * ((Aj.I.nray + rbx::kWaves - 1) / rbx::kWaves);
+    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_jacobian: too many blocks");
+    hipLaunchKernelGGL(rbx::jacobian_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes),
+                       0, static_cast<hipStream_t>(stream), Aj);
+    return rbx::launch_status("jacobian_kernel");
+}
+
 extern "C" int rbx_jacobian(const void *alpha, int alpha_precision, const double *dalpha, const double *T, int nl,
                             int nf, const double *hds, int ld, const int *nsteps, int nray, double tau_cut, float *J,
                             double *Tb, void *stream) {
     int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, alpha_precision);
     if (rc) return rc;
+    RBX_CHECK_ARG(ld >= nl - 1, "rbx_jacobian: ld=%d < nl-1=%d", ld, nl - 1);
     RBX_CHECK_ARG(dalpha != nullptr && J != nullptr, "rbx_jacobian: null dalpha or J");
     if (nray == 0) return RBX_OK;
     rbx::JacArgs Aj;
@@ -336,11 +415,31 @@ extern "C" int rbx_jacobian(const void *alpha, int alpha_precision, const double
     Aj.I.Tb = Tb;
     Aj.dalpha = dalpha;
     Aj.J = J;
-    const long long blocks = static_cast<long long>(Aj.I.ngroups) * ((nray + rbx::kWaves - 1) / rbx::kWaves);
-    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_jacobian: too many blocks");
-    hipLaunchKernelGGL(rbx::jacobian_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes),
-                       0, static_cast<hipStream_t>(stream), Aj);
-    return rbx::launch_status("jacobian_kernel");
+    Aj.ldj = nl;
+    return launch_jacobian(Aj, stream);
+}
+
+extern "C" int rbx_jacobian_batch(const void *alpha, int alpha_precision, const double *dalpha, const double *T,
+                                  int nl_total, int nf, const int *layer_off, int natm, const double *hds, int ld,
+                                  const int *nsteps, const int *ray_atm, const int *order, int nray, int precision,
+                                  double tau_cut, float *J, int ldj, double *Tb, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    RBX_CHECK_ARG(layer_off && ray_atm && natm > 0, "rbx_jacobian_batch: null offsets / ray map or natm=%d", natm);
+    RBX_CHECK_ARG(ld >= 1 && ldj >= 2, "rbx_jacobian_batch: ld=%d ldj=%d", ld, ldj);
+    RBX_CHECK_ARG(dalpha != nullptr && J != nullptr, "rbx_jacobian_batch: null dalpha or J");
+    if (nray == 0) return RBX_OK;
+    rbx::JacArgs Aj;
+    rbx::fill_int_args(Aj.I, alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision, tau_cut);
+    Aj.I.Tb = Tb;
+    Aj.I.ray_atm = ray_atm;
+    Aj.I.layer_off = layer_off;
+    Aj.I.natm = natm;
+    Aj.I.order = order;
+    Aj.dalpha = dalpha;
+    Aj.J = J;
+    Aj.ldj = ldj;
+    return launch_jacobian(Aj, stream);
 }
 
 extern "C" int rbx_integrate_profiles(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
@@ -348,6 +447,7 @@ extern "C" int rbx_integrate_profiles(const void *alpha, int alpha_precision, co
                                       int precision, double *tau, double *W, double *Tb_lyr, void *stream) {
     int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
     if (rc) return rc;
+    RBX_CHECK_ARG(ld >= nl - 1, "rbx_integrate_profiles: ld=%d < nl-1=%d", ld, nl - 1);
     RBX_CHECK_ARG(tau && W && Tb_lyr, "rbx_integrate_profiles: null output");
     RBX_CHECK_ARG(mode == 0 || mode == 1, "rbx_integrate_profiles: bad mode %d", mode);
     if (nray == 0) return RBX_OK;

@@ -92,6 +92,10 @@ struct RayArgs {
     int ld;
     int *nsteps;
     int limb;
+    // several atmospheres laid end to end (null: one atmosphere of nl layers):
+    // ray r uses layers [layer_off[a], layer_off[a + 1]) with a = ray_atm[r]
+    const int *ray_atm, *layer_off;
+    int natm;
 };
 
 __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
@@ -100,7 +104,21 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
     const double b0 = A.b[2 * ray], b1 = A.b[2 * ray + 1];
     double *out = A.ds + static_cast<size_t>(ray) * A.ld;
     const Geo &G = A.G;
-    const double rNorm = A.R[0];
+    const double *Rl = A.R, *Nl = A.N;
+    int nl = A.nl;
+    if (A.ray_atm) {
+        const int a = A.ray_atm[ray];
+        const int l0 = (a >= 0 && a < A.natm) ? A.layer_off[a] : -1;
+        const int l1 = (a >= 0 && a < A.natm) ? A.layer_off[a + 1] : -1;
+        if (l0 < 0 || l1 > A.nl || l1 - l0 < 2 || l1 - l0 - 1 > A.ld) {   // bad index or offsets
+            A.nsteps[ray] = -2;
+            return;
+        }
+        Rl += l0;
+        Nl += l0;
+        nl = l1 - l0;
+    }
+    const double rNorm = Rl[0];
     const double tmp = b0 * b0 + b1 * b1;
     if (!(tmp < 1.0)) {                                // raypath.py:126-127 (NaN b: off planet too)
         A.nsteps[ray] = -1;
@@ -152,7 +170,7 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
     V3 r = geo.r;
     V3 n = geo.n;
     double c = -dot(s, n);                            // cos(t_inc)
-    double nratio = A.N[0] / A.N[1];
+    double nratio = Nl[0] / Nl[1];
     double st = nratio * sqrt(1.0 - c * c);          // sin(t_tran)
     double ct = sqrt(1.0 - st * st);                 // cos(t_tran)
     const double cm_half = (A.units == 1);
@@ -161,7 +179,7 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
     while (true) {
         s = nratio * s + (nratio * c - ct) * n;      // raypath.py:179-180 (ingress)
         const double rNow = geo.rmag;
-        const double rNext = ellipse(G, A.R[layer + 1], q).rmag;
+        const double rNext = ellipse(G, Rl[layer + 1], q).rmag;
         const double rdots = dot(r, s);
         const double disc = rdots * rdots + rNext * rNext - rNow * rNow;
         double ds = -rdots - sqrt(disc);             // NaN when the shell is missed
@@ -171,12 +189,12 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
         ++count;
         const V3 rn = r + ds * s;
         q = latlng(rn);
-        geo = ellipse(G, A.R[layer + 1], q);
+        geo = ellipse(G, Rl[layer + 1], q);
         r = rn;
         n = geo.n;
         ++layer;
         c = -dot(s, n);
-        if (layer + 1 >= A.nl) break;                // IndexError on nr[layer + 1]
+        if (layer + 1 >= nl) break;                  // IndexError on nr[layer + 1]
         nratio = 1.0;
         st = sqrt(1.0 - c * c);
         ct = sqrt(1.0 - st * st);
@@ -186,18 +204,16 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
 
 }  // namespace rbx
 
-extern "C" int rbx_raypaths(const rbx_planet *planet, const double *R, const double *N, int nl, const double *b,
-                            int nray, int out_units, double *ds, int ld, int *nsteps, void *stream) {
-    RBX_CHECK_ARG(planet && R && N && b && ds && nsteps, "rbx_raypaths: null argument");
-    RBX_CHECK_ARG(nl >= 2, "rbx_raypaths: need at least 2 layers (nl=%d)", nl);
-    RBX_CHECK_ARG(nray >= 0, "rbx_raypaths: negative ray count");
-    RBX_CHECK_ARG(ld >= nl - 1, "rbx_raypaths: ld=%d < nl-1=%d", ld, nl - 1);
-    RBX_CHECK_ARG(out_units == 0 || out_units == 1, "rbx_raypaths: bad out_units %d", out_units);
-    RBX_CHECK_ARG(planet->gtype == 0 || planet->gtype == 1, "rbx_raypaths: bad gtype %d", planet->gtype);
-    RBX_CHECK_ARG(planet->limb == 0 || planet->limb == 1, "rbx_raypaths: bad limb %d", planet->limb);
-    RBX_CHECK_ARG(planet->req > 0.0 && planet->rpol > 0.0, "rbx_raypaths: radii must be positive");
-    if (nray == 0) return RBX_OK;
-    rbx::RayArgs A;
+// planet shape, view and sizes shared by rbx_raypaths and rbx_raypaths_batch
+static int fill_ray_args(rbx::RayArgs &A, const rbx_planet *planet, const double *R, const double *N, int nl,
+                         const double *b, int nray, int out_units, double *ds, int ld, int *nsteps, const char *who) {
+    RBX_CHECK_ARG(planet && R && N && b && ds && nsteps, "%s: null argument", who);
+    RBX_CHECK_ARG(nl >= 2, "%s: need at least 2 layers (nl=%d)", who, nl);
+    RBX_CHECK_ARG(nray >= 0, "%s: negative ray count", who);
+    RBX_CHECK_ARG(out_units == 0 || out_units == 1, "%s: bad out_units %d", who, out_units);
+    RBX_CHECK_ARG(planet->gtype == 0 || planet->gtype == 1, "%s: bad gtype %d", who, planet->gtype);
+    RBX_CHECK_ARG(planet->limb == 0 || planet->limb == 1, "%s: bad limb %d", who, planet->limb);
+    RBX_CHECK_ARG(planet->req > 0.0 && planet->rpol > 0.0, "%s: radii must be positive", who);
     // computeAspect (raypath.py:39-44)
     const double f = 1.0 - planet->rpol / planet->req;
     const double tip = -planet->orientation[0] * rbx::kPi / 180.0;
@@ -217,8 +233,40 @@ extern "C" int rbx_raypaths(const rbx_planet *planet, const double *R, const dou
     A.ld = ld;
     A.nsteps = nsteps;
     A.limb = planet->limb;
+    A.ray_atm = nullptr;
+    A.layer_off = nullptr;
+    A.natm = 0;
+    return RBX_OK;
+}
+
+static int launch_rays(const rbx::RayArgs &A, void *stream) {
     const int threads = 64;
-    hipLaunchKernelGGL(rbx::raypath_kernel, dim3((nray + threads - 1) / threads), dim3(threads), 0,
+    hipLaunchKernelGGL(rbx::raypath_kernel, dim3((A.nray + threads - 1) / threads), dim3(threads), 0,
                        static_cast<hipStream_t>(stream), A);
     return rbx::launch_status("raypath_kernel");
+}
+
+extern "C" int rbx_raypaths(const rbx_planet *planet, const double *R, const double *N, int nl, const double *b,
+                            int nray, int out_units, double *ds, int ld, int *nsteps, void *stream) {
+    rbx::RayArgs A;
+    int rc = fill_ray_args(A, planet, R, N, nl, b, nray, out_units, ds, ld, nsteps, "rbx_raypaths");
+    if (rc) return rc;
+    RBX_CHECK_ARG(ld >= nl - 1, "rbx_raypaths: ld=%d < nl-1=%d", ld, nl - 1);
+    if (nray == 0) return RBX_OK;
+    return launch_rays(A, stream);
+}
+
+extern "C" int rbx_raypaths_batch(const rbx_planet *planet, const double *R, const double *N, int nl_total,
+                                  const int *layer_off, int natm, const double *b, int nray, const int *ray_atm,
+                                  int out_units, double *ds, int ld, int *nsteps, void *stream) {
+    rbx::RayArgs A;
+    int rc = fill_ray_args(A, planet, R, N, nl_total, b, nray, out_units, ds, ld, nsteps, "rbx_raypaths_batch");
+    if (rc) return rc;
+    RBX_CHECK_ARG(layer_off && ray_atm && natm > 0, "rbx_raypaths_batch: null offsets / ray map or natm=%d", natm);
+    RBX_CHECK_ARG(ld >= 1, "rbx_raypaths_batch: ld=%d", ld);
+    if (nray == 0) return RBX_OK;
+    A.ray_atm = ray_atm;
+    A.layer_off = layer_off;
+    A.natm = natm;
+    return launch_rays(A, stream);
 }

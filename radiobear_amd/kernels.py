@@ -174,14 +174,17 @@ def alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec, reuse=None):
     return torch.empty((max(1, (nb.value + 7) // 8),), dtype=torch.float64, device=device()), nb.value
 
 
-def alpha_tangent(catalog, absorbers, P, T, mix, freqs, scale=None):
-    """d alpha / d(T, q_NH3, q_H2O): (3, nl, nf) float64 on the device."""
+def alpha_tangent(catalog, absorbers, P, T, mix, freqs, scale=None, precision='fp64', with_alpha=False):
+    """d alpha / d(T, q_NH3, q_H2O): (3, nl, nf) float64 on the device; with_alpha also
+    returns alpha (nl, nf) float64 from the same pass: (alpha, dalpha)."""
+    prec = _lib.PRECISIONS[precision]
     P, T, mix, fd, fn, nf_low, sc, nl = _alpha_inputs(P, T, mix, freqs, scale)
     nf = fd.shape[0]
     out = torch.empty((3, nl, nf), dtype=torch.float64, device=device())
+    val = torch.empty((nl, nf), dtype=torch.float64, device=device()) if with_alpha else None
     check(lib().rbx_alpha_tangent(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), nl,
-                                  _ptr(fd), _ptr(fn), nf, nf_low, _ptr(sc), _ptr(out), _stream()))
-    return out
+                                  _ptr(fd), _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(val), _ptr(out), _stream()))
+    return (val, out) if with_alpha else out
 
 
 def planet_desc(req, rpol, orientation=(0.0, 0.0), gtype='ellipse', limb='shape'):
@@ -201,11 +204,18 @@ def planet_desc(req, rpol, orientation=(0.0, 0.0), gtype='ellipse', limb='shape'
     return d
 
 
+def _rays(b):
+    """(nray, 2) float64 device tensor of impact parameters."""
+    if isinstance(b, torch.Tensor):
+        return as_device(b).reshape(-1, 2)
+    return as_device(np.asarray(b, dtype=np.float64).reshape(-1, 2))
+
+
 def raypaths(planet, R, N, b, units='half_cm'):
     """Ray paths for impact parameters b (nray, 2): (ds (nray, nl-1) float64, nsteps (nray,) int32)."""
     R = as_device(R)
     N = as_device(N)
-    b = as_device(np.asarray(b, dtype=np.float64).reshape(-1, 2))
+    b = _rays(b)
     nl = R.shape[0]
     nray = b.shape[0]
     ld = max(nl - 1, 1)
@@ -224,10 +234,11 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
     T = as_device(T)
     nl, nf = a.shape
     nray, ld = hds.shape
+    hds, nsteps = hds.contiguous(), nsteps.contiguous()   # held here: the C call reads their memory
     Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
     iw = torch.empty((nray, nf), dtype=torch.float64, device=device()) if with_intW else None
-    check(lib().rbx_integrate(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds.contiguous()), ld,
-                              _ptr(nsteps.contiguous()), nray, 1 if disc else 0, prec, float(tau_cut),
+    check(lib().rbx_integrate(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds), ld,
+                              _ptr(nsteps), nray, 1 if disc else 0, prec, float(tau_cut),
                               _ptr(Tb), _ptr(iw), _stream()))
     return (Tb, iw) if with_intW else Tb
 
@@ -240,9 +251,10 @@ def integrate_profiles(alpha_table, T, hds, nsteps, disc=False, precision=None):
     T = as_device(T)
     nl, nf = a.shape
     nray, ld = hds.shape
+    hds, nsteps = hds.contiguous(), nsteps.contiguous()   # held here: the C call reads their memory
     out = [torch.empty((nray, nl, nf), dtype=torch.float64, device=device()) for _ in range(3)]
-    check(lib().rbx_integrate_profiles(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds.contiguous()), ld,
-                                       _ptr(nsteps.contiguous()), nray, 1 if disc else 0, prec,
+    check(lib().rbx_integrate_profiles(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds), ld,
+                                       _ptr(nsteps), nray, 1 if disc else 0, prec,
                                        *[_ptr(t) for t in out], _stream()))
     return tuple(out)
 
@@ -254,10 +266,12 @@ def jacobian(alpha_table, dalpha, T, hds, nsteps, tau_cut=0.0):
     T = as_device(T)
     nl, nf = a.shape
     nray, ld = hds.shape
+    hds, nsteps = hds.contiguous(), nsteps.contiguous()   # held here: the C call reads their memory
+    dalpha = dalpha.contiguous()
     J = torch.empty((nray, 2, nl, nf), dtype=torch.float32, device=device())
     Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
-    check(lib().rbx_jacobian(_ptr(a), aprec, _ptr(dalpha.contiguous()), _ptr(T), nl, nf,
-                             _ptr(hds.contiguous()), ld, _ptr(nsteps.contiguous()), nray, float(tau_cut),
+    check(lib().rbx_jacobian(_ptr(a), aprec, _ptr(dalpha), _ptr(T), nl, nf,
+                             _ptr(hds), ld, _ptr(nsteps), nray, float(tau_cut),
                              _ptr(J), _ptr(Tb), _stream()))
     return J, Tb
 
@@ -288,3 +302,86 @@ def jupiter_catalog(raw, truncate_strength=None, truncate_freq=None, nh3_inversi
     h2o_lines = L.h2o_bk_lines() if h2o is None else h2o
     cat.set_lines(_lib.LINES_H2O, L.truncate_h2o(h2o_lines, ts.get('h2o'), tf.get('h2o')))
     return cat
+
+
+# ------------------------------------------------------- many atmospheres
+def _i32(x):
+    return as_device(np.asarray(x, dtype=np.int32), dtype=torch.int32)
+
+
+def raypaths_batch(planet, R, N, layer_off, ray_atm, b, ld, units='half_cm', off_dev=None, ray_atm_dev=None):
+    """Ray paths through several atmospheres laid end to end (rbx_raypaths_batch).
+
+    R, N: (nl_total,) concatenated layer radii / refractive indices; layer_off (natm+1,)
+    host ints; ray_atm (nray,) host ints (checked here); off_dev / ray_atm_dev: their
+    int32 device copies if the caller holds them; ld >= max layers - 1.
+    Returns (ds, nsteps)."""
+    off = np.asarray(layer_off, dtype=np.int64)
+    ra = np.asarray(ray_atm, dtype=np.int64)
+    _check_batch(off, ra, R.shape[0] if isinstance(R, torch.Tensor) else len(R), ld)
+    R = as_device(R)
+    N = as_device(N)
+    b = _rays(b)
+    nray = b.shape[0]
+    off_d = off_dev if off_dev is not None else _i32(off)     # keep the device copies alive over the call
+    ra_d = ray_atm_dev if ray_atm_dev is not None else _i32(ra)
+    ds = torch.empty((nray, ld), dtype=torch.float64, device=device())
+    ns = torch.empty((nray,), dtype=torch.int32, device=device())
+    check(lib().rbx_raypaths_batch(ctypes.byref(planet), _ptr(R), _ptr(N), R.shape[0], _ptr(off_d), len(off) - 1,
+                                   _ptr(b), nray, _ptr(ra_d), 1 if units == 'half_cm' else 0, _ptr(ds), ld,
+                                   _ptr(ns), _stream()))
+    return ds, ns
+
+
+def _check_batch(off, ray_atm, nl_total, ld):
+    if off.ndim != 1 or len(off) < 2 or off[0] != 0 or off[-1] != nl_total or np.any(np.diff(off) < 2):
+        raise ValueError('layer offsets must run from 0 to nl_total with >= 2 layers per atmosphere')
+    if len(ray_atm) and (ray_atm.min() < 0 or ray_atm.max() >= len(off) - 1):
+        raise ValueError('ray atmosphere index out of range')
+    if ld < int(np.max(np.diff(off))) - 1:
+        raise ValueError('ld={} < largest atmosphere - 1'.format(ld))
+
+
+def jacobian_batch(alpha_table, dalpha, T, layer_off, ray_atm, hds, nsteps, ldj, order=None, tau_cut=0.0,
+                   precision=None):
+    """Tb (nray, nf) and J (nray, 2, ldj, nf) float32 for rays through several atmospheres.
+
+    alpha_table (nl_total, nf), dalpha (3, nl_total, nf) and T (nl_total,) concatenated;
+    layer_off / ray_atm / order as device or host int arrays (order None: 0..nray-1)."""
+    aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
+    prec = aprec if precision is None else _lib.PRECISIONS[precision]
+    a = alpha_table.contiguous()
+    T = as_device(T)
+    nl, nf = a.shape
+    nray, ld = hds.shape
+    hds, nsteps = hds.contiguous(), nsteps.contiguous()   # held here: the C call reads their memory
+    off = layer_off if isinstance(layer_off, torch.Tensor) else _i32(layer_off)
+    ra = ray_atm if isinstance(ray_atm, torch.Tensor) else _i32(ray_atm)
+    od = None if order is None else (order if isinstance(order, torch.Tensor) else _i32(order))
+    dalpha = dalpha.contiguous()
+    J = torch.empty((nray, 2, ldj, nf), dtype=torch.float32, device=device())
+    Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
+    check(lib().rbx_jacobian_batch(_ptr(a), aprec, _ptr(dalpha), _ptr(T), nl, nf, _ptr(off),
+                                   off.shape[0] - 1, _ptr(hds), ld, _ptr(nsteps), _ptr(ra),
+                                   _ptr(od), nray, prec, float(tau_cut), _ptr(J), ldj, _ptr(Tb), _stream()))
+    return J, Tb
+
+
+def integrate_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, order=None, disc=False, precision=None,
+                    tau_cut=0.0):
+    """Tb (nray, nf) float64 for rays through several atmospheres laid end to end."""
+    aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
+    prec = aprec if precision is None else _lib.PRECISIONS[precision]
+    a = alpha_table.contiguous()
+    T = as_device(T)
+    nl, nf = a.shape
+    nray, ld = hds.shape
+    hds, nsteps = hds.contiguous(), nsteps.contiguous()   # held here: the C call reads their memory
+    off = layer_off if isinstance(layer_off, torch.Tensor) else _i32(layer_off)
+    ra = ray_atm if isinstance(ray_atm, torch.Tensor) else _i32(ray_atm)
+    od = None if order is None else (order if isinstance(order, torch.Tensor) else _i32(order))
+    Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
+    check(lib().rbx_integrate_batch(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(off), off.shape[0] - 1,
+                                    _ptr(hds), ld, _ptr(nsteps), _ptr(ra), _ptr(od), nray,
+                                    1 if disc else 0, prec, float(tau_cut), _ptr(Tb), None, _stream()))
+    return Tb

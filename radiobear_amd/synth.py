@@ -165,11 +165,18 @@ def random_disc_rays(n, seed, rmax=1.02):
     return np.stack([r * np.cos(th), r * np.sin(th)], axis=1)
 
 
+_GP_FACTORS = {}
+
+
 def gp_perturbation(rng, logp, scale, length):
-    """Squared-exponential Gaussian-process draw over log10 P."""
-    d = logp[:, None] - logp[None, :]
-    K = scale ** 2 * np.exp(-0.5 * (d / length) ** 2) + 1e-10 * np.eye(len(logp))
-    L = np.linalg.cholesky(K)
+    """Squared-exponential Gaussian-process draw over log10 P (Cholesky factors cached
+    per grid, scale and length)."""
+    key = (len(logp), float(logp[0]), float(logp[-1]), scale, length)
+    L = _GP_FACTORS.get(key)
+    if L is None:
+        d = logp[:, None] - logp[None, :]
+        K = scale ** 2 * np.exp(-0.5 * (d / length) ** 2) + 1e-10 * np.eye(len(logp))
+        L = _GP_FACTORS[key] = np.linalg.cholesky(K)
     return L @ rng.standard_normal(len(logp))
 
 

@@ -1,0 +1,134 @@
+"""Many atmospheres in one launch: the batched path against one atmosphere at a time,
+the fused tangent pass against rbx_alpha, and the FP32 tangents against FP64."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from radiobear_amd import synth
+
+pytestmark = pytest.mark.gpu
+C = synth.GAS_ROWS
+JUPITER_ALPHA = {'h2': 'h2_jj_ddb', 'h2o': 'h2o_bk', 'h2s': 'h2s_ddb', 'nh3': 'nh3_hs_sjs', 'ph3': 'ph3_jh'}
+
+
+@pytest.fixture(scope='module')
+def K():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no HIP device')
+    from radiobear_amd import kernels
+    return kernels
+
+
+@pytest.fixture(scope='module')
+def small_batch(K, raw_lines):
+    from radiobear_amd.retrieval import RetrievalBatch
+    atms = synth.retrieval_batch(4, seed=7, nl_range=(25, 70))
+    inv = synth.nh3_inversion_lines(700, 3)
+    cat = K.jupiter_catalog(raw_lines, nh3_inversion=inv)
+    ab = K.absorbers_struct(JUPITER_ALPHA, 'e')
+    f = np.linspace(1.0, 50.0, 75)                     # straddles 30 GHz inside a 64-column tile
+    rays = np.array([[0.0, 0.0], [0.4, 0.3], [0.8, -0.2], [0.97, 0.05], [1.2, 0.0]])
+    a0 = atms[0]
+    out = {}
+    for prec in ('fp64', 'fp32'):
+        out[prec] = RetrievalBatch(atms, cat, ab, f, rays, a0.Req, a0.Rpol, C, precision=prec)
+    return atms, cat, ab, f, rays, out
+
+
+@pytest.mark.parametrize('prec', ['fp64', 'fp32'])
+def test_batch_equals_one_atmosphere_at_a_time(K, small_batch, prec):
+    atms, cat, ab, f, rays, batches = small_batch
+    rb = batches[prec]
+    Tb, J = rb.run()
+    Tb, J = Tb.cpu().numpy(), J.cpu().numpy()
+    pl = K.planet_desc(atms[0].Req, atms[0].Rpol)
+    for a, atm in enumerate(atms):
+        rows, _ = rb.atmosphere_slice(a)
+        mix = K.mix_table(atm.gas, C)
+        alpha, dal = K.alpha_tangent(cat, ab, atm.P, atm.T, mix, f, precision=prec, with_alpha=True)
+        hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
+        J1, Tb1 = K.jacobian(alpha.float() if prec == 'fp32' else alpha, dal, atm.T, hds, ns,
+                             tau_cut=rb.tau_cut)
+        if prec == 'fp64':
+            assert np.array_equal(Tb[rows], Tb1.cpu().numpy()), a
+            assert np.array_equal(J[rows, :, :atm.nl], J1.cpu().numpy()), a
+        else:   # the single-atmosphere call reads FP32 alpha; the batch reads the FP64 value
+            assert np.max(np.abs(Tb[rows] - Tb1.cpu().numpy())) < 1e-3, a
+            j1 = J1.cpu().numpy()
+            scale = np.max(np.abs(j1), axis=2, keepdims=True) + 1e-30
+            assert np.max(np.abs(J[rows, :, :atm.nl] - j1) / scale) < 1e-4, a
+        assert not np.any(J[rows, :, atm.nl:]), a              # rows past the atmosphere
+    assert np.all(Tb[4::5] == 2.725)                           # off-disc rays: T_cmb
+
+
+def test_fused_alpha_value_and_fp32_tangents(K, small_batch):
+    atms, cat, ab, f, rays, batches = small_batch
+    atm = atms[1]
+    mix = K.mix_table(atm.gas, C)
+    ref = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp64').cpu().numpy()
+    a64, d64 = K.alpha_tangent(cat, ab, atm.P, atm.T, mix, f, precision='fp64', with_alpha=True)
+    a32, d32 = K.alpha_tangent(cat, ab, atm.P, atm.T, mix, f, precision='fp32', with_alpha=True)
+    assert np.max(np.abs(a64.cpu().numpy() - ref) / ref) < 1e-12
+    assert np.max(np.abs(a32.cpu().numpy() - ref) / ref) < 1e-5
+    d64, d32 = d64.cpu().numpy(), d32.cpu().numpy()
+    for m in range(3):
+        scale = np.max(np.abs(d64[m]), axis=1, keepdims=True) + 1e-300
+        assert np.max(np.abs(d32[m] - d64[m]) / scale) < 1e-5, m
+
+
+def test_bad_atmosphere_index_is_flagged_not_read(K, small_batch):
+    """An out-of-range atmosphere index in the device ray map gives nsteps -2 and Tb NaN
+    (the kernels check it before any load); the host wrapper refuses it up front."""
+    import torch
+    from radiobear_amd import _lib
+    atms, cat, ab, f, rays, batches = small_batch
+    rb = batches['fp64']
+    with pytest.raises(ValueError):
+        K.raypaths_batch(rb.planet, rb.R, rb.N, rb.layer_off, np.full(rb.nray, rb.natm), rb.b, rb.ld)
+    bad = rb.ray_atm.copy()
+    bad[3] = rb.natm + 5
+    bad[7] = -1
+    bad_d = K._i32(bad)
+    ds = torch.empty((rb.nray, rb.ld), dtype=torch.float64, device='cuda')
+    ns = torch.empty((rb.nray,), dtype=torch.int32, device='cuda')
+    _lib.check(_lib.lib().rbx_raypaths_batch(ctypes.byref(rb.planet), K._ptr(rb.R), K._ptr(rb.N), rb.nl_total,
+                                             K._ptr(rb.off_d), rb.natm, K._ptr(rb.b), rb.nray, K._ptr(bad_d), 1,
+                                             K._ptr(ds), rb.ld, K._ptr(ns), K._stream()))
+    nsn = ns.cpu().numpy()
+    assert nsn[3] == -2 and nsn[7] == -2 and np.all(nsn[[0, 1, 2]] >= 0)
+    alpha, dal = rb.absorption()
+    Tb = K.integrate_batch(alpha, rb.T, rb.off_d, bad_d, ds, ns).cpu().numpy()
+    assert np.all(np.isnan(Tb[[3, 7]])) and not np.any(np.isnan(Tb[[0, 1, 2]]))
+    J, Tb2 = K.jacobian_batch(alpha, dal, rb.T, rb.off_d, bad_d, ds, ns, rb.ldj)
+    assert np.all(np.isnan(Tb2.cpu().numpy()[[3, 7]]))
+    assert not torch.any(J[[3, 7]]).item()
+
+
+def test_retrieval_rows_vs_torch_autograd(K, raw_lines, jupiter_lines):
+    """Two atmospheres of a batch against float64 autograd of the restated forward model."""
+    from oracle import geometry, jacobian as OJ
+    from radiobear_amd.retrieval import RetrievalBatch
+    atms = synth.retrieval_batch(2, seed=11, nl_range=(30, 45))
+    cat = K.jupiter_catalog(raw_lines)
+    ab = K.absorbers_struct(JUPITER_ALPHA, 'e')
+    f = np.array([2.0, 12.0, 23.0, 29.5, 31.0, 44.0])
+    rays = np.array([[0.1, 0.0], [0.7, 0.4]])
+    a0 = atms[0]
+    for prec in ('fp64', 'fp32'):
+        rb = RetrievalBatch(atms, cat, ab, f, rays, a0.Req, a0.Rpol, C, precision=prec)
+        Tb, J = rb.run()
+        Tb, J = Tb.cpu().numpy(), J.double().cpu().numpy()
+        for a, atm in enumerate(atms):
+            rows, _ = rb.atmosphere_slice(a)
+            paths = []
+            for b in rays:
+                r = geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol)
+                paths.append((r.ds, r.layer4ds))
+            Tr, Jr = OJ.jacobian(f, atm.T, atm.P, K.mix_table(atm.gas, C), jupiter_lines, paths)
+            assert np.max(np.abs(Tb[rows] - Tr)) < (1e-8 if prec == 'fp64' else 1e-3)
+            for slot in (0, 1):
+                scale = np.max(np.abs(Jr[:, slot]), axis=1, keepdims=True) + 1e-30
+                err = np.max(np.abs(J[rows, slot, :atm.nl] - Jr[:, slot]) / scale)
+                assert err < 1e-3, (prec, a, slot, err)
