@@ -255,35 +255,61 @@ def cpu_baseline(prob, args):
             'sample': sample, 'seconds_extrapolated': total}
 
 
-def jacobian_rate(args):
-    """Workload 4: forward + adjoint Jacobian rows/s (dTb/dT and dTb/dq_NH3 per layer)."""
+def _rows_rate(rb, reps):
+    """Forward + adjoint Jacobian rows/s of a RetrievalBatch (events on the launch stream)."""
     import torch
+    from radiobear_amd import kernels as K
+    rb.run()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        Tb, J = rb.run()
+    ev[1].record()
+    torch.cuda.synchronize()
+    dt = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+    # one pass split by stage
+    st = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    st[0].record()
+    alpha, dal = rb.absorption()
+    st[1].record()
+    hds, ns = rb.paths()
+    st[2].record()
+    K.jacobian_batch(alpha, dal, rb.T, rb.off_d, rb.ray_atm_d, hds, ns, rb.ldj, order=rb.order_d,
+                     tau_cut=rb.tau_cut, precision=rb.precision)
+    st[3].record()
+    torch.cuda.synchronize()
+    stages = {'rbx_alpha_tangent': st[0].elapsed_time(st[1]), 'rbx_raypaths_batch': st[1].elapsed_time(st[2]),
+              'rbx_jacobian_batch': st[2].elapsed_time(st[3])}
+    return {'rows_per_s': rb.rows() / dt, 'ms_per_step': dt * 1e3, 'rows': rb.rows(), 'stage_ms': stages,
+            'nan_rows': int(torch.isnan(Tb).any(dim=1).sum().item())}
+
+
+def jacobian_rate(args):
+    """Workload 4 (500 layers, 10000 NH3 lines, 64 freqs, 256 rays): forward + adjoint
+    Jacobian rows/s (dTb/dT and dTb/dq_NH3 per layer), fused FP32 alpha+tangent pass."""
     from radiobear_amd import kernels as K, lines, synth
+    from radiobear_amd.retrieval import RetrievalBatch
     w = synth.workload('c4_jacobian')
     atm = w['atm']
     cat = K.jupiter_catalog(lines.raw_catalog(), nh3_inversion=w['nh3_inv'])
     ab = K.absorbers_struct({'h2': 'h2_jj_ddb', 'h2o': 'h2o_bk', 'h2s': 'h2s_ddb', 'nh3': 'nh3_hs_sjs',
                              'ph3': 'ph3_jh'}, 'e')
-    P, T, mix = K.as_device(atm.P), K.as_device(atm.T), K.as_device(K.mix_table(atm.gas, atm.C))
-    pl = K.planet_desc(atm.Req, atm.Rpol)
+    rb = RetrievalBatch([atm], cat, ab, w['freqs'], w['rays'], atm.Req, atm.Rpol, atm.C, precision='fp32')
+    out = _rows_rate(rb, 10)
+    out.update({'workload': 'c4_jacobian', 'entries_per_row': 2 * atm.nl, 'precision': 'fp32 line sums and tangents'})
+    return out
 
-    def run():
-        tab = K.alpha(cat, ab, P, T, mix, w['freqs'], 'fp32')
-        dal = K.alpha_tangent(cat, ab, P, T, mix, w['freqs'])
-        hds, ns = K.raypaths(pl, atm.R, atm.N, w['rays'])
-        return K.jacobian(tab, dal, T, hds, ns, tau_cut=50.0)
 
-    run()
-    torch.cuda.synchronize()
-    n = 5
-    t0 = time.perf_counter()
-    for _ in range(n):
-        J, Tb = run()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / n
-    rows = len(w['rays']) * len(w['freqs'])
-    return {'workload': 'c4_jacobian', 'rows_per_s': rows / dt, 'ms_per_step': dt * 1e3, 'rows': rows,
-            'entries_per_row': 2 * atm.nl}
+def retrieval_rate(args):
+    """Workload 5: 1024 perturbed atmospheres (200-800 layers each), 10000 NH3 lines,
+    128 freqs, 16 rays each, forward + adjoint in one batch: Jacobian rows/s."""
+    from radiobear_amd import retrieval
+    rb = retrieval.synthetic(n_atm=1024, precision='fp32')
+    out = _rows_rate(rb, 2)
+    out.update({'workload': 'c5_retrieval', 'atmospheres': rb.natm, 'layers_total': rb.nl_total,
+                'rays': rb.nray, 'freqs': len(rb.freqs), 'precision': 'fp32 line sums and tangents'})
+    return out
 
 
 def main():
@@ -346,6 +372,8 @@ def main():
         res['speedup_fp32_vs_fp64'] = dt64 / dt32
         res['jacobian'] = jacobian_rate(args)
         log(args, 'jacobian', res['jacobian'])
+        res['retrieval'] = retrieval_rate(args)
+        log(args, 'retrieval', res['retrieval'])
 
     # roofline of the dominant kernel (by duration)
     dom = max(kt32, key=kt32.get)

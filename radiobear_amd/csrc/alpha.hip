@@ -238,10 +238,21 @@ struct H2oPrep {   // h2o_bk lines, VVW with base (h2o_bk.py:83-99, 130-189)
 struct Staged64 {
     double D[2][kThreads], a[2][kThreads], c[2][kThreads], g[2][kThreads], G[2][kThreads];
 };
+// FP32 mode stages the denominator as (x - Ds)^2 + h with Ds = D - g/2 and
+// h = g (D - g/4) + G, the same polynomial in x as (x - D)^2 + g x + G; it is
+// evaluated in FP64 (h < 0 for lines wider than their centre, where the two
+// terms cancel) and rounded once, so each line costs one FMA less.
 struct Staged32 {
-    double D[2][kThreads];
-    float4 q[2][kThreads];  // a, c, g, G
+    alignas(16) double D[2][kThreads];   // Ds
+    alignas(16) double h[2][kThreads];
+    alignas(16) float2 ac[2][kThreads];  // a, c (scaled)
 };
+
+template <class S>
+__device__ inline void shifted_denominator(const CoefT<S> &q, S &Ds, S &h) {
+    Ds = q.D - 0.5 * q.g;
+    h = q.g * (q.D - 0.25 * q.g) + q.G;
+}
 template <int N>
 struct SjsLines {
     double f0[N], w[N], g0[N];
@@ -279,10 +290,12 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, int mine
                 CoefT<double> q = {0.0, 0.0, 0.0, 0.0, 1.0};   // padding: term = 0 / ((x)^2 + 1)
                 if (i < le) q = prep[s](B, i, base);
                 if constexpr (F32) {
-                    sh.s32.D[s][threadIdx.x] = q.D;
-                    sh.s32.q[s][threadIdx.x] =
-                        make_float4(static_cast<float>(q.a * scale[s]), static_cast<float>(q.c * scale[s]),
-                                    static_cast<float>(q.g), static_cast<float>(q.G));
+                    double Ds, h;
+                    shifted_denominator(q, Ds, h);
+                    sh.s32.D[s][threadIdx.x] = Ds;
+                    sh.s32.h[s][threadIdx.x] = h;
+                    sh.s32.ac[s][threadIdx.x] =
+                        make_float2(static_cast<float>(q.a * scale[s]), static_cast<float>(q.c * scale[s]));
                 } else {
                     sh.s64.D[s][threadIdx.x] = q.D;
                     sh.s64.a[s][threadIdx.x] = q.a;
@@ -295,26 +308,29 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, int mine
         __syncthreads();
         if (mine >= 0) {
             if constexpr (F32) {
-                const double *sD = sh.s32.D[mine];
-                const float4 *sq = sh.s32.q[mine];
+                // pairs of lines as 16-byte LDS reads off one base address
+                const double2 *sD = reinterpret_cast<const double2 *>(sh.s32.D[mine]);
+                const double2 *sh_ = reinterpret_cast<const double2 *>(sh.s32.h[mine]);
+                const float4 *sac = reinterpret_cast<const float4 *>(sh.s32.ac[mine]);
                 for (int j0 = 0; j0 < cntp; j0 += kSub) {
                     float p[K];
 #pragma unroll
                     for (int k = 0; k < K; ++k) p[k] = 0.0f;
-                    // two lines over one denominator: n1/d1 + n2/d2 = (n1 d2 + n2 d1)/(d1 d2),
+                    // two lines over one denominator: n1/e1 + n2/e2 = (n1 e2 + n2 e1)/(e1 e2),
                     // one reciprocal per pair (denominators stay within 1e-17..1e27 here)
 #pragma unroll
                     for (int u = 0; u < kSub; u += 2) {
-                        const double D1 = sD[j0 + u], D2 = sD[j0 + u + 1];
-                        const float4 q1 = sq[j0 + u], q2 = sq[j0 + u + 1];
+                        const double2 Dp = sD[(j0 + u) / 2], hp = sh_[(j0 + u) / 2];
+                        const float4 qp = sac[(j0 + u) / 2];
+                        const double D1 = Dp.x, D2 = Dp.y, h1 = hp.x, h2 = hp.y;
+                        const float2 q1 = make_float2(qp.x, qp.y), q2 = make_float2(qp.z, qp.w);
 #pragma unroll
                         for (int k = 0; k < K; ++k) {
-                            const float d1 = static_cast<float>(x[k] - D1);
-                            const float d2 = static_cast<float>(x[k] - D2);
+                            const double d1 = x[k] - D1, d2 = x[k] - D2;
+                            const float e1 = static_cast<float>(fma(d1, d1, h1));
+                            const float e2 = static_cast<float>(fma(d2, d2, h2));
                             const float n1 = fmaf(q1.x, xf[k], q1.y);
                             const float n2 = fmaf(q2.x, xf[k], q2.y);
-                            const float e1 = fmaf(d1, d1, fmaf(q1.z, xf[k], q1.w));
-                            const float e2 = fmaf(d2, d2, fmaf(q2.z, xf[k], q2.w));
                             const float num = fmaf(n2, e1, n1 * e2);
                             p[k] = fmaf(num, __builtin_amdgcn_rcpf(e1 * e2), p[k]);
                         }
@@ -558,7 +574,7 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
     const int layer = blockIdx.x / bpl, bx = blockIdx.x % bpl;
     const int part = blockIdx.y, nsplit = gridDim.y;
     const LayerT<double> L = layer_of(A, layer);
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
     const int t = bx * (kThreads / 64) + wave;
     const int regime = wave_regime(A, t);                 // -1: no columns for this wavefront
     int need_inv = 0;
@@ -747,11 +763,11 @@ constexpr int kSubT = 4;
 template <bool F32>
 struct TanLines;
 template <>
-struct TanLines<true> {
-    double D[2][kWaveLines + kSetPad];
-    float4 q[2][kWaveLines + kSetPad];       // a s, c s, g, G
-    float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s, c s, g, G)
-    float dD[2][2][kWaveLines + kSetPad];    // [set][slot] dD
+struct TanLines<true> {   // shifted denominator (x - Ds)^2 + h, as the forward FP32 staging
+    double D[2][kWaveLines + kSetPad];       // Ds
+    double h[2][kWaveLines + kSetPad];
+    float2 ac[2][kWaveLines + kSetPad];      // a s, c s
+    float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s), d(c s), dh, dDs
 };
 template <>
 struct TanLines<false> {
@@ -793,16 +809,17 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                 const int sl[2] = {0, qslot};
                 if constexpr (F32) {
                     const double sc = scale[s];
-                    st.D[s][lane] = q.D.v;
-                    st.q[s][lane] = make_float4(static_cast<float>(q.a.v * sc), static_cast<float>(q.c.v * sc),
-                                                static_cast<float>(q.g.v), static_cast<float>(q.G.v));
+                    Dual Ds, h;
+                    shifted_denominator(q, Ds, h);
+                    st.D[s][lane] = Ds.v;
+                    st.h[s][lane] = h.v;
+                    st.ac[s][lane] = make_float2(static_cast<float>(q.a.v * sc), static_cast<float>(q.c.v * sc));
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
                         const int d = sl[m];
                         st.dq[s][m][lane] =
                             make_float4(static_cast<float>(q.a.d[d] * sc), static_cast<float>(q.c.d[d] * sc),
-                                        static_cast<float>(q.g.d[d]), static_cast<float>(q.G.d[d]));
-                        st.dD[s][m][lane] = static_cast<float>(q.D.d[d]);
+                                        static_cast<float>(h.d[d]), static_cast<float>(Ds.d[d]));
                     }
                 } else {
                     const Dual *c5[5] = {&q.D, &q.a, &q.c, &q.g, &q.G};
@@ -822,10 +839,11 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
 #pragma unroll
                 for (int u = 0; u < kSubT; ++u) {
                     const int j = j0 + u;
-                    const float d = static_cast<float>(x - st.D[mine][j]);
-                    const float4 q = st.q[mine][j];
-                    const float N = fmaf(q.x, xf, q.y);
-                    const float Q = fmaf(d, d, fmaf(q.z, xf, q.w));
+                    const double dd = x - st.D[mine][j];
+                    const float Q = static_cast<float>(fma(dd, dd, st.h[mine][j]));
+                    const float m2d = -2.0f * static_cast<float>(dd);
+                    const float2 ac = st.ac[mine][j];
+                    const float N = fmaf(ac.x, xf, ac.y);
                     const float r = __builtin_amdgcn_rcpf(Q);
                     const float t = N * r;
                     p += t;
@@ -833,7 +851,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                     for (int m = 0; m < 2; ++m) {
                         const float4 dq = st.dq[mine][m][j];
                         const float dN = fmaf(dq.x, xf, dq.y);
-                        const float dQ = fmaf(-2.0f * d, st.dD[mine][m][j], fmaf(dq.z, xf, dq.w));
+                        const float dQ = fmaf(m2d, dq.w, dq.z);   // d[(x - Ds)^2 + h]
                         dp[m] = fmaf(fmaf(-t, dQ, dN), r, dp[m]);
                     }
                 }
@@ -873,7 +891,7 @@ __device__ inline Dual wave_sum(const Dual &v) {
 
 template <bool F32>
 __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan<F32> &sh) {
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
     const long long item = static_cast<long long>(blockIdx.x) * (kThreads / 64) + wave;
     const int tiles = (A.nf + 63) / 64;
     if (item >= static_cast<long long>(A.nl) * tiles) return;   // the whole wavefront: no block barriers follow

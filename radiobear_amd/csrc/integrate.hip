@@ -40,19 +40,21 @@ struct IntArgs {
 // The ray a wavefront integrates (-1: none) and its first layer row and layer
 // count; false when its atmosphere index or offsets are out of range.
 __device__ inline int ray_of(const IntArgs &A, int w) {
+    w = __builtin_amdgcn_readfirstlane(w);   // wavefront-uniform (threadIdx.x / 64 is)
     if (w >= A.nray) return -1;
-    const int r = A.order ? A.order[w] : w;
-    return (r >= 0 && r < A.nray) ? __builtin_amdgcn_readfirstlane(r) : -1;
+    const int r = __builtin_amdgcn_readfirstlane(A.order ? A.order[w] : w);
+    return (r >= 0 && r < A.nray) ? r : -1;
 }
 
 __device__ inline bool ray_span(const IntArgs &A, int ray, int &base, int &nl) {
     base = 0;
     nl = A.nl;
     if (!A.ray_atm) return true;
-    const int a = A.ray_atm[ray];
+    // the ray is the wavefront's: keep its atmosphere's offsets in scalar registers
+    const int a = __builtin_amdgcn_readfirstlane(A.ray_atm[ray]);
     if (a < 0 || a >= A.natm) return false;
-    base = A.layer_off[a];
-    const int end = A.layer_off[a + 1];
+    base = __builtin_amdgcn_readfirstlane(A.layer_off[a]);
+    const int end = __builtin_amdgcn_readfirstlane(A.layer_off[a + 1]);
     nl = end - base;
     return base >= 0 && end <= A.nl && nl >= 2 && nl - 1 <= A.ld;
 }
