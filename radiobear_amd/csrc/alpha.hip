@@ -567,6 +567,40 @@ __device__ __host__ inline int rounds_of(int n) { return (n + kThreads - 1) / kT
 __device__ inline bool nh3_uses_hs(int sel, double P) { return sel == 2 || (sel == 1 && P <= 2000.0); }
 __device__ inline bool nh3_uses_sjs(int sel, double P) { return sel == 3 || (sel == 1 && P >= 400.0); }
 
+// A layer's staging rounds per segment (h2o, h2s, nh3 inversion, rotational, v2,
+// ph3), the slot each segment sums into, and the round range of each part.
+constexpr int kSegs = 6;
+constexpr int kSegSlot[kSegs] = {kSlotH2o, kSlotH2s, kSlotNh3, kSlotNh3, kSlotNh3, kSlotPh3};
+constexpr int kSegKind[kSegs] = {RBX_LINES_H2O, RBX_LINES_H2S, RBX_LINES_NH3_INV,
+                                 RBX_LINES_NH3_ROT, RBX_LINES_NH3_V2, RBX_LINES_PH3};
+
+struct LinePlan {
+    int nr[kSegs];
+    int R;
+    __device__ LinePlan(const AlphaArgs &A, double P) {
+        const bool hs = A.abs.nh3 && nh3_uses_hs(A.abs.nh3, P);
+        const bool on[kSegs] = {A.abs.h2o != 0, A.abs.h2s != 0, hs, hs, hs, A.abs.ph3 != 0};
+        R = 0;
+        for (int s = 0; s < kSegs; ++s) {
+            nr[s] = on[s] ? rounds_of(A.cat.blk[kSegKind[s]].n) : 0;
+            R += nr[s];
+        }
+    }
+    __device__ int begin(int part, int nsplit) const {
+        return static_cast<int>(static_cast<long long>(part) * R / nsplit);
+    }
+    // bitmask of the slots whose segments part `part` has rounds of
+    __device__ int slots(int part, int nsplit) const {
+        const int r0 = begin(part, nsplit), r1 = begin(part + 1, nsplit);
+        int m = 0, off = 0;
+        for (int s = 0; s < kSegs; ++s) {
+            if (max(r0, off) < min(r1, off + nr[s])) m |= 1 << kSegSlot[s];
+            off += nr[s];
+        }
+        return m;
+    }
+};
+
 template <int K, bool F32>
 __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
     __shared__ SharedFwd sh;
@@ -600,16 +634,9 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
     }
 
     const DevBlock *blk = A.cat.blk;
-    const bool hs = A.abs.nh3 && nh3_uses_hs(A.abs.nh3, L.P);
-    const int nr[6] = {A.abs.h2o ? rounds_of(blk[RBX_LINES_H2O].n) : 0,
-                       A.abs.h2s ? rounds_of(blk[RBX_LINES_H2S].n) : 0,
-                       hs ? rounds_of(blk[RBX_LINES_NH3_INV].n) : 0,
-                       hs ? rounds_of(blk[RBX_LINES_NH3_ROT].n) : 0,
-                       hs ? rounds_of(blk[RBX_LINES_NH3_V2].n) : 0,
-                       A.abs.ph3 ? rounds_of(blk[RBX_LINES_PH3].n) : 0};
-    const int R = nr[0] + nr[1] + nr[2] + nr[3] + nr[4] + nr[5];
-    const int r0 = static_cast<int>(static_cast<long long>(part) * R / nsplit);
-    const int r1 = static_cast<int>(static_cast<long long>(part + 1) * R / nsplit);
+    const LinePlan plan(A, L.P);
+    const int *nr = plan.nr;
+    const int r0 = plan.begin(part, nsplit), r1 = plan.begin(part + 1, nsplit);
     int seg = 0, off = 0, lb = 0, le = 0;
     // lines [lb, le) of the next segment that fall in this part's rounds
     auto next = [&](int n) {
@@ -676,17 +703,19 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
     }
     if (did_h2o) base = block_reduce_sum(base, sh.red);   // did_h2o is the same for the whole block
 
-    const int en[kSlots] = {A.abs.h2o, A.abs.h2s, A.abs.nh3, A.abs.ph3};
+    // only the slots this part has lines of; the finish kernel reads the same set
+    const int touched = plan.slots(part, nsplit);
     const size_t plane = static_cast<size_t>(A.nl) * A.nf;
 #pragma unroll
     for (int q = 0; q < kSlots; ++q) {
-        if (!en[q]) continue;
+        if (!(touched & (1 << q))) continue;
         double *w = A.ws + (static_cast<size_t>(q) * nsplit + part) * plane + static_cast<size_t>(layer) * A.nf;
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (ok[k]) w[col[k]] = acc[q][k];
     }
-    if (A.abs.h2o && bx == 0 && threadIdx.x == 0) A.ws_base[static_cast<size_t>(part) * A.nl + layer] = base;
+    if ((touched & (1 << kSlotH2o)) && bx == 0 && threadIdx.x == 0)
+        A.ws_base[static_cast<size_t>(part) * A.nl + layer] = base;
 }
 
 template <bool F32>
@@ -704,10 +733,12 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
     const double x = f[0] * f[0], xn = fn * fn;
     const size_t plane = static_cast<size_t>(nl) * A.nf;
     const size_t o = static_cast<size_t>(layer) * A.nf + cc;
+    const LinePlan plan(A, L.P);
     auto parts = [&](int q) {
         const double *w = A.ws + static_cast<size_t>(q) * nsplit * plane + o;
         double s = 0.0;
-        for (int p = 0; p < nsplit; ++p) s += w[static_cast<size_t>(p) * plane];
+        for (int p = 0; p < nsplit; ++p)
+            if (plan.slots(p, nsplit) & (1 << q)) s += w[static_cast<size_t>(p) * plane];
         return s;
     };
 
@@ -720,7 +751,8 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
     if (A.abs.h2) emit(RBX_C_H2, h2_cia(L, A.abs.h2) * x);
     if (A.abs.h2o) {
         double base = 0.0;
-        for (int p = 0; p < nsplit; ++p) base += A.ws_base[static_cast<size_t>(p) * nl + layer];
+        for (int p = 0; p < nsplit; ++p)
+            if (plan.slots(p, nsplit) & (1 << kSlotH2o)) base += A.ws_base[static_cast<size_t>(p) * nl + layer];
         emit(RBX_C_H2O, x * (parts(kSlotH2o) + base + h2o_continuum(L)));
     }
     if (A.abs.h2s) emit(RBX_C_H2S, x * parts(kSlotH2s));
