@@ -92,8 +92,7 @@ struct AlphaArgs {
     const double *scale;
     void *alpha;       // forward: alpha [nl][nf]; tangent: dalpha [3][nl][nf] (double)
     double *by_c;
-    int tiles_low;     // tiles covering the columns < nf_low
-    int tiles_total;   // forward kernel: tiles of both regimes
+    int tiles_total;   // forward kernel: wavefront tiles of tile_width columns
     int tile_width;    // columns per tile
     double *value;     // tangent: alpha [nl][nf] (float64) or null
     double *ws;        // forward: partial line sums [kSlots][nsplit][nl][nf]
@@ -242,10 +241,12 @@ struct Staged64 {
 // h = g (D - g/4) + G, the same polynomial in x as (x - D)^2 + g x + G; it is
 // evaluated in FP64 (h < 0 for lines wider than their centre, where the two
 // terms cancel) and rounded once, so each line costs one FMA less.
+// (set 1 starts 4 entries later than a multiple of the bank count, so a wavefront
+// whose columns read both sets does not conflict)
 struct Staged32 {
-    alignas(16) double D[2][kThreads];   // Ds
-    alignas(16) double h[2][kThreads];
-    alignas(16) float2 ac[2][kThreads];  // a, c (scaled)
+    alignas(16) double D[2][kThreads + 4];   // Ds
+    alignas(16) double h[2][kThreads + 4];
+    alignas(16) float2 ac[2][kThreads + 4];  // a, c (scaled)
 };
 
 template <class S>
@@ -267,18 +268,84 @@ union SharedFwd {
     double red[kThreads];
 };
 
+// ---- forward line sums of the staged lines for K frequencies per lane ----
+// set[k]: the coefficient set column k reads (the nh3_hs inversion lines have a
+// <= 30 GHz and a > 30 GHz set; every other block has one).  UNI: every column of
+// the wavefront reads the same set, so one LDS read serves all K columns.
+template <int K, bool UNI>
+__device__ __forceinline__ void sum_staged_f32(const Staged32 &st, const int (&set)[K], int cntp,
+                                               const double (&x)[K], const float (&xf)[K], double (&acc)[K]) {
+    for (int j0 = 0; j0 < cntp; j0 += kSub) {
+        float p[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) p[k] = 0.0f;
+        // two lines over one denominator: n1/e1 + n2/e2 = (n1 e2 + n2 e1)/(e1 e2),
+        // one reciprocal per pair (denominators stay within 1e-17..1e27 here)
+#pragma unroll
+        for (int u = 0; u < kSub; u += 2) {
+            double2 Dp[K], hp[K];
+            float4 qp[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (UNI && k > 0) {
+                    Dp[k] = Dp[0];
+                    hp[k] = hp[0];
+                    qp[k] = qp[0];
+                } else {   // pairs of lines as 16-byte LDS reads
+                    Dp[k] = reinterpret_cast<const double2 *>(st.D[set[k]])[(j0 + u) / 2];
+                    hp[k] = reinterpret_cast<const double2 *>(st.h[set[k]])[(j0 + u) / 2];
+                    qp[k] = reinterpret_cast<const float4 *>(st.ac[set[k]])[(j0 + u) / 2];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double d1 = x[k] - Dp[k].x, d2 = x[k] - Dp[k].y;
+                const float e1 = static_cast<float>(fma(d1, d1, hp[k].x));
+                const float e2 = static_cast<float>(fma(d2, d2, hp[k].y));
+                const float n1 = fmaf(qp[k].x, xf[k], qp[k].y);
+                const float n2 = fmaf(qp[k].z, xf[k], qp[k].w);
+                const float num = fmaf(n2, e1, n1 * e2);
+                p[k] = fmaf(num, __builtin_amdgcn_rcpf(e1 * e2), p[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += static_cast<double>(p[k]);
+    }
+}
+
+template <int K, bool UNI>
+__device__ __forceinline__ void sum_staged_f64(const Staged64 &st, const int (&set)[K], int cntp,
+                                               const double (&x)[K], double (&acc)[K]) {
+    for (int j = 0; j < cntp; ++j) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int m = UNI ? set[0] : set[k];
+            const double D = st.D[m][j], a = st.a[m][j], c = st.c[m][j], g = st.g[m][j], G = st.G[m][j];
+            const double d = x[k] - D;
+            const double num = fma(a, x[k], c);
+            const double den = fma(d, d, fma(g, x[k], G));
+            acc[k] += num / den;
+        }
+    }
+}
+
 // ---- forward line sum over lines [lb, le) of one block for K frequencies per lane ----
 // prep[0..1]: coefficient sets (two only for the nh3_hs inversion lines, whose
 // <= / > 30 GHz constants differ); `need` is the block's bitmask of sets to
-// stage, `mine` the set this wavefront sums (-1: the wavefront has no columns).
+// stage; `active`: the wavefront has columns (the others only help staging).
 // Staged lines are padded to a multiple of kSub with zero terms so the sum
 // loop has a fixed trip count and unrolls.
 template <int K, bool F32, class Prep>
-__device__ void line_sum(const DevBlock &B, const Prep *prep, int need, int mine, const double *scale,
-                         const double (&x)[K], double (&acc)[K], SharedFwd &sh, double &base, int lb, int le) {
+__device__ void line_sum(const DevBlock &B, const Prep *prep, int need, bool active, const int (&set)[K],
+                         const double *scale, const double (&x)[K], double (&acc)[K], SharedFwd &sh, double &base,
+                         int lb, int le) {
     float xf[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) xf[k] = static_cast<float>(x[k]);
+    bool uni = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) uni = uni && set[k] == set[0];
+    uni = __all(uni && set[0] == __builtin_amdgcn_readfirstlane(set[0]));
     for (int l0 = lb; l0 < le; l0 += kThreads) {
         const int i = l0 + threadIdx.x;
         const int cnt = min(kThreads, le - l0);
@@ -306,55 +373,17 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, int mine
             }
         }
         __syncthreads();
-        if (mine >= 0) {
+        if (active) {
             if constexpr (F32) {
-                // pairs of lines as 16-byte LDS reads off one base address
-                const double2 *sD = reinterpret_cast<const double2 *>(sh.s32.D[mine]);
-                const double2 *sh_ = reinterpret_cast<const double2 *>(sh.s32.h[mine]);
-                const float4 *sac = reinterpret_cast<const float4 *>(sh.s32.ac[mine]);
-                for (int j0 = 0; j0 < cntp; j0 += kSub) {
-                    float p[K];
-#pragma unroll
-                    for (int k = 0; k < K; ++k) p[k] = 0.0f;
-                    // two lines over one denominator: n1/e1 + n2/e2 = (n1 e2 + n2 e1)/(e1 e2),
-                    // one reciprocal per pair (denominators stay within 1e-17..1e27 here)
-#pragma unroll
-                    for (int u = 0; u < kSub; u += 2) {
-                        const double2 Dp = sD[(j0 + u) / 2], hp = sh_[(j0 + u) / 2];
-                        const float4 qp = sac[(j0 + u) / 2];
-                        const double D1 = Dp.x, D2 = Dp.y, h1 = hp.x, h2 = hp.y;
-                        const float2 q1 = make_float2(qp.x, qp.y), q2 = make_float2(qp.z, qp.w);
-#pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                            const double d1 = x[k] - D1, d2 = x[k] - D2;
-                            const float e1 = static_cast<float>(fma(d1, d1, h1));
-                            const float e2 = static_cast<float>(fma(d2, d2, h2));
-                            const float n1 = fmaf(q1.x, xf[k], q1.y);
-                            const float n2 = fmaf(q2.x, xf[k], q2.y);
-                            const float num = fmaf(n2, e1, n1 * e2);
-                            p[k] = fmaf(num, __builtin_amdgcn_rcpf(e1 * e2), p[k]);
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < K; ++k) acc[k] += static_cast<double>(p[k]);
-                }
+                if (uni)
+                    sum_staged_f32<K, true>(sh.s32, set, cntp, x, xf, acc);
+                else
+                    sum_staged_f32<K, false>(sh.s32, set, cntp, x, xf, acc);
             } else {
-                const double *sD = sh.s64.D[mine], *sa = sh.s64.a[mine], *sc = sh.s64.c[mine];
-                const double *sg = sh.s64.g[mine], *sG = sh.s64.G[mine];
-                for (int j0 = 0; j0 < cntp; j0 += 4) {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int j = j0 + u;
-                        const double D = sD[j], a = sa[j], c = sc[j], g = sg[j], G = sG[j];
-#pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                            double d = x[k] - D;
-                            double num = fma(a, x[k], c);
-                            double den = fma(d, d, fma(g, x[k], G));
-                            acc[k] += num / den;
-                        }
-                    }
-                }
+                if (uni)
+                    sum_staged_f64<K, true>(sh.s64, set, cntp, x, acc);
+                else
+                    sum_staged_f64<K, false>(sh.s64, set, cntp, x, acc);
             }
         }
         __syncthreads();
@@ -539,13 +568,6 @@ __device__ inline S nh3_select(int sel, double P, const S &hs, const S &sj) {
     return W * sj + (1.0 - W) * hs;
 }
 
-// Forward kernel tiling: a wavefront owns a tile of 64 K columns (lane + k*64)
-// inside one nh3_hs regime (columns < nf_low use the <= 30 GHz set); the 4
-// wavefronts of a block take consecutive tiles, so a block may span the switch.
-__device__ inline int wave_regime(const AlphaArgs &A, int t) {
-    return t < A.tiles_low ? 0 : (t < A.tiles_total ? 1 : -1);
-}
-
 // ---- forward path: split line sums, then a per-column finish ----
 //
 // A layer's lines are visited as one sequence of staging rounds (kThreads lines
@@ -604,31 +626,30 @@ struct LinePlan {
 template <int K, bool F32>
 __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
     __shared__ SharedFwd sh;
+    // wavefront tile t = columns [t 64K, (t + 1) 64K); a tile may straddle the nh3_hs
+    // 30 GHz switch (column nf_low): each column reads its own coefficient set
     const int bpl = (A.tiles_total + kThreads / 64 - 1) / (kThreads / 64);   // blocks per layer
     const int layer = blockIdx.x / bpl, bx = blockIdx.x % bpl;
     const int part = blockIdx.y, nsplit = gridDim.y;
     const LayerT<double> L = layer_of(A, layer);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
     const int t = bx * (kThreads / 64) + wave;
-    const int regime = wave_regime(A, t);                 // -1: no columns for this wavefront
-    int need_inv = 0;
-    for (int w = 0; w < kThreads / 64; ++w) {
-        const int r = wave_regime(A, bx * (kThreads / 64) + w);
-        if (r >= 0) need_inv |= 1 << r;
-    }
-    const int mine = regime >= 0 ? 0 : -1;
-    const int f_begin = regime == 0 ? t * A.tile_width : A.nf_low + (t - A.tiles_low) * A.tile_width;
-    const int f_end = regime == 0 ? A.nf_low : A.nf;
+    const bool active = t < A.tiles_total;                // the block's last wavefronts may have no columns
+    const int c0 = bx * kThreads * K, c1 = min(c0 + kThreads * K, A.nf);   // the block's columns
+    const int need_inv = (c0 < A.nf_low ? 1 : 0) | (c1 > A.nf_low ? 2 : 0);
+    const int one[K] = {};                                // every other block: set 0
 
-    int col[K];
+    int col[K], set[K];
     bool ok[K];
     double x[K], xn[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        col[k] = f_begin + lane + k * 64;
-        ok[k] = regime >= 0 && col[k] < f_end;
-        const double f = ok[k] ? A.freq[col[k]] : 1.0;
-        const double fn = ok[k] ? (A.freq_nh3 ? A.freq_nh3[col[k]] : f) : 1.0;
+        col[k] = t * A.tile_width + lane + k * 64;
+        ok[k] = active && col[k] < A.nf;
+        const int cc = min(col[k], A.nf - 1);
+        set[k] = cc < A.nf_low ? 0 : 1;
+        const double f = A.freq[cc];
+        const double fn = A.freq_nh3 ? A.freq_nh3[cc] : f;
         x[k] = f * f;
         xn[k] = fn * fn;
     }
@@ -654,7 +675,7 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         H2oPrep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_H2O].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_H2O], &prep, 1, mine, s, x, a2, sh, base, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_H2O], &prep, 1, active, one, s, x, a2, sh, base, lb, le);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[kSlotH2o][k] = a2[k] / s[0];
         did_h2o = true;
@@ -663,7 +684,7 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         H2sPrep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_H2S].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_H2S], &prep, 1, mine, s, x, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_H2S], &prep, 1, active, one, s, x, a2, sh, unused, lb, le);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[kSlotH2s][k] = a2[k] / s[0];
     }
@@ -672,16 +693,15 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         const double s[2] = {F32 ? pow2_scale(prep[0].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0,
                              F32 ? pow2_scale(prep[1].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_NH3_INV], prep, need_inv, regime, s, xn, a2, sh, unused, lb, le);
-        const double sm = s[regime > 0 ? 1 : 0];
+        line_sum<K, F32>(blk[RBX_LINES_NH3_INV], prep, need_inv, active, set, s, xn, a2, sh, unused, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / sm;
+        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[set[k]];
     }
     if (next(blk[RBX_LINES_NH3_ROT].n)) {
         RotPrep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_NH3_ROT].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_NH3_ROT], &prep, 1, mine, s, xn, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_NH3_ROT], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[0];
     }
@@ -689,7 +709,7 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         V2Prep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_NH3_V2].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_NH3_V2], &prep, 1, mine, s, xn, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_NH3_V2], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[0];
     }
@@ -697,7 +717,7 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         Ph3Prep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_PH3].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_PH3], &prep, 1, mine, s, x, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_PH3], &prep, 1, active, one, s, x, a2, sh, unused, lb, le);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[kSlotPh3][k] = a2[k] / s[0];
     }
@@ -1094,8 +1114,7 @@ static int alpha_tile_k() {
 
 static void set_wave_tiles(AlphaArgs &A, int K) {
     A.tile_width = 64 * K;   // one wavefront tile
-    A.tiles_low = (A.nf_low + A.tile_width - 1) / A.tile_width;
-    A.tiles_total = A.tiles_low + (A.nf - A.nf_low + A.tile_width - 1) / A.tile_width;
+    A.tiles_total = (A.nf + A.tile_width - 1) / A.tile_width;
 }
 
 template <int K>
