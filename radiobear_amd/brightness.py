@@ -31,16 +31,33 @@ class Brightness:
         tau_cut = self.tau_cut if self.tau_cut is not None else (50.0 if prec == 'fp32' else 0.0)
         return prec, tau_cut
 
-    def batch(self, bs, freqs, atm, alpha, orientation=None):
-        """Tb (nray, nf) numpy float64 for the rays bs; 'disc' rays use the disc average."""
-        if alpha.table is None:
-            alpha.get_layers(freqs, atm)
+    def start_paths(self, bs, atm, orientation=None):
+        """Launch the ray paths of bs on a side stream (they do not depend on the
+        absorption, so they run beside it); batch(..., paths=) joins them."""
         disc = [isinstance(b, str) or (not np.isscalar(b) and isinstance(b[0], str)) for b in bs]
         rays = np.array([[0.0, 0.0] if d else [float(b[0]), float(b[1])] for b, d in zip(bs, disc)])
-        hds, ns = raypath.batch(atm, rays, orientation)
+        main = torch.cuda.current_stream()
+        side = torch.cuda.Stream()
+        side.wait_stream(main)                    # inputs uploaded on the main stream
+        with torch.cuda.stream(side):
+            hds, ns = raypath.batch(atm, rays, orientation)
+        return disc, hds, ns, side
+
+    def batch(self, bs, freqs, atm, alpha, orientation=None, paths=None):
+        """Tb (nray, nf) numpy float64 for the rays bs; 'disc' rays use the disc average.
+        paths: what start_paths returned for these rays, or None to compute them here."""
+        if paths is None:
+            paths = self.start_paths(bs, atm, orientation)
+        disc, hds, ns, side = paths
+        if alpha.table is None:
+            alpha.get_layers(freqs, atm)
+        torch.cuda.current_stream().wait_stream(side)
+        hds.record_stream(torch.cuda.current_stream())
+        ns.record_stream(torch.cuda.current_stream())
+        rays = hds                                    # (row count only)
         T = atm.gas[atm.config.C['T']]
         prec, tau_cut = self._modes(alpha)
-        Tb = np.empty((len(rays), alpha.table.shape[1]))
+        Tb = np.empty((rays.shape[0], alpha.table.shape[1]))
         point = [i for i, d in enumerate(disc) if not d]
         if point:
             sel = torch.as_tensor(point, device=hds.device)

@@ -63,52 +63,25 @@ __device__ inline double load_alpha(const void *p, int f32, size_t i) {
     return f32 ? static_cast<double>(static_cast<const float *>(p)[i]) : static_cast<const double *>(p)[i];
 }
 
-// One step of brightness.py:93-101 (DISC: W = 2 a1 E2(tau)), with the trapezoid
-// sums regrouped by path point: sum_i (W_i + W_{i+1}) h_i = sum_k W_k c_k with
-// c_k = h_{k-1} + h_k (h_{-1} = 0; c of the last point is h_{n-1}), so a point adds
-// W c and W (T c) with one FMA each (W_0 = 0: the top point adds nothing); c and
-// T c are the same for every lane and are prepared once per batch of steps.
-// FAST: e^{-tau} advances by the step's factor e^{-dtau} = 2^{(a0 + a1) hl},
-// hl = -h log2(e) (FP32 exp2 of an FP64 exponent), and is recomputed from tau at the
-// start of every batch (refresh), so the factors' rounding never accumulates over
-// more than a batch.
-struct StepIn {
-    double h, c, Tc, hl;   // h_i, c_{i+1}, T_{i+1} c_{i+1}, -h_i log2(e)
-};
-
+// One trapezoid step of brightness.py:93-101 (DISC: W = 2 a1 E2(tau)).
 template <int MODE, bool FAST>
 struct Step {
-    double tau = 0.0, E = 1.0, intW = 0.0, num = 0.0, a0 = 0.0;
-    // step i: a1 = alpha at point i + 1
-    __device__ __forceinline__ void operator()(double a1, const StepIn &q) {
-        const double s = a0 + a1;
-        tau = fma(s, q.h, tau);
+    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0, a0 = 0.0;
+    __device__ __forceinline__ void operator()(double a1, double hh, double T1) {
+        tau = tau + (a0 + a1) * hh;
         double W1;
-        if constexpr (MODE == 0 && FAST) {
-            E = E * static_cast<double>(__builtin_amdgcn_exp2f(static_cast<float>(s * q.hl)));
-            W1 = a1 * E;
-        } else if constexpr (MODE == 1) {
+        if constexpr (MODE == 1)
             W1 = 2.0 * a1 * expint_e2(tau);
-        } else {
-            W1 = a1 * exp(-tau);
-        }
-        intW = fma(W1, q.c, intW);
-        num = fma(W1, q.Tc, num);
+        else
+            W1 = a1 * (FAST ? exp_neg_mixed(tau) : exp(-tau));
+        intW = intW + (W1 + W0) * hh;
+        const double TW1 = T1 * W1;
+        num = num + (TW1 + TW0) * hh;
         a0 = a1;
-    }
-    __device__ __forceinline__ void refresh() {
-        if constexpr (MODE == 0 && FAST) E = exp_neg_mixed(tau);
+        W0 = W1;
+        TW0 = TW1;
     }
 };
-
-__device__ __forceinline__ StepIn step_in(const double *h, const double *Tl, int i, int n) {
-    StepIn q;
-    q.h = h[i];
-    q.c = i + 1 < n ? q.h + h[i + 1] : q.h;
-    q.Tc = Tl[i + 1] * q.c;
-    q.hl = q.h * -kLog2e;
-    return q;
-}
 
 // Path steps are taken U at a time: the U rows of alpha, ds and T of a batch are
 // all requested before the first is used, so one memory latency covers U steps.
@@ -129,60 +102,28 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     const size_t nf = A.nf;
     const AT *al = static_cast<const AT *>(A.alpha) + static_cast<size_t>(base) * nf + jj;
     const double *Tl = A.T + base;
-    __shared__ StepIn stage[kWaves][U];            // this wavefront's batch of step inputs
     Step<MODE, FAST> st;
     if (S > 1) {
-        const int n = S - 1;                       // steps; h_0 .. h_{n-1}
+        const int n = S - 1;
         st.a0 = static_cast<double>(al[0]);
         int i = 0;
-        if (U <= n) {
-            // Batch b's alpha rows and step inputs are loaded while batch b - 1 is
-            // computed, so the wavefront waits for memory once, not once per batch.
+        for (; i + U <= n; i += U) {
             AT ab[U];
-            double hv = 0.0, hn = 0.0, tv = 0.0;   // lanes 0..U-1: h_i, h_{i+1}, T_{i+1} of step i + lane
-            auto load = [&](int i0, AT (&a)[U], double &h0, double &h1, double &t1) {
+            double hb[U], Tb[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) a[u] = al[(i0 + u + 1) * nf];
-                if (lane < U) {
-                    h0 = h[i0 + lane];
-                    h1 = i0 + lane + 1 < n ? h[i0 + lane + 1] : 0.0;
-                    t1 = Tl[i0 + lane + 1];
-                }
-            };
-            load(0, ab, hv, hn, tv);
-            for (; i + U <= n; i += U) {
-                AT an[U];
-                double hv2 = 0.0, hn2 = 0.0, tv2 = 0.0;
-                const bool more = i + 2 * U <= n;
-                if (more) load(i + U, an, hv2, hn2, tv2);
-                if (lane < U) {                    // lanes 0..U-1 prepare this batch's step inputs
-                    StepIn q;
-                    q.h = hv;
-                    q.c = hv + hn;
-                    q.Tc = tv * q.c;
-                    q.hl = hv * -kLog2e;
-                    stage[wave][lane] = q;
-                }
-                wave_sync();
-                st.refresh();
+            for (int u = 0; u < U; ++u) {
+                ab[u] = al[(i + u + 1) * nf];
+                hb[u] = h[i + u];
+                Tb[u] = Tl[i + u + 1];
+            }
 #pragma unroll
-                for (int u = 0; u < U; ++u) st(static_cast<double>(ab[u]), stage[wave][u]);
-                wave_sync();                       // read before the next batch overwrites
-                if (A.tau_cut > 0.0 && __all(!ok || st.tau > A.tau_cut)) {
-                    i = n;
-                    break;
-                }
-                if (more) {
-#pragma unroll
-                    for (int u = 0; u < U; ++u) ab[u] = an[u];
-                    hv = hv2;
-                    hn = hn2;
-                    tv = tv2;
-                }
+            for (int u = 0; u < U; ++u) st(static_cast<double>(ab[u]), hb[u], Tb[u]);
+            if (A.tau_cut > 0.0 && __all(!ok || st.tau > A.tau_cut)) {
+                i = n;
+                break;
             }
         }
-        st.refresh();
-        for (; i < n; ++i) st(static_cast<double>(al[(i + 1) * nf]), step_in(h, Tl, i, n));
+        for (; i < n; ++i) st(static_cast<double>(al[(i + 1) * nf]), h[i], Tl[i + 1]);
     }
     const double intW = st.intW, num = st.num;
     if (!ok) return;

@@ -145,6 +145,9 @@ class Planet:
         save_alpha = self.alpha_options[save_alpha[0].lower()]
         freqs, freqUnit = self.set_freqs(freqs, freqUnit)
         reuse = self.check_reuse(freqs, scale, get_alpha, save_alpha, reuse_override=reuse_override.lower())
+        self.set_b(b, block)
+        # the ray paths do not depend on the absorption: launched first, they run beside it
+        paths = self.bright.start_paths(self.b, self.atmos[0], self.config.orientation)
         if not reuse:
             self.freqs = freqs
             self.freqUnit = utils.proc_unit(freqUnit)
@@ -152,10 +155,10 @@ class Planet:
             self.get_alpha = get_alpha
             self.save_alpha = save_alpha
             self.alpha_layers(self.freqs, self.atmos, scale=scale, get_alpha=get_alpha, save_alpha=save_alpha)
-        self.set_b(b, block)
         is_img = self.set_image()
         run_start = datetime.datetime.now()
-        Tb = self.bright.batch(self.b, self.freqs, self.atmos[0], self.alpha[0], self.config.orientation)
+        Tb = self.bright.batch(self.b, self.freqs, self.atmos[0], self.alpha[0], self.config.orientation,
+                               paths=paths)
         run_stop = datetime.datetime.now()
         if is_img.true:
             n = self.imSize[0]
