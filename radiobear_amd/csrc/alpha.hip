@@ -115,19 +115,27 @@ struct InvPrep {   // nh3_hs inversion lines (nh3_hs.py:198-255)
         eT = -(1.0 / L.T - 1.0 / 300.0) * kHsHcKb;
         pref = k[13] * kHsCoef * (L.nh3 / 300.0) * pow(300.0 / L.T, 3.5) * ((2.0 / kPi) * (1.0 / kGHzToInvCm));
     }
-    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
-        const double fo = B.col[0][i], Io = B.col[1][i], Eo = B.col[2][i], go = B.col[3][i];
+    // the two coefficient sets share eT, so a line's weight before pref is common to both
+    static constexpr bool kShared = true;
+    __device__ S weight(const DevBlock &B, int i) const {
+        const double fo = B.col[0][i];
+        return exp(eT * B.col[2][i]) * (B.col[1][i] / (fo * fo));
+    }
+    __device__ CoefT<S> coef(const DevBlock &B, int i, const S &w) const {
+        const double fo = B.col[0][i], go = B.col[3][i];
         S gam = Ag + Bg * go;
         S zet = Az + Bz * go;
         S fd = fo + d * gam;
-        S W = pref * Io * exp(eT * Eo) / (fo * fo);
+        S W = pref * w;
         S Dd = fd * fd + gam * gam - zet * zet;
         return {Dd, W * (gam - zet), W * (gam + zet) * Dd, 4.0 * gam * gam, S(0.0)};
     }
+    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const { return coef(B, i, weight(B, i)); }
 };
 
 template <class S>
 struct RotPrep {   // nh3_hs rotational lines, Gross (nh3_hs.py:257-291)
+    static constexpr bool kShared = false;
     S c1, c2, c3, eT, pref;
     __device__ explicit RotPrep(const LayerT<S> &L) {
         S Tdiv = 300.0 / L.T;
@@ -147,6 +155,7 @@ struct RotPrep {   // nh3_hs rotational lines, Gross (nh3_hs.py:257-291)
 
 template <class S>
 struct V2Prep {    // nh3_hs v2 roto-vibrational lines, Gross (nh3_hs.py:293-328)
+    static constexpr bool kShared = false;
     S gam, eT, pref;
     __device__ explicit V2Prep(const LayerT<S> &L) {
         S Tdiv = 300.0 / L.T;
@@ -163,6 +172,7 @@ struct V2Prep {    // nh3_hs v2 roto-vibrational lines, Gross (nh3_hs.py:293-328
 
 template <class S>
 struct H2sPrep {   // h2s_ddb, Ben-Reuven with zeta = gamma (h2s_ddb.py:42-87)
+    static constexpr bool kShared = false;
     S Ag, Bg, del, eT, pref;
     __device__ explicit H2sPrep(const LayerT<S> &L) {
         const double T0 = 296.0;
@@ -185,6 +195,7 @@ struct H2sPrep {   // h2s_ddb, Ben-Reuven with zeta = gamma (h2s_ddb.py:42-87)
 
 template <class S>
 struct Ph3Prep {   // ph3_jh, Ben-Reuven with zeta = delta = 0 (ph3_jh.py:64-108)
+    static constexpr bool kShared = false;
     S A, Bs, eT, pref;
     __device__ explicit Ph3Prep(const LayerT<S> &L) {
         const double T0 = 300.0;
@@ -204,6 +215,7 @@ struct Ph3Prep {   // ph3_jh, Ben-Reuven with zeta = delta = 0 (ph3_jh.py:64-108
 
 template <class S>
 struct H2oPrep {   // h2o_bk lines, VVW with base (h2o_bk.py:83-99, 130-189)
+    static constexpr bool kShared = false;
     S lnTh, Th25, Th, h2o, h2, he, pref;
     __device__ explicit H2oPrep(const LayerT<S> &L) {
         const double M = 8.314472 / 0.46151805;
@@ -267,6 +279,19 @@ union SharedFwd {
     StagedSjs sjs;
     double red[kThreads];
 };
+
+// coefficient sets `need` of line i (the inversion sets share the line weight)
+template <class Prep, class S>
+__device__ __forceinline__ void line_coefs(const Prep *prep, int need, const DevBlock &B, int i, S &base,
+                                           CoefT<S> (&q)[2]) {
+    if constexpr (Prep::kShared) {
+        const S w = prep[0].weight(B, i);
+        if (need & 1) q[0] = prep[0].coef(B, i, w);
+        if (need & 2) q[1] = prep[1].coef(B, i, w);
+    } else {
+        if (need & 1) q[0] = prep[0](B, i, base);
+    }
+}
 
 // ---- forward line sums of the staged lines for K frequencies per lane ----
 // set[k]: the coefficient set column k reads (the nh3_hs inversion lines have a
@@ -351,11 +376,12 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, bool act
         const int cnt = min(kThreads, le - l0);
         const int cntp = (cnt + kSub - 1) / kSub * kSub;
         if (static_cast<int>(threadIdx.x) < cntp) {
+            CoefT<double> qs[2] = {{0.0, 0.0, 0.0, 0.0, 1.0}, {0.0, 0.0, 0.0, 0.0, 1.0}};   // padding: 0
+            if (i < le) line_coefs(prep, need, B, i, base, qs);
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 if (!(need & (1 << s))) continue;
-                CoefT<double> q = {0.0, 0.0, 0.0, 0.0, 1.0};   // padding: term = 0 / ((x)^2 + 1)
-                if (i < le) q = prep[s](B, i, base);
+                const CoefT<double> &q = qs[s];
                 if constexpr (F32) {
                     double Ds, h;
                     shifted_denominator(q, Ds, h);
@@ -816,10 +842,10 @@ template <bool F32>
 struct TanLines;
 template <>
 struct TanLines<true> {   // shifted denominator (x - Ds)^2 + h, as the forward FP32 staging
-    double D[2][kWaveLines + kSetPad];       // Ds
-    double h[2][kWaveLines + kSetPad];
-    float2 ac[2][kWaveLines + kSetPad];      // a s, c s
-    float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s), d(c s), dh, dDs
+    alignas(16) double D[2][kWaveLines + kSetPad];       // Ds
+    alignas(16) double h[2][kWaveLines + kSetPad];
+    alignas(16) float2 ac[2][kWaveLines + kSetPad];      // a s, c s
+    alignas(16) float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s), d(c s), dh, dDs
 };
 template <>
 struct TanLines<false> {
@@ -853,11 +879,13 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
         const int cnt = min(kWaveLines, B.n - l0);
         const int cntp = (cnt + kSubT - 1) / kSubT * kSubT;
         if (lane < cntp) {
+            CoefT<Dual> qs[2] = {{Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)},
+                                 {Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)}};   // padding: t = 0
+            if (i < B.n) line_coefs(prep, need, B, i, base, qs);
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 if (!(need & (1 << s))) continue;
-                CoefT<Dual> q = {Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)};   // padding: t = 0
-                if (i < B.n) q = prep[s](B, i, base);
+                const CoefT<Dual> &q = qs[s];
                 const int sl[2] = {0, qslot};
                 if constexpr (F32) {
                     const double sc = scale[s];
@@ -886,15 +914,21 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
         }
         wave_sync();
         if constexpr (F32) {
+            const double2 *sD = reinterpret_cast<const double2 *>(st.D[mine]);
+            const double2 *sH = reinterpret_cast<const double2 *>(st.h[mine]);
+            const float4 *sAC = reinterpret_cast<const float4 *>(st.ac[mine]);
             for (int j0 = 0; j0 < cntp; j0 += kSubT) {
                 float p = 0.0f, dp[2] = {0.0f, 0.0f};
 #pragma unroll
                 for (int u = 0; u < kSubT; ++u) {
                     const int j = j0 + u;
-                    const double dd = x - st.D[mine][j];
-                    const float Q = static_cast<float>(fma(dd, dd, st.h[mine][j]));
+                    // lines j, j + 1 come as one 16-byte read of each array (kSubT is even)
+                    const double2 D2 = sD[j / 2], H2 = sH[j / 2];
+                    const float4 AC2 = sAC[j / 2];
+                    const double dd = x - ((u & 1) ? D2.y : D2.x);
+                    const float Q = static_cast<float>(fma(dd, dd, (u & 1) ? H2.y : H2.x));
                     const float m2d = -2.0f * static_cast<float>(dd);
-                    const float2 ac = st.ac[mine][j];
+                    const float2 ac = (u & 1) ? make_float2(AC2.z, AC2.w) : make_float2(AC2.x, AC2.y);
                     const float N = fmaf(ac.x, xf, ac.y);
                     const float r = __builtin_amdgcn_rcpf(Q);
                     const float t = N * r;

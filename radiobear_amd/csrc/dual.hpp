@@ -34,8 +34,8 @@ __device__ inline Dual operator*(const Dual &a, const Dual &b) {
     return {a.v * b.v, a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]};
 }
 __device__ inline Dual operator/(const Dual &a, const Dual &b) {
-    const double q = a.v / b.v;
-    const double r = 1.0 / b.v;
+    const double r = 1.0 / b.v;   // one division; the quotient and tangents multiply by it
+    const double q = a.v * r;
     return {q, (a.d[0] - q * b.d[0]) * r, (a.d[1] - q * b.d[1]) * r, (a.d[2] - q * b.d[2]) * r};
 }
 __device__ inline Dual operator+(const Dual &a, double b) { return {a.v + b, a.d[0], a.d[1], a.d[2]}; }
@@ -44,7 +44,10 @@ __device__ inline Dual operator-(const Dual &a, double b) { return {a.v - b, a.d
 __device__ inline Dual operator-(double b, const Dual &a) { return {b - a.v, -a.d[0], -a.d[1], -a.d[2]}; }
 __device__ inline Dual operator*(const Dual &a, double b) { return {a.v * b, a.d[0] * b, a.d[1] * b, a.d[2] * b}; }
 __device__ inline Dual operator*(double b, const Dual &a) { return a * b; }
-__device__ inline Dual operator/(const Dual &a, double b) { return {a.v / b, a.d[0] / b, a.d[1] / b, a.d[2] / b}; }
+__device__ inline Dual operator/(const Dual &a, double b) {
+    const double r = 1.0 / b;
+    return {a.v * r, a.d[0] * r, a.d[1] * r, a.d[2] * r};
+}
 __device__ inline Dual operator/(double a, const Dual &b) {
     const double q = a / b.v;
     const double s = -q / b.v;
