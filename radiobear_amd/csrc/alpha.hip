@@ -305,8 +305,11 @@ __device__ __forceinline__ void sum_staged_f32(const Staged32 &st, const int (&s
 #pragma unroll
         for (int k = 0; k < K; ++k) p[k] = 0.0f;
         // two lines over one denominator: n1/e1 + n2/e2 = (n1 e2 + n2 e1)/(e1 e2),
-        // one reciprocal per pair (denominators stay within 1e-17..1e27 here)
-#pragma unroll
+        // one reciprocal per pair (denominators stay within 1e-17..1e27 here).  The
+        // mixed-set path (one wavefront per layer at most) reads K times as much
+        // and is not unrolled, to stay within the uniform path's registers.
+        constexpr int kPairUnroll = UNI ? kSub / 2 : 1;
+#pragma unroll kPairUnroll
         for (int u = 0; u < kSub; u += 2) {
             double2 Dp[K], hp[K];
             float4 qp[K];
