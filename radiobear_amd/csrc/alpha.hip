@@ -653,7 +653,7 @@ struct LinePlan {
 };
 
 template <int K, bool F32>
-__global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
+__global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(AlphaArgs A) {
     __shared__ SharedFwd sh;
     // wavefront tile t = columns [t 64K, (t + 1) 64K); a tile may straddle the nh3_hs
     // 30 GHz switch (column nf_low): each column reads its own coefficient set
@@ -697,7 +697,22 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         return a < b;
     };
 
-    double acc[kSlots][K] = {};
+    // each slot's segments are consecutive: a slot's partial is written as soon as
+    // its last segment is summed, so only one accumulator row is live at a time
+    // (the finish kernel reads exactly the slots a part touches)
+    const int touched = plan.slots(part, nsplit);
+    const size_t plane = static_cast<size_t>(A.nl) * A.nf;
+    double acc[K] = {};
+    auto store = [&](int q) {
+        if (touched & (1 << q)) {
+            double *w = A.ws + (static_cast<size_t>(q) * nsplit + part) * plane + static_cast<size_t>(layer) * A.nf;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (ok[k]) w[col[k]] = acc[k];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    };
     double base = 0.0, unused = 0.0;
     bool did_h2o = false;
     if (next(blk[RBX_LINES_H2O].n)) {
@@ -706,17 +721,19 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         double a2[K] = {};
         line_sum<K, F32>(blk[RBX_LINES_H2O], &prep, 1, active, one, s, x, a2, sh, base, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[kSlotH2o][k] = a2[k] / s[0];
+        for (int k = 0; k < K; ++k) acc[k] = a2[k] / s[0];
         did_h2o = true;
     }
+    store(kSlotH2o);
     if (next(blk[RBX_LINES_H2S].n)) {
         H2sPrep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_H2S].wscale) : 1.0, 1.0};
         double a2[K] = {};
         line_sum<K, F32>(blk[RBX_LINES_H2S], &prep, 1, active, one, s, x, a2, sh, unused, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[kSlotH2s][k] = a2[k] / s[0];
+        for (int k = 0; k < K; ++k) acc[k] = a2[k] / s[0];
     }
+    store(kSlotH2s);
     if (next(blk[RBX_LINES_NH3_INV].n)) {
         const InvPrep<double> prep[2] = {InvPrep<double>(L, 0), InvPrep<double>(L, 1)};
         const double s[2] = {F32 ? pow2_scale(prep[0].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0,
@@ -724,7 +741,7 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         double a2[K] = {};
         line_sum<K, F32>(blk[RBX_LINES_NH3_INV], prep, need_inv, active, set, s, xn, a2, sh, unused, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[set[k]];
+        for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[set[k]];
     }
     if (next(blk[RBX_LINES_NH3_ROT].n)) {
         RotPrep<double> prep(L);
@@ -732,7 +749,7 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         double a2[K] = {};
         line_sum<K, F32>(blk[RBX_LINES_NH3_ROT], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[0];
+        for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
     }
     if (next(blk[RBX_LINES_NH3_V2].n)) {
         V2Prep<double> prep(L);
@@ -740,29 +757,20 @@ __global__ __launch_bounds__(kThreads, 4) void alpha_lines_kernel(AlphaArgs A) {
         double a2[K] = {};
         line_sum<K, F32>(blk[RBX_LINES_NH3_V2], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[kSlotNh3][k] += a2[k] / s[0];
+        for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
     }
+    store(kSlotNh3);
     if (next(blk[RBX_LINES_PH3].n)) {
         Ph3Prep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_PH3].wscale) : 1.0, 1.0};
         double a2[K] = {};
         line_sum<K, F32>(blk[RBX_LINES_PH3], &prep, 1, active, one, s, x, a2, sh, unused, lb, le);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[kSlotPh3][k] = a2[k] / s[0];
+        for (int k = 0; k < K; ++k) acc[k] = a2[k] / s[0];
     }
+    store(kSlotPh3);
     if (did_h2o) base = block_reduce_sum(base, sh.red);   // did_h2o is the same for the whole block
 
-    // only the slots this part has lines of; the finish kernel reads the same set
-    const int touched = plan.slots(part, nsplit);
-    const size_t plane = static_cast<size_t>(A.nl) * A.nf;
-#pragma unroll
-    for (int q = 0; q < kSlots; ++q) {
-        if (!(touched & (1 << q))) continue;
-        double *w = A.ws + (static_cast<size_t>(q) * nsplit + part) * plane + static_cast<size_t>(layer) * A.nf;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (ok[k]) w[col[k]] = acc[q][k];
-    }
     if ((touched & (1 << kSlotH2o)) && bx == 0 && threadIdx.x == 0)
         A.ws_base[static_cast<size_t>(part) * A.nl + layer] = base;
 }
