@@ -244,6 +244,20 @@ int rbx_jacobian_batch(const void *alpha, int alpha_precision, const double *dal
                        const int *nsteps, const int *ray_atm, const int *order, int nray, int precision,
                        double tau_cut, float *J, int ldj, double *Tb, void *stream);
 
+/* ---- atmosphere construction (computeProp on the device, atm_base.py:120-181) ----
+ * For natm atmospheres laid end to end (layer_off as above), each ordered top
+ * (lowest P) to bottom, from P [nl_total] bar, T [nl_total] K and
+ * mix [RBX_NUM_SPECIES][nl_total]:
+ *   Z [nl_total] km: copied from Z_in, or (Z_in NULL) hydrostatic with the deepest
+ *     layer at 0: dz = (H_i + H_{i+1})/2 ln(P_{i+1}/P_i), H = R T / (m g), g = gm / req^2
+ *     (a T perturbation moves the shells, which the ray paths then follow);
+ *   R = req + Z - Z(layer nearest p_ref), N = 1 + 1e-6 sum_s refr_s(T) q_s P 293/T.
+ * gm in km^3/s^2 (config.GM_ref), req in km (config.Req), p_ref in bar.
+ */
+int rbx_atmosphere_batch(const double *P, const double *T, const double *mix, const double *Z_in,
+                         int nl_total, const int *layer_off, int natm, double req, double gm,
+                         double p_ref, double *Z, double *R, double *N, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'librbx.so')
-SOURCES = ['catalog.hip', 'alpha.hip', 'raypath.hip', 'integrate.hip']
+SOURCES = ['catalog.hip', 'alpha.hip', 'raypath.hip', 'integrate.hip', 'atmosphere.hip']
 
 
 def hipcc():

@@ -385,3 +385,22 @@ def integrate_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, order=None,
                                     _ptr(hds), ld, _ptr(nsteps), _ptr(ra), _ptr(od), nray,
                                     1 if disc else 0, prec, float(tau_cut), _ptr(Tb), None, _stream()))
     return Tb
+
+
+def atmosphere_batch(P, T, mix, layer_off, req, gm, p_ref=1.0, Z=None):
+    """Z, R, N (each (nl_total,) float64 on the device) of atmospheres laid end to end;
+    Z None: hydrostatic z (rbx_atmosphere_batch)."""
+    P, T, mix = as_device(P), as_device(T), as_device(mix)
+    nlt = P.shape[0]
+    if T.shape != (nlt,) or mix.shape != (len(_lib.SPECIES), nlt):
+        raise ValueError('shape mismatch: P {}, T {}, mix {}'.format(tuple(P.shape), tuple(T.shape),
+                                                                     tuple(mix.shape)))
+    off = np.asarray(layer_off.cpu() if isinstance(layer_off, torch.Tensor) else layer_off, dtype=np.int64)
+    if off[0] != 0 or off[-1] != nlt or np.any(np.diff(off) < 1):
+        raise ValueError('layer offsets must run from 0 to nl_total')
+    off_d = _i32(off)
+    Zin = None if Z is None else as_device(Z)
+    Zo, R, N = (torch.empty((nlt,), dtype=torch.float64, device=device()) for _ in range(3))
+    check(lib().rbx_atmosphere_batch(_ptr(P), _ptr(T), _ptr(mix), _ptr(Zin), nlt, _ptr(off_d), len(off) - 1,
+                                     float(req), float(gm), float(p_ref), _ptr(Zo), _ptr(R), _ptr(N), _stream()))
+    return Zo, R, N

@@ -30,7 +30,7 @@ class RetrievalBatch:
     """
 
     def __init__(self, atmospheres, catalog, absorbers, freqs, rays, req, rpol, C, precision='fp32',
-                 tau_cut=None, orientation=(0.0, 0.0)):
+                 tau_cut=None, orientation=(0.0, 0.0), gm=None, p_ref=1.0, device_props=False):
         if precision not in _lib.PRECISIONS:
             raise ValueError('precision must be fp32 or fp64')
         self.precision = precision
@@ -64,6 +64,23 @@ class RetrievalBatch:
         self.ray_atm_d = K._i32(self.ray_atm)
         self.order_d = K._i32(self.order)
         self.planet = K.planet_desc(req, rpol, orientation)
+        self.req, self.p_ref = float(req), float(p_ref)
+        self.gm = None if gm is None else float(gm)
+        if device_props:
+            self.update()
+
+    def update(self, T=None, mix=None):
+        """New layer temperatures and/or mixing ratios (concatenated layout, host or device):
+        the shells move with them — hydrostatic z, R and N are rebuilt on the device
+        (rbx_atmosphere_batch; needs gm, the planet's GM in km^3/s^2)."""
+        if self.gm is None:
+            raise ValueError('RetrievalBatch needs gm (km^3/s^2) to rebuild the atmosphere')
+        if T is not None:
+            self.T = K.as_device(T)
+        if mix is not None:
+            self.mix = K.as_device(mix)
+        self.Z, self.R, self.N = K.atmosphere_batch(self.P, self.T, self.mix, self.layer_off, self.req, self.gm,
+                                                    self.p_ref)
 
     def absorption(self):
         """alpha (nl_total, nf) and dalpha (3, nl_total, nf), float64, one fused pass."""
@@ -108,6 +125,7 @@ def synthetic(n_atm=1024, nr=16, seed_rays=None, precision='fp32', nh3_lines=100
     freqs = np.linspace(1.0, 50.0, nf)
     rays = synth.random_disc_rays(nr, s_rays if seed_rays is None else seed_rays, 0.95)
     a0 = atms[0]
+    kw.setdefault('gm', synth.JUPITER_GM)
     batch = RetrievalBatch(atms, cat, ab, freqs, rays, a0.Req, a0.Rpol, synth.GAS_ROWS, precision=precision, **kw)
     batch.atmospheres = atms
     batch.nh3_inv = inv

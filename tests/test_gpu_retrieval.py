@@ -132,3 +132,49 @@ def test_retrieval_rows_vs_torch_autograd(K, raw_lines, jupiter_lines):
                 scale = np.max(np.abs(Jr[:, slot]), axis=1, keepdims=True) + 1e-30
                 err = np.max(np.abs(J[rows, slot, :atm.nl] - Jr[:, slot]) / scale)
                 assert err < 1e-3, (prec, a, slot, err)
+
+
+def test_device_atmosphere_matches_host_construction(K):
+    """Hydrostatic z, R and N on the device vs the host construction of the same
+    perturbed atmospheres (synth.jupiter_like's integration), to rounding."""
+    atms = synth.retrieval_batch(5, seed=3, nl_range=(40, 90))
+    off = np.concatenate([[0], np.cumsum([a.nl for a in atms])])
+    cat = lambda f: np.concatenate([f(a) for a in atms])
+    mix = np.concatenate([K.mix_table(a.gas, C) for a in atms], axis=1)
+    Z, R, N = K.atmosphere_batch(cat(lambda a: a.P), cat(lambda a: a.T), mix, off, synth.JUPITER_REQ,
+                                 synth.JUPITER_GM)
+    Z, R, N = Z.cpu().numpy(), R.cpu().numpy(), N.cpu().numpy()
+    assert np.max(np.abs(Z - cat(lambda a: a.gas[C['Z']]))) < 1e-9               # km
+    assert np.max(np.abs(R - cat(lambda a: a.R)) / cat(lambda a: a.R)) < 1e-14
+    assert np.max(np.abs(N - cat(lambda a: a.N))) < 1e-15
+
+
+def test_device_atmosphere_given_z_matches_oracle_compute_prop(K, bench_gas_text):
+    """With z from the benchmark atmosphere file, R and N equal the oracle's computeProp
+    (itself pinned to the reference's benchmark fixture)."""
+    from oracle import pipeline
+    from oracle import atmosphere as OA
+    gas, prop = pipeline.benchmark_atmosphere(bench_gas_text)
+    mix = K.mix_table(gas, C)
+    P, T = gas[C['P']], gas[C['T']]
+    Z, R, N = K.atmosphere_batch(P, T, mix, [0, gas.shape[1]], pipeline.JUPITER_REQ, synth.JUPITER_GM,
+                                 Z=gas[C['Z']])
+    ref = OA.compute_prop(gas, C, pipeline.JUPITER_REQ)
+    assert np.max(np.abs(R.cpu().numpy() - ref['R']) / ref['R']) < 1e-14
+    assert np.max(np.abs(N.cpu().numpy() - ref['N'])) < 1e-15
+    assert np.max(np.abs(R.cpu().numpy() - prop['R']) / prop['R']) < 1e-14
+
+
+def test_retrieval_with_device_atmosphere(K, small_batch):
+    """The batch rebuilt on the device (hydrostatic z) integrates to the same Tb and J as
+    with the host-built shells."""
+    import torch
+    from radiobear_amd.retrieval import RetrievalBatch
+    atms, cat, ab, f, rays, batches = small_batch
+    a0 = atms[0]
+    rb = RetrievalBatch(atms, cat, ab, f, rays, a0.Req, a0.Rpol, C, precision='fp64', gm=synth.JUPITER_GM,
+                        device_props=True)
+    Tb, J = rb.run()
+    Tb0, J0 = batches['fp64'].run()
+    assert torch.max(torch.abs(Tb - Tb0)).item() < 1e-9
+    assert torch.max(torch.abs(J - J0)).item() <= 1e-6 * torch.max(torch.abs(J0)).item()
