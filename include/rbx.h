@@ -160,6 +160,16 @@ int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
 int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
                              int nf_nh3_low, int precision, size_t *bytes);
 
+/* ---- reuse of a saved absorption table (Alpha get_alpha='memory'/'file',
+ * alpha.py:110-149 and total_layer_alpha alpha.py:151-192) ----
+ * absorb          [nl][nf][nc] float64 per-constituent absorption in the saved
+ *                 column order (Alpha.alpha_data)
+ * scale           [nc][nl] per-constituent per-layer factor, or NULL for 1.0
+ * alpha           [nl][nf] float64 (RBX_FP64) or float32 (RBX_FP32): sum_c scale * absorb
+ */
+int rbx_alpha_rescale(const double *absorb, int nl, int nf, int nc, const double *scale, int precision,
+                      void *alpha, void *stream);
+
 /* ---- absorption and its tangents for the adjoint (one fused pass) ----
  * Same inputs as rbx_alpha.  precision RBX_FP64 (plain FP64) or RBX_FP32
  * (FP32 line sums as in rbx_alpha; tangents good to ~1e-6 relative).

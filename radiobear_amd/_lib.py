@@ -34,7 +34,7 @@ EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_tune', 
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
            'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles',
            'rbx_jacobian', 'rbx_raypaths_batch', 'rbx_integrate_batch', 'rbx_jacobian_batch',
-           'rbx_atmosphere_batch')
+           'rbx_atmosphere_batch', 'rbx_alpha_rescale')
 
 
 class RbxLibraryError(RuntimeError):
@@ -82,6 +82,7 @@ _SIGS = {
     'rbx_integrate': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _VP]),
     'rbx_integrate_profiles': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
     'rbx_jacobian': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP]),
+    'rbx_alpha_rescale': (_I, [_VP, _I, _I, _I, _VP, _I, _VP, _VP]),
     'rbx_atmosphere_batch': (_I, [_VP, _VP, _VP, _VP, _I, _VP, _I, _D, _D, _D, _VP, _VP, _VP, _VP]),
     'rbx_raypaths_batch': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _VP, _I, _VP, _I, _VP, _I,
                                 _VP, _VP]),

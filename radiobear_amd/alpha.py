@@ -13,7 +13,6 @@ import os
 from argparse import Namespace
 
 import numpy as np
-import torch
 
 from . import _lib, kernels, lines, utils
 
@@ -162,12 +161,9 @@ class Alpha:
             self.read_alpha_data(get_alpha)
             data = np.asarray(self.alpha_data, dtype=np.float64)          # (nl, nf, nc)
             cols = [_lib.CONSTITUENTS.index(c) for c in self.ordered_constituents]
-            s = sc[cols].T[:, None, :]                                     # (nl, 1, nc)
-            dev = kernels.device()
-            tot = (torch.as_tensor(data, device=dev) * torch.as_tensor(s, device=dev)).sum(dim=2)
-            self.table = tot.to(torch.float32 if self.precision == 'fp32' else torch.float64).contiguous()
+            self.table = kernels.alpha_rescale(data, sc[cols], self.precision)
             if save_memfil:
-                self.tosave = data * s
+                self.tosave = data * sc[cols].T[:, None, :]
                 self.save_alpha_data(save_alpha)
             return
         P, T, mix = atm.gas[C['P']], atm.gas[C['T']], kernels.mix_table(atm.gas, C)

@@ -1329,3 +1329,42 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
                            0, st, A);
     return rbx::launch_status("alpha_tangent_kernel");
 }
+
+// ---- reuse path: a saved per-constituent absorption table, rescaled ----
+// alpha[l][f] = sum_c scale[c][l] absorb[l][f][c] in the saved column order (the
+// reference's Alpha.alpha_data layout, alpha.py:110-192): one thread per
+// (layer, frequency), HBM-bound.
+namespace rbx {
+template <bool F32>
+__global__ __launch_bounds__(256) void alpha_rescale_kernel(const double *absorb, int nl, int nf, int nc,
+                                                            const double *scale, void *alpha) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<long long>(nl) * nf) return;
+    const int l = static_cast<int>(i / nf);
+    const double *a = absorb + i * nc;
+    double t = 0.0;
+    for (int c = 0; c < nc; ++c) t += a[c] * (scale ? scale[static_cast<size_t>(c) * nl + l] : 1.0);
+    if (F32)
+        static_cast<float *>(alpha)[i] = static_cast<float>(t);
+    else
+        static_cast<double *>(alpha)[i] = t;
+}
+}  // namespace rbx
+
+extern "C" int rbx_alpha_rescale(const double *absorb, int nl, int nf, int nc, const double *scale, int precision,
+                                 void *alpha, void *stream) {
+    RBX_CHECK_ARG(absorb && alpha, "rbx_alpha_rescale: null device pointer");
+    RBX_CHECK_ARG(nl > 0 && nf > 0 && nc > 0 && nc <= 64, "rbx_alpha_rescale: nl=%d nf=%d nc=%d", nl, nf, nc);
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_rescale: bad precision %d", precision);
+    const long long n = static_cast<long long>(nl) * nf;
+    const long long blocks = (n + 255) / 256;
+    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_alpha_rescale: table too large");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (precision == RBX_FP32)
+        hipLaunchKernelGGL(rbx::alpha_rescale_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
+                           absorb, nl, nf, nc, scale, alpha);
+    else
+        hipLaunchKernelGGL(rbx::alpha_rescale_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
+                           absorb, nl, nf, nc, scale, alpha);
+    return rbx::launch_status("alpha_rescale_kernel");
+}

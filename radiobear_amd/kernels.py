@@ -163,6 +163,21 @@ def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by
     return (out, byc) if by_constituent else out
 
 
+def alpha_rescale(absorb, scale=None, precision='fp64'):
+    """alpha[nl][nf] = sum_c scale[c][l] absorb[l][f][c] (rbx_alpha_rescale) on the device."""
+    prec = _lib.PRECISIONS[precision]
+    a = as_device(absorb)
+    nl, nf, nc = a.shape
+    sc = None
+    if scale is not None:
+        sc = as_device(scale)
+        if sc.shape != (nc, nl):
+            raise ValueError('scale must be ({}, {})'.format(nc, nl))
+    out = torch.empty((nl, nf), dtype=torch.float32 if prec == _lib.FP32 else torch.float64, device=device())
+    check(lib().rbx_alpha_rescale(_ptr(a), nl, nf, nc, _ptr(sc), prec, _ptr(out), _stream()))
+    return out
+
+
 def alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec, reuse=None):
     """Device scratch for rbx_alpha (torch's stream-ordered allocator): (tensor, bytes).
     `reuse` is returned instead of a new tensor when it is large enough."""

@@ -329,3 +329,22 @@ def test_workload_full_shape_sampled(K, raw_lines, jupiter_lines, name):
         assert np.max(np.abs(Tb[i] - tb)) < 1e-6, i
         assert np.max(np.abs(Tb32[i] - tb)) < 0.01, i
     del torch
+
+
+def test_alpha_rescale_reuses_saved_constituents(K, jcat, bench_atm):
+    """Saved per-constituent absorption (Alpha.alpha_data layout, nl x nf x nc) rescaled on
+    the device equals rbx_alpha run with the same per-constituent scales, and numpy."""
+    gas, _ = bench_atm
+    g = gas[:, [3, 200, 400, 520]]
+    ab = K.absorbers_struct(JUPITER_ALPHA, 'e')
+    P, T, mix = g[C['P']], g[C['T']], K.mix_table(g, C)
+    rng = np.random.default_rng(5)
+    scale = rng.uniform(0.3, 3.0, size=(5, g.shape[1]))
+    ref = K.alpha(jcat, ab, P, T, mix, FREQS, 'fp64', scale=scale).cpu().numpy()
+    _, byc = K.alpha(jcat, ab, P, T, mix, FREQS, 'fp64', by_constituent=True)
+    absorb = np.transpose(byc.cpu().numpy(), (1, 2, 0))          # (nl, nf, nc), sorted constituents
+    got = K.alpha_rescale(absorb, scale, 'fp64').cpu().numpy()
+    assert rel(got, ref) < 1e-14
+    assert rel(got, np.einsum('lfc,cl->lf', absorb, scale)) < 1e-14
+    got32 = K.alpha_rescale(absorb, None, 'fp32').double().cpu().numpy()
+    assert rel(got32, absorb.sum(axis=2)) < 1e-7
