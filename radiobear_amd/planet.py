@@ -173,6 +173,10 @@ class Planet:
             self.header['orientation'] = '# orientation:   {}'.format(repr(cfg.orientation))
             self.header['aspect'] = '# aspect tip, rotate:  {:.4f}  {:.4f}'.format(utils.r2d(tip), utils.r2d(rotate))
             self.header['rNorm'] = '# rNorm: {}'.format(self.rNorm)
+            if self.data_type == 'image':                  # planet_base.py:370-375
+                self.header['imgSize'] = '# imgSize: {}'.format(self.imSize)
+                resolution = utils.r2asec(np.arctan(abs(self.b[1][0] - self.b[0][0]) * self.rNorm / cfg.distance))
+                self.header['res'] = '# res:  {} arcsec'.format(resolution)
         else:
             self.rNorm = None
             self.header['orientation'] = '# orientation not set'
@@ -189,6 +193,12 @@ class Planet:
         self.data_return.set('Tb', self.Tb)
         self.data_return.set('type', self.data_type)
         self.data_return.set('header', self.header)
+        if cfg.write_output_files:                          # planet.py:139-147
+            from .fileio import FileIO, output_name
+            out = output_name(self.planet, self.data_type, is_img.block, run_start, cfg.output_directory)
+            FileIO(cfg.output_directory, os.path.join(cfg.scratch_directory, 'specoutputline.dat')).write(
+                out, self.data_return)
+            self.output_file = out
         return self.data_return
 
 

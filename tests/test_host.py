@@ -96,3 +96,41 @@ def test_synthetic_atmosphere_generator():
     w2 = synth.workload('c3_large')
     assert np.array_equal(w1['rays'], w2['rays']) and np.array_equal(w1['nh3_inv']['fo'], w2['nh3_inv']['fo'])
     assert len(w1['freqs']) == 1024 and len(w1['rays']) == 16384 and len(w1['h2o']['f_o']) == 5000
+
+
+def test_result_files_in_reference_formats(tmp_path):
+    """FileIO writes the reference's text layouts (fileIO.py:19-104): sorted header lines,
+    spectrum rows '{:.2f}' + '  {:9.4f}' columns, profile rows, image rows, disc label."""
+    import datetime
+    from radiobear_amd.data_handling import Data
+    from radiobear_amd.fileio import FileIO, output_name
+    d = Data()
+    d.set('f', [1.5, 22.235])
+    d.set('freqUnit', 'GHz')
+    d.set('b', [[0.0, 0.0], [0.25, 0.5]])
+    d.set('Tb', [[200.25, 150.5], [180.125, 140.0]])
+    d.set('header', {'zz': '# last', 'aa': '# first'})
+    d.set('type', 'spectrum')
+    io = FileIO(str(tmp_path), str(tmp_path / 'line.dat'))
+    out = io.write(str(tmp_path / 's.dat'), d)
+    lines = open(out).read().split('\n')
+    assert lines[0] == '# first' and lines[1] == '# last'
+    assert lines[2] == '# GHz  K@b  \t(0.000,0.000)\t(0.250,0.500)'
+    f32 = [float(np.float32(v)) for v in (1.5, 22.235, 200.25, 150.5, 180.125, 140.0)]
+    assert lines[3] == '{:.2f}\t    {:9.4f}  \t  {:9.4f}'.format(f32[0], f32[2], f32[4])
+    assert lines[4] == '22.24\t    {:9.4f}  \t  {:9.4f}'.format(f32[3], f32[5])
+    assert open(tmp_path / 'line.dat').read().split('\n')[:2] == lines[3:5]
+    d.set('type', 'profile')
+    lines = open(io.write(str(tmp_path / 'p.dat'), d)).read().split('\n')
+    assert lines[2] == '# b  K@GHz \t     1.5000   \t    22.2350'
+    assert lines[3] == '0.000 0.000\t  200.25\t   150.50'
+    d.set('type', 'image')
+    lines = open(io.write(str(tmp_path / 'i.dat'), d)).read().split('\n')
+    assert lines[2] == '200.25\t 150.50' and lines[3] == '180.12\t 140.00'
+    d.set('b', 'disc')
+    d.set('Tb', [[200.0, 150.0]])
+    d.set('type', 'spectrum')
+    lines = open(io.write(str(tmp_path / 'd.dat'), d)).read().split('\n')
+    assert lines[2] == '# GHz  K@b  \tdisc'
+    t = datetime.datetime(2024, 5, 6, 7, 8, 9)
+    assert output_name('Jupiter', 'spectrum', '_', t, 'Out') == os.path.join('Out', 'Jupiter_spectrum_20240506_070809.dat')
