@@ -188,6 +188,24 @@ def autotune(prob, prec, args):
             'int_unroll': ki['best'], 'alpha_k_ms': ka['ms'], 'alpha_split_ms': ks['ms'], 'integrate_ms': ki['ms']}
 
 
+def other_workloads(args):
+    """The nadir-ray case (workload 1: 150 layers, 2000 NH3 lines, 20 freqs, FP64 plain; launch
+    bound, so its absolute time is the figure) and the image case (workload 2: 400 layers,
+    10000 lines, 512 freqs, 64x64 pixels; FP64 and FP32), one graph-replayed step each."""
+    from radiobear_amd import _lib
+    out = {}
+    for name, precs in (('c1_plain_fp64', ('fp64',)), ('c2_image', ('fp64', 'fp32'))):
+        prob = Problem(name)
+        rec = {'layers': prob.nl, 'freqs': prob.nf, 'rays': prob.nray}
+        for p in precs:
+            prec = _lib.PRECISIONS[p]
+            dt, _ = timed_graph(prob, prec, 20, 3)
+            on = int((prob.ns.cpu().numpy() >= 0).sum())
+            rec[p] = {'ms_per_step': dt * 1e3, 'ray_freq_Tb_per_s': on * prob.nf / dt, 'on_planet_rays': on}
+        out[name] = rec
+    return out
+
+
 def timed_graph(prob, prec, steps, warmup):
     import torch
     stream = torch.cuda.Stream()
@@ -394,6 +412,8 @@ def main():
         log(args, 'jacobian', res['jacobian'])
         res['retrieval'] = retrieval_rate(args)
         log(args, 'retrieval', res['retrieval'])
+        res['other_workloads'] = other_workloads(args)
+        log(args, 'other workloads', res['other_workloads'])
 
     # roofline of the dominant kernel (by duration)
     dom = max(kt32, key=kt32.get)

@@ -167,3 +167,26 @@ def test_planet_synthetic_image_and_profile():
     assert j.data_type == 'profile' and rv.Tb.shape == (4, 2)
     rv = j.run([5.0, 22.0], b='disc', reuse_override='False')
     assert rv.b == ['disc'] and rv.Tb.shape == (1, 2)
+
+
+def test_planet_run_writes_result_files(tmp_path):
+    """write_output_files: Planet.run writes the spectrum / image file (planet.py:139-147) with
+    the header, the b columns and Tb rows of the run it returns."""
+    from radiobear_amd.planet import Planet
+    atm = synth.jupiter_like(60)
+    j = Planet('jupiter', config_file=None, verbose=False, atmosphere=atm)
+    j.config.write_output_files = True
+    j.config.output_directory = str(tmp_path)
+    j.config.scratch_directory = str(tmp_path)
+    rv = j.run([5.0, 22.0], b=[[0.0, 0.0], [0.5, 0.0]], reuse_override='False')
+    text = open(j.output_file).read().split('\n')
+    assert os.path.basename(j.output_file).startswith('Jupiter_spectrum')
+    assert '#* type:  spectrum' in text
+    k = text.index('# GHz  K@b  \t(0.000,0.000)\t(0.500,0.000)')
+    row = text[k + 1].split()
+    assert row[0] == '5.00' and abs(float(row[1]) - rv.Tb[0][0]) < 1e-4 and abs(float(row[2]) - rv.Tb[1][0]) < 1e-4
+    rv = j.run([22.0], b=0.4, reuse_override='False')
+    text = open(j.output_file).read().split('\n')
+    assert any(t.startswith('# imgSize: ') for t in text) and any(t.startswith('# res:  ') for t in text)
+    rows = [t for t in text if t and not t.startswith('#')]
+    assert len(rows) == j.imSize[0]
