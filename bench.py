@@ -113,7 +113,7 @@ class Problem:
         out = self.alpha32 if prec == _lib.FP32 else self.alpha64
         self.ws, nbytes = K.alpha_workspace(self.cat, self.ab, self.nl, self.nf, self.nf_low, prec, reuse=self.ws)
         _lib.check(_lib.lib().rbx_alpha(self.cat.handle, ctypes.byref(self.ab), K._ptr(self.P), K._ptr(self.T),
-                                        K._ptr(self.mix), self.nl, K._ptr(self.f), None, self.nf, self.nf_low,
+                                        K._ptr(self.mix), None, self.nl, K._ptr(self.f), None, self.nf, self.nf_low,
                                         None, prec, K._ptr(out), None, K._ptr(self.ws), nbytes, K._stream()))
 
     def alpha_parts(self, prec):

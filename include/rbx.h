@@ -21,11 +21,15 @@
  *                       -> <constituent>.alpha -> total_layer_alpha
  *                       (radiobear/alpha.py:151-305; constituents/nh3/nh3_hs_sjs.py:6-26,
  *                        nh3/nh3_hs.py:96-340, nh3/nh3_sjs.py:26-128, h2o/h2o_bk.py:65-189,
- *                        h2s/h2s_ddb.py:42-87, ph3/ph3_jh.py:64-108, h2/h2_jj_ddb.py:7-38)
+ *                        h2s/h2s_ddb.py:42-87, ph3/ph3_jh.py:64-108, h2/h2_jj_ddb.py:7-38;
+ *                        the other planets' and optional absorbers nh3/nh3_dbs_sjs.py:6-26,
+ *                        nh3/nh3_dbs.py:66-313, co/co_ddb.py:22-97, clouds/clouds_idp.py:6-101,
+ *                        h2/h2_orton.py:17-222 with h2/h2_jj.py:7-22)
  *   rbx_alpha_tangent   (no reference counterpart: d alpha / d{T, q_NH3, q_H2O} for the adjoint)
  *   rbx_catalog_*       the line-list loads readInputFiles of the same modules
  *                       (nh3_hs.py:88-93, nh3_sjs.py:17-23, h2s_ddb.py:14-39,
- *                        ph3_jh.py:18-61, h2o_bk.py:17-62)
+ *                        ph3_jh.py:18-61, h2o_bk.py:17-62, co_ddb.py:14-19,
+ *                        h2_orton.py:17-97)
  *   rbx_raypaths        raypath.compute_ds / findEdge / Shape._calcEllipse
  *                       (radiobear/raypath.py:39-273, radiobear/shape.py:223-298)
  *   rbx_integrate       Brightness.single's tau / W / Tb loop and final spectrum
@@ -43,7 +47,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 2
+#define RBX_ABI_VERSION 3
 
 /* status codes */
 #define RBX_OK          0
@@ -67,7 +71,15 @@ extern "C" {
 #define RBX_LINES_PH3      5  /* f0, I0, E, WgtI0, WgtFGB, WgtSB  (ph3jh.npz + PH3WGT.npz) */
 #define RBX_LINES_H2O      6  /* f_o, I_o, E_o, w_s, x_s, w_h2, x_h2, w_he, x_he, SR
                                  (h2o_bk schema, widths in GHz/bar) */
-#define RBX_NUM_LINE_KINDS 7
+#define RBX_LINES_CO       7  /* f0, I0, E                                 (co.npz) */
+#define RBX_TABLE_H2_ORTON 8  /* Orton's H2 CIA tables; n = table frequencies, and the
+                                 columns are NOT n long: 0 frequencies (GHz, n, ascending),
+                                 1 natural-log absorption [6 n][RBX_ORTON_NTEMP] (tables
+                                 eh2h2, nh2h2, eh2he, nh2he, eh2ch4, nh2ch4, one row per
+                                 frequency), 2 the table temperatures (K, RBX_ORTON_NTEMP,
+                                 ascending) -- orton_H2.tables as h2_orton.py:17-97 reads it */
+#define RBX_NUM_LINE_KINDS 9
+#define RBX_ORTON_NTEMP    10
 
 /* constituent slots, in the reference's sorted order (alpha.py:80) */
 #define RBX_C_H2   0
@@ -75,7 +87,9 @@ extern "C" {
 #define RBX_C_H2S  2
 #define RBX_C_NH3  3
 #define RBX_C_PH3  4
-#define RBX_NUM_CONSTITUENTS 5
+#define RBX_C_CO   5
+#define RBX_C_CLOUDS 6
+#define RBX_NUM_CONSTITUENTS 7   /* the total sums them by name: clouds, co, h2, h2o, h2s, nh3, ph3 */
 
 /* rows of the mixing-ratio table mix[RBX_NUM_SPECIES][nl] */
 #define RBX_X_H2   0
@@ -85,17 +99,37 @@ extern "C" {
 #define RBX_X_H2O  4
 #define RBX_X_H2S  5
 #define RBX_X_PH3  6
-#define RBX_NUM_SPECIES 7
+#define RBX_X_CO   7   /* read by co_ddb only (its molar mass and refractivity are chemistry.py's NULL, 0) */
+#define RBX_NUM_SPECIES 8
+
+/* rows of the cloud-density table cloud[RBX_NUM_CLOUDS][nl] (g/cm^3, the reference's
+ * atm.cloud rows H2O, SOLN, NH4SH, NH3, H2S, CH4), in the order clouds_idp sums them */
+#define RBX_CL_H2O   0   /* water ice; other_dict flag ice_p */
+#define RBX_CL_SOLN  1   /* liquid water / solution; water_p */
+#define RBX_CL_NH4SH 2   /* nh4sh_p */
+#define RBX_CL_NH3   3   /* ammonia ice; nh3ice_p */
+#define RBX_CL_H2S   4   /* h2sice_p */
+#define RBX_CL_CH4   5   /* the module's flag is 'ch4' (clouds_idp.py:46) */
+#define RBX_NUM_CLOUDS 6
 
 typedef struct rbx_catalog rbx_catalog;
 
 /* Absorber selection: which formalism each constituent uses (0 = not loaded). */
 typedef struct rbx_absorbers {
-    int h2;    /* 1: h2_jj_ddb h2state 'e'; 2: h2_jj_ddb h2state 'n' */
+    int h2;    /* 1: h2_jj_ddb h2state 'e'; 2: h2_jj_ddb h2state 'n';
+                  3: h2_orton h2state 'e'; 4: h2_orton h2state 'n' */
     int h2o;   /* 1: h2o_bk */
     int h2s;   /* 1: h2s_ddb */
-    int nh3;   /* 1: nh3_hs_sjs; 2: nh3_hs; 3: nh3_sjs */
+    int nh3;   /* 1: nh3_hs_sjs; 2: nh3_hs; 3: nh3_sjs; 4: nh3_dbs_sjs; 5: nh3_dbs
+                  (constituents/nh3/nh3_dbs_sjs.py:6-26, nh3_dbs.py:66-313; the Saturn,
+                  Uranus and Neptune default, <planet>/config.par:23) */
     int ph3;   /* 1: ph3_jh */
+    int co;    /* 1: co_ddb (the Neptune default) */
+    int co_shape;     /* co_ddb coshape: 0 'voigt' (the default), 1 'vvw', 2 'diff' -- each the sum
+                         of the line shapes without line strengths, as co_ddb.py:86-91 -- or
+                         3 (any other value): pressure blend of the two, times the strengths */
+    int clouds;       /* 1: clouds_idp */
+    int cloud_types;  /* clouds_idp: bit RBX_CL_x set = that cloud's other_dict flag is > 0 */
 } rbx_absorbers;
 
 /* Planet shape and view for the ray tracer. */
@@ -134,6 +168,8 @@ int rbx_catalog_destroy(rbx_catalog *cat);
 /* ---- absorption: alpha[nl][nf] in cm^-1 ----
  * P, T            [nl]   bar, K (device)
  * mix             [RBX_NUM_SPECIES][nl] mixing ratios (device)
+ * cloud           [RBX_NUM_CLOUDS][nl] cloud densities, g/cm^3 (device); read only
+ *                 when clouds_idp is selected, may be NULL otherwise
  * freq            [nf]   GHz (device), the request order
  * freq_nh3        [nf]   GHz (device) or NULL: the frequency each output column
  *                 uses for NH3 (nh3_hs returns its <= 30 GHz half first,
@@ -150,7 +186,7 @@ int rbx_catalog_destroy(rbx_catalog *cat);
  *                 call that runs at the same time (a stream-ordered allocation is right)
  */
 int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
-              const double *P, const double *T, const double *mix, int nl,
+              const double *P, const double *T, const double *mix, const double *cloud, int nl,
               const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
               const double *scale, int precision, void *alpha, double *by_constituent,
               void *workspace, size_t workspace_bytes, void *stream);
@@ -171,7 +207,8 @@ int rbx_alpha_rescale(const double *absorb, int nl, int nf, int nc, const double
                       void *alpha, void *stream);
 
 /* ---- absorption and its tangents for the adjoint (one fused pass) ----
- * Same inputs as rbx_alpha.  precision RBX_FP64 (plain FP64) or RBX_FP32
+ * Same inputs as rbx_alpha, for the h2_jj_ddb, h2o_bk, h2s_ddb, nh3_* and ph3_jh
+ * absorbers (a selection with co_ddb, clouds_idp or h2_orton is rejected).  precision RBX_FP64 (plain FP64) or RBX_FP32
  * (FP32 line sums as in rbx_alpha; tangents good to ~1e-6 relative).
  * alpha           [nl][nf] float64 alpha, or NULL
  * dalpha          [3][nl][nf] float64: d alpha / d T, d alpha / d q_NH3,

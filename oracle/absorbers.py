@@ -317,6 +317,11 @@ HS_INV_LO = (1.640, 0.75, 0.852, 0.7756, 0.666, 1.0,
              1.262, 0.3, 0.5296, 0.7964, 0.667, 1.554, -0.0498, 0.9301)
 HS_INV_HI = (1.7465, 0.9779, 0.7298, 0.8202, 1.0, 1.0,
              1.2163, 0.0291, 0.5152, 0.8873, 0.8994, 2.0 / 3.0, -0.0627, 0.9862)
+# nh3_dbs differs from nh3_hs only in its <= 30 GHz inversion set (the Bellotti/Steffes
+# update, constituents/nh3/nh3_dbs.py:135-150); its > 30 GHz set and the rotational
+# and v2 blocks are nh3_hs's (nh3_dbs.py:151-165, 205-297)
+DBS_INV_LO = (1.6937, 0.6997, 0.7523, 0.8085, 1.0, 1.0,
+              1.3263, 0.1607, 0.6162, 0.8199, 0.0, 1.3832, -0.0139, 0.9619)
 
 
 def _gross(f, fo, gamma):
@@ -325,8 +330,9 @@ def _gross(f, fo, gamma):
     return (4.0 / np.pi) * f2 * gamma / (np.square(np.square(fo) - f2) + 4.0 * f2 * np.square(gamma))
 
 
-def _nh3_hs_part(freq, T, P, X, P_dict, data):
-    """nh3_hs.__alpha__ (constituents/nh3/nh3_hs.py:117-340) for one side of 30 GHz."""
+def _nh3_hs_part(freq, T, P, X, P_dict, data, low_set=HS_INV_LO):
+    """nh3_hs.__alpha__ (constituents/nh3/nh3_hs.py:117-340) for one side of 30 GHz;
+    with low_set=DBS_INV_LO it is nh3_dbs.__alpha__ (constituents/nh3/nh3_dbs.py:92-313)."""
     freq = np.asarray(freq, dtype=np.float64)
     P_h2 = P * X[P_dict['H2']]
     P_he = P * X[P_dict['HE']]
@@ -334,7 +340,7 @@ def _nh3_hs_part(freq, T, P, X, P_dict, data):
     Tdiv = HS_TO / T
     eta = 3.0 / 2.0
     if np.all(freq <= HS_F_SPLIT):
-        cs = HS_INV_LO
+        cs = low_set
     elif np.all(freq > HS_F_SPLIT):
         cs = HS_INV_HI
     else:
@@ -377,7 +383,7 @@ def _nh3_hs_part(freq, T, P, X, P_dict, data):
     return total
 
 
-def nh3_hs(freq, T, P, X, P_dict, other_dict, data=None, **kwargs):
+def nh3_hs(freq, T, P, X, P_dict, other_dict, data=None, low_set=HS_INV_LO, **kwargs):
     """nh3_hs.alpha: split at 30 GHz, low half first (nh3_hs.py:96-114).
 
     Units and the clamp alpha<0 -> 1e-8 are applied per half (nh3_hs.py:333-338).
@@ -387,7 +393,7 @@ def nh3_hs(freq, T, P, X, P_dict, other_dict, data=None, **kwargs):
     for sel in (freq <= HS_F_SPLIT, freq > HS_F_SPLIT):
         frq = freq[sel]
         if len(frq):
-            a = _nh3_hs_part(frq, T, P, X, P_dict, data)
+            a = _nh3_hs_part(frq, T, P, X, P_dict, data, low_set)
             if _units(kwargs) == 'dBperkm':
                 a = a * DB_PER_INVCM
             a[a < 0.0] = 1.0E-8
@@ -395,14 +401,29 @@ def nh3_hs(freq, T, P, X, P_dict, other_dict, data=None, **kwargs):
     return out
 
 
-def nh3_hs_sjs(freq, T, P, X, P_dict, other_dict, data_hs=None, data_sjs=None, **kwargs):
-    """Pressure blend of nh3_hs and nh3_sjs (constituents/nh3/nh3_hs_sjs.py:6-26)."""
+def nh3_dbs(freq, T, P, X, P_dict, other_dict, data=None, **kwargs):
+    """nh3_dbs.alpha (constituents/nh3/nh3_dbs.py:66-87): nh3_hs with the Bellotti/Steffes
+    <= 30 GHz inversion set."""
+    return nh3_hs(freq, T, P, X, P_dict, other_dict, data=data, low_set=DBS_INV_LO, **kwargs)
+
+
+def nh3_hs_sjs(freq, T, P, X, P_dict, other_dict, data_hs=None, data_sjs=None, low_set=HS_INV_LO,
+               **kwargs):
+    """Pressure blend of nh3_hs and nh3_sjs (constituents/nh3/nh3_hs_sjs.py:6-26); with
+    low_set=DBS_INV_LO the blend of nh3_dbs and nh3_sjs (nh3/nh3_dbs_sjs.py:6-26, the
+    same 400 / 2000 bar linear weight)."""
     PLower, PHigher = 400.0, 2000.0
     if P < PLower:
-        return nh3_hs(freq, T, P, X, P_dict, other_dict, data=data_hs, **kwargs)
+        return nh3_hs(freq, T, P, X, P_dict, other_dict, data=data_hs, low_set=low_set, **kwargs)
     if P > PHigher:
         return nh3_sjs(freq, T, P, X, P_dict, other_dict, data=data_sjs, **kwargs)
     a2 = nh3_sjs(freq, T, P, X, P_dict, other_dict, data=data_sjs, **kwargs)
-    a1 = nh3_hs(freq, T, P, X, P_dict, other_dict, data=data_hs, **kwargs)
+    a1 = nh3_hs(freq, T, P, X, P_dict, other_dict, data=data_hs, low_set=low_set, **kwargs)
     W = (P - PLower) / (PHigher - PLower)
     return W * a2 + (1.0 - W) * a1
+
+
+def nh3_dbs_sjs(freq, T, P, X, P_dict, other_dict, data_hs=None, data_sjs=None, **kwargs):
+    """nh3_dbs_sjs.alpha (constituents/nh3/nh3_dbs_sjs.py:6-26)."""
+    return nh3_hs_sjs(freq, T, P, X, P_dict, other_dict, data_hs=data_hs, data_sjs=data_sjs,
+                      low_set=DBS_INV_LO, **kwargs)

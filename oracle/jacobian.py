@@ -120,10 +120,11 @@ def _gross(f, fo, gamma):
     return (4.0 / math.pi) * ff ** 2 * gamma / ((fo ** 2 - ff ** 2) ** 2 + 4.0 * ff ** 2 * gamma ** 2)
 
 
-def _nh3_hs_part(f, T, P, mix, d):
-    """nh3_hs.__alpha__ (constituents/nh3/nh3_hs.py:117-340) for one side of 30 GHz."""
+def _nh3_hs_part(f, T, P, mix, d, low_set=A.HS_INV_LO):
+    """nh3_hs.__alpha__ (constituents/nh3/nh3_hs.py:117-340) for one side of 30 GHz
+    (nh3_dbs.__alpha__, nh3_dbs.py:92-313, with low_set=DBS_INV_LO)."""
     P_h2, P_he, P_nh3 = _pp(P, mix, 'H2'), _pp(P, mix, 'HE'), _pp(P, mix, 'NH3')
-    cs = A.HS_INV_LO if bool(torch.all(f <= A.HS_F_SPLIT)) else A.HS_INV_HI
+    cs = low_set if bool(torch.all(f <= A.HS_F_SPLIT)) else A.HS_INV_HI
     (gH2, gHe, gNH3, GH2, GHe, GNH3, zH2, zHe, zNH3, ZH2, ZHe, ZNH3, dd, Con) = cs
     Tdiv = (300.0 / T)[:, None]
     T295 = (295.0 / T)[:, None]
@@ -156,7 +157,7 @@ def _nh3_hs_part(f, T, P, mix, d):
     return a_inv + a_rot + a_v2
 
 
-def nh3_hs(f, T, P, mix, data):
+def nh3_hs(f, T, P, mix, data, low_set=A.HS_INV_LO):
     """nh3_hs.alpha (nh3_hs.py:96-114, 333-338): low half then high half, each clamped."""
     if bool(torch.any(P >= 400.0)):
         raise ValueError('oracle.jacobian restates nh3_hs_sjs for P < 400 bar only')
@@ -166,20 +167,22 @@ def nh3_hs(f, T, P, mix, data):
     parts = []
     for sel in (f <= A.HS_F_SPLIT, f > A.HS_F_SPLIT):
         if bool(sel.any()):
-            a = _nh3_hs_part(f[sel], T, P, mix, d)
+            a = _nh3_hs_part(f[sel], T, P, mix, d, low_set)
             parts.append(torch.where(a < 0.0, torch.full_like(a, 1.0E-8), a))
     return torch.cat(parts, dim=1)
 
 
-def alpha_layers(freqs, T, P, mix, lines, h2state='e'):
+def alpha_layers(freqs, T, P, mix, lines, h2state='e', nh3='nh3_hs_sjs'):
     """Alpha.get_layers of the Jupiter absorber set, layer-major (nl, nf) cm^-1
-    (alpha.py:151-305; sorted constituent order h2, h2o, h2s, nh3, ph3).
+    (alpha.py:151-305; sorted constituent order h2, h2o, h2s, nh3, ph3); nh3='nh3_dbs_sjs'
+    gives the Saturn / Uranus / Neptune set.
     freqs must be ascending (nh3_hs returns its <= 30 GHz half first)."""
+    low = {'nh3_hs_sjs': A.HS_INV_LO, 'nh3_dbs_sjs': A.DBS_INV_LO}[nh3]
     f = _t(freqs)
     if bool(torch.any(f[1:] < f[:-1])):
         raise ValueError('freqs must be ascending')
     return (h2_jj_ddb(f, T, P, mix, h2state) + h2o_bk(f, T, P, mix, lines['h2o'])
-            + h2s_ddb(f, T, P, mix, lines['h2s']) + nh3_hs(f, T, P, mix, lines['nh3_hs'])
+            + h2s_ddb(f, T, P, mix, lines['h2s']) + nh3_hs(f, T, P, mix, lines['nh3_hs'], low)
             + ph3_jh(f, T, P, mix, lines['ph3']))
 
 

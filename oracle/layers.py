@@ -72,19 +72,21 @@ class AbsorberSet:
         return np.array(cols).transpose()
 
 
-def jupiter_absorbers(lines, h2state='e'):
-    """Absorber set of the Jupiter config (default_config.json 'alpha', config.par:23).
+def jupiter_absorbers(lines, h2state='e', nh3='nh3_hs_sjs'):
+    """Absorber set of the Jupiter config (default_config.json 'alpha', config.par:23);
+    nh3='nh3_dbs_sjs' gives the Saturn / Uranus / Neptune set (their config.par:23).
 
     ``lines`` maps 'nh3_hs', 'nh3_sjs', 'h2s', 'ph3', 'h2o' to already-truncated
     line dicts (h2s / ph3 at the Jupiter default I0 > 1e-22 cut).
     """
+    nh3_fn = {'nh3_hs_sjs': absorbers.nh3_hs_sjs, 'nh3_dbs_sjs': absorbers.nh3_dbs_sjs}[nh3]
     def bind(fn, **data):
         return lambda f, T, P, X, D, o, **kw: fn(f, T, P, X, D, o, **data, **kw)
     funcs = {
         'h2': absorbers.h2_jj_ddb,
         'h2o': bind(absorbers.h2o_bk, data=lines['h2o']),
         'h2s': bind(absorbers.h2s_ddb, data=lines['h2s']),
-        'nh3': bind(absorbers.nh3_hs_sjs, data_hs=lines['nh3_hs'], data_sjs=lines['nh3_sjs']),
+        'nh3': bind(nh3_fn, data_hs=lines['nh3_hs'], data_sjs=lines['nh3_sjs']),
         'ph3': bind(absorbers.ph3_jh, data=lines['ph3']),
     }
     return AbsorberSet(funcs, other_dict={'h2': {'h2state': h2state}})

@@ -7,13 +7,15 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'librbx.so')
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 FP64 = 0
 FP32 = 1
 PRECISIONS = {'fp64': FP64, 'fp32': FP32}
 
-LINES_NH3_INV, LINES_NH3_ROT, LINES_NH3_V2, LINES_NH3_SJS, LINES_H2S, LINES_PH3, LINES_H2O = range(7)
+(LINES_NH3_INV, LINES_NH3_ROT, LINES_NH3_V2, LINES_NH3_SJS, LINES_H2S, LINES_PH3, LINES_H2O, LINES_CO,
+ TABLE_H2_ORTON) = range(9)
+ORTON_NTEMP = 10
 LINE_COLUMNS = {
     LINES_NH3_INV: ('fo', 'Io', 'Eo', 'gammaNH3o'),
     LINES_NH3_ROT: ('fo_rot', 'Io_rot', 'Eo_rot', 'gNH3_rot', 'gH2_rot', 'gHe_rot'),
@@ -22,9 +24,14 @@ LINE_COLUMNS = {
     LINES_H2S: ('f0', 'I0', 'E', 'GH2S'),
     LINES_PH3: ('f0', 'I0', 'E', 'WgtI0', 'WgtFGB', 'WgtSB'),
     LINES_H2O: ('f_o', 'I_o', 'E_o', 'w_s', 'x_s', 'w_h2', 'x_h2', 'w_he', 'x_he', 'SR'),
+    LINES_CO: ('f0', 'I0', 'E'),
+    TABLE_H2_ORTON: ('ftab_ghz', 'logv', 'Ttab'),      # lengths n, 6 n ORTON_NTEMP, ORTON_NTEMP
 }
-CONSTITUENTS = ('h2', 'h2o', 'h2s', 'nh3', 'ph3')          # RBX_C_* order (sorted names)
-SPECIES = ('H2', 'HE', 'CH4', 'NH3', 'H2O', 'H2S', 'PH3')  # RBX_X_* rows of mix
+CONSTITUENTS = ('h2', 'h2o', 'h2s', 'nh3', 'ph3', 'co', 'clouds')   # RBX_C_* rows
+SPECIES = ('H2', 'HE', 'CH4', 'NH3', 'H2O', 'H2S', 'PH3', 'CO')      # RBX_X_* rows of mix
+CLOUDS = ('H2O', 'SOLN', 'NH4SH', 'NH3', 'H2S', 'CH4')               # RBX_CL_* rows of the cloud table
+CLOUD_FLAGS = ('ice_p', 'water_p', 'nh4sh_p', 'nh3ice_p', 'h2sice_p', 'ch4')   # clouds_idp other_dict keys
+CO_SHAPES = {'voigt': 0, 'vvw': 1, 'diff': 2}                         # any other coshape: 3
 
 TUNE_ALPHA_K = 0
 TUNE_INT_UNROLL = 1
@@ -51,7 +58,8 @@ class RbxError(RuntimeError):
 
 class Absorbers(ctypes.Structure):
     _fields_ = [('h2', ctypes.c_int), ('h2o', ctypes.c_int), ('h2s', ctypes.c_int),
-                ('nh3', ctypes.c_int), ('ph3', ctypes.c_int)]
+                ('nh3', ctypes.c_int), ('ph3', ctypes.c_int), ('co', ctypes.c_int), ('co_shape', ctypes.c_int),
+                ('clouds', ctypes.c_int), ('cloud_types', ctypes.c_int)]
 
 
 class PlanetDesc(ctypes.Structure):
@@ -72,7 +80,7 @@ _SIGS = {
     'rbx_catalog_set_lines': (_I, [_VP, _I, _I, ctypes.POINTER(ctypes.POINTER(_D))]),
     'rbx_catalog_lines': (_I, [_VP, _I, ctypes.POINTER(_I)]),
     'rbx_catalog_destroy': (_I, [_VP]),
-    'rbx_alpha': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
+    'rbx_alpha': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
                        _VP, _I, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     'rbx_alpha_workspace_size': (_I, [_VP, ctypes.POINTER(Absorbers), _I, _I, _I, _I,
                                       ctypes.POINTER(ctypes.c_size_t)]),

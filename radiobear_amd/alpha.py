@@ -48,6 +48,12 @@ class Alpha:
         for c, absorber in self.config.constituent_alpha.items():
             if absorber is None:
                 continue
+            if c == 'cloud':
+                # the reference looks for the module under constituents/cloud, which does not
+                # exist, and skips it with this warning (alpha.py:63-71); 'clouds' is the key
+                s = "WARNING:  CAN'T LOAD " + absorber + '.  '
+                print(s * 3)
+                continue
             kernels.absorbers_struct({c: absorber}, self.config.h2state if c == 'h2' else 'e')
             self.constituent[c] = absorber
             self.truncate_freq[c] = None
@@ -57,10 +63,19 @@ class Alpha:
                 for tm in meth.split(','):
                     getattr(self, tm)[c] = getattr(self.config, tm)[c]
         self.ordered_constituents = sorted(self.constituent)
-        self.other_dict = {c: {} for c in self.ordered_constituents}
-        if 'h2' in self.other_dict:
-            self.other_dict['h2'] = {'h2state': self.config.h2state, 'h2newset': self.config.h2newset}
-        self.absorbers = kernels.absorbers_struct(self.constituent, self.config.h2state)
+        # config values copied into each module's other_dict (alpha.py:93-102)
+        other_to_copy = {'h2': ['h2state', 'h2newset'],
+                         'clouds': ['water_p', 'ice_p', 'nh4sh_p', 'nh3ice_p', 'h2sice_p', 'ch4_p'],
+                         'co': ['coshape']}
+        self.other_dict = {}
+        for c in self.ordered_constituents:
+            self.other_dict[c] = {}
+            for oc in other_to_copy.get(c, []):
+                if hasattr(self.config, oc):
+                    self.other_dict[c][oc] = getattr(self.config, oc)
+                else:
+                    print("The config for {} has no attribute {}".format(c, oc))
+        self.absorbers = kernels.absorbers_struct(self.constituent, self.config.h2state, self.other_dict)
         self.catalog = None
 
     def set_lines(self, nh3_inversion=None, h2o=None):
@@ -74,6 +89,7 @@ class Alpha:
     def _catalog(self):
         if self.catalog is None:
             self.catalog = kernels.jupiter_catalog(lines.raw_catalog(), self.truncate_strength, self.truncate_freq,
+                                                   co=self.absorbers.co != 0, orton=self.absorbers.h2 >= 3,
                                                    **self.line_overrides)
         return self.catalog
 
@@ -114,7 +130,7 @@ class Alpha:
         return layer_scale
 
     def scale_table(self, scale, N):
-        """(5, nl) per-constituent scale factors in RBX_C_* order for rbx_alpha."""
+        """(7, nl) per-constituent scale factors in RBX_C_* order for rbx_alpha."""
         ls = self.get_layer_scale(scale, N)
         out = np.ones((len(_lib.CONSTITUENTS), N))
         for lyr, s in enumerate(ls):
@@ -167,8 +183,11 @@ class Alpha:
                 self.save_alpha_data(save_alpha)
             return
         P, T, mix = atm.gas[C['P']], atm.gas[C['T']], kernels.mix_table(atm.gas, C)
+        cloud = None
+        if self.absorbers.clouds:
+            cloud = kernels.cloud_table(atm.cloud, atm.config.Cl)
         res = kernels.alpha(self._catalog(), self.absorbers, P, T, mix, self.freqs, self.precision,
-                            scale=sc, by_constituent=save_memfil)
+                            scale=sc, by_constituent=save_memfil, cloud=cloud)
         if save_memfil:
             self.table, byc = res
             cols = [_lib.CONSTITUENTS.index(c) for c in self.ordered_constituents]
@@ -199,3 +218,5 @@ class _OneLayer:
     def __init__(self, atm, layer):
         self.config = atm.config
         self.gas = np.ascontiguousarray(atm.gas[:, layer:layer + 1])
+        cloud = getattr(atm, 'cloud', None)
+        self.cloud = None if cloud is None else np.ascontiguousarray(np.asarray(cloud)[:, layer:layer + 1])

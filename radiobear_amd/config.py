@@ -52,6 +52,8 @@ TOKENS = {
     'output_directory': ('output_directory', None), 'log_directory': ('log_directory', None),
     'normalize_weighting': ('normalize_weighting', None), 'atm_run_type': ('atm_run_type', None),
     'output_type': ('output_type', None), 'h2newset': ('h2newset', None), 'zonal': ('zonal', None),
+    'water': ('water_p', 'um'), 'ice': ('ice_p', 'um'), 'h2sice': ('h2sice_p', 'um'), 'nh3ice': ('nh3ice_p', 'um'),
+    'nh4sh': ('nh4sh_p', 'um'), 'ch4': ('ch4_p', 'um'), 'coshape': ('coshape', None),
 }
 
 
@@ -116,6 +118,11 @@ class planetConfig:
         self.limb = 'shape'
         self.h2state = 'e'
         self.h2newset = True
+        # cloud particle sizes (um; clouds_idp only tests them > 0) and the CO line shape
+        # (default_config.json 'water' .. 'ch4', 'coshape')
+        for attr in ('water_p', 'ice_p', 'h2sice_p', 'nh3ice_p', 'nh4sh_p', 'ch4_p'):
+            setattr(self, attr, utils.convert_unit(1.0E-4, 'um'))
+        self.coshape = 'voigt'
         self.Doppler = False
         self.regridType = 'grid.dat'
         self.tweakmodule = p['tweakmodule']

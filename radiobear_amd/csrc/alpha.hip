@@ -22,12 +22,14 @@
 //   below 400 bar (nh3_hs_sjs.py:14-25).
 //
 // Forward path: the line-sum kernel's grid is (layer x group of four 64K-column
-// wavefront tiles, part of the line list); a tile never straddles the 30 GHz
-// switch of nh3_hs.  Each 256-thread block stages 256 lines at a time in LDS:
+// wavefront tiles, part of the line list); a tile may straddle the 30 GHz
+// switch of nh3_hs (its columns then read two coefficient sets).  Each 256-thread
+// block stages 256 lines at a time in LDS:
 // thread t derives line t's canonical coefficients for this layer (FP64, with
 // exp/pow), then every thread sums all staged lines for its K frequencies,
 // reading the coefficients as LDS broadcasts.  A finish kernel adds the parts
-// and the per-layer closed forms.
+// and the per-layer closed forms.  The optional absorbers co_ddb, clouds_idp and
+// h2_orton (extra.hpp) run in a small kernel of their own before the finish.
 //
 // FP64 mode sums in FP64 with IEEE division.  FP32 mode: the detuning x - D
 // is formed in FP64 (cancellation near line centre) and rounded, the rest of
@@ -61,14 +63,29 @@ constexpr double kHck = 1.438396;                          // h2s_ddb.py:8
 constexpr double kGeisa = 7.244E+21;                       // h2s_ddb.py:6
 constexpr double kGHz = 29.9792458;                        // h2s_ddb.py:9
 constexpr double kDbPerInvCm = 434294.5;
+}  // namespace rbx
 
-// nh3_hs inversion coefficient sets, <= 30 GHz and > 30 GHz (nh3_hs.py:165-194):
+#include "extra.hpp"
+
+namespace rbx {
+
+// inversion-line coefficient sets: 0 nh3_hs <= 30 GHz, 1 nh3_hs and nh3_dbs > 30 GHz
+// (nh3_hs.py:165-194), 2 nh3_dbs <= 30 GHz (the Bellotti/Steffes set, nh3_dbs.py:135-150):
 // gnu_H2, gnu_He, gnu_NH3, GAMMA_H2, GAMMA_He, GAMMA_NH3,
 // zeta_H2, zeta_He, zeta_NH3, Z_H2, Z_He, Z_NH3, d, Con
-__constant__ double kHsInv[2][14] = {
+__constant__ double kHsInv[3][14] = {
     {1.640, 0.75, 0.852, 0.7756, 0.666, 1.0, 1.262, 0.3, 0.5296, 0.7964, 0.667, 1.554, -0.0498, 0.9301},
     {1.7465, 0.9779, 0.7298, 0.8202, 1.0, 1.0, 1.2163, 0.0291, 0.5152, 0.8873, 0.8994, 2.0 / 3.0,
-     -0.0627, 0.9862}};
+     -0.0627, 0.9862},
+    {1.6937, 0.6997, 0.7523, 0.8085, 1.0, 1.0, 1.3263, 0.1607, 0.6162, 0.8199, 0.0, 1.3832, -0.0139, 0.9619}};
+
+// rbx_absorbers.nh3: 1 nh3_hs_sjs, 2 nh3_hs, 3 nh3_sjs, 4 nh3_dbs_sjs, 5 nh3_dbs.  nh3_dbs is
+// nh3_hs with coefficient set 2 below 30 GHz (nh3_dbs.py:92-313); nh3_dbs_sjs blends it with
+// nh3_sjs exactly as nh3_hs_sjs does (nh3_dbs_sjs.py:6-26).
+__host__ __device__ inline bool nh3_has_hs(int sel) { return sel == 1 || sel == 2 || sel == 4 || sel == 5; }
+__host__ __device__ inline bool nh3_has_sjs(int sel) { return sel == 1 || sel == 3 || sel == 4; }
+__host__ __device__ inline bool nh3_blended(int sel) { return sel == 1 || sel == 4; }
+__host__ __device__ inline int nh3_low_set(int sel) { return sel >= 4 ? 2 : 0; }
 
 // one layer: T and partial pressures (bar); S = double or Dual
 template <class S>
@@ -86,6 +103,7 @@ struct AlphaArgs {
     DevCatalog cat;
     rbx_absorbers abs;
     const double *P, *T, *mix;
+    const double *cloud;   // [RBX_NUM_CLOUDS][nl] or null (clouds_idp)
     int nl;
     const double *freq, *freq_nh3;
     int nf, nf_low;
@@ -97,6 +115,7 @@ struct AlphaArgs {
     double *value;     // tangent: alpha [nl][nf] (float64) or null
     double *ws;        // forward: partial line sums [kSlots][nsplit][nl][nf]
     double *ws_base;   // forward: partial h2o base terms [nsplit][nl]
+    double *ws_extra;  // forward: optional absorbers [kNumExtra][nl][nf] (when any is selected)
 };
 
 // ---- per-layer coefficient sets: each derives (D, a, c, g, G) of one line ----
@@ -104,8 +123,8 @@ template <class S>
 struct InvPrep {   // nh3_hs inversion lines (nh3_hs.py:198-255)
     S Ag, Bg, Az, Bz, eT, pref;
     double d;
-    __device__ InvPrep(const LayerT<S> &L, int hi) {
-        const double *k = kHsInv[hi];
+    __device__ InvPrep(const LayerT<S> &L, int set) {
+        const double *k = kHsInv[set];
         S Tdiv = 300.0 / L.T, t295 = 295.0 / L.T;
         Ag = (k[0] * L.h2) * pow(Tdiv, k[3]) + (k[1] * L.he) * pow(Tdiv, k[4]);
         Bg = k[2] * L.nh3 * pow(t295, k[5]);
@@ -586,11 +605,10 @@ __device__ inline LayerT<double> layer_of(const AlphaArgs &A, int layer) {
     return L;
 }
 
-// nh3_hs_sjs pressure blend (nh3_hs_sjs.py:6-26); sel 1 blend, 2 hs only, 3 sjs only
+// nh3_hs_sjs / nh3_dbs_sjs pressure blend (nh3_hs_sjs.py:6-26, nh3_dbs_sjs.py:6-26)
 template <class S>
 __device__ inline S nh3_select(int sel, double P, const S &hs, const S &sj) {
-    if (sel == 2) return hs;
-    if (sel == 3) return sj;
+    if (!nh3_blended(sel)) return nh3_has_hs(sel) ? hs : sj;
     if (P < 400.0) return hs;
     if (P > 2000.0) return sj;
     const double W = (P - 400.0) / (2000.0 - 400.0);
@@ -614,9 +632,9 @@ enum { kSlotH2o = 0, kSlotH2s = 1, kSlotNh3 = 2, kSlotPh3 = 3 };
 
 __device__ __host__ inline int rounds_of(int n) { return (n + kThreads - 1) / kThreads; }
 
-// nh3_hs_sjs (nh3_hs_sjs.py:6-26): which parts a layer needs; sel 1 blend, 2 hs, 3 sjs
-__device__ inline bool nh3_uses_hs(int sel, double P) { return sel == 2 || (sel == 1 && P <= 2000.0); }
-__device__ inline bool nh3_uses_sjs(int sel, double P) { return sel == 3 || (sel == 1 && P >= 400.0); }
+// which NH3 parts a layer needs (nh3_hs_sjs.py:14-25)
+__device__ inline bool nh3_uses_hs(int sel, double P) { return nh3_has_hs(sel) && (!nh3_blended(sel) || P <= 2000.0); }
+__device__ inline bool nh3_uses_sjs(int sel, double P) { return nh3_has_sjs(sel) && (!nh3_blended(sel) || P >= 400.0); }
 
 // A layer's staging rounds per segment (h2o, h2s, nh3 inversion, rotational, v2,
 // ph3), the slot each segment sums into, and the round range of each part.
@@ -735,7 +753,7 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
     }
     store(kSlotH2s);
     if (next(blk[RBX_LINES_NH3_INV].n)) {
-        const InvPrep<double> prep[2] = {InvPrep<double>(L, 0), InvPrep<double>(L, 1)};
+        const InvPrep<double> prep[2] = {InvPrep<double>(L, nh3_low_set(A.abs.nh3)), InvPrep<double>(L, 1)};
         const double s[2] = {F32 ? pow2_scale(prep[0].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0,
                              F32 ? pow2_scale(prep[1].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0};
         double a2[K] = {};
@@ -775,6 +793,31 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         A.ws_base[static_cast<size_t>(part) * A.nl + layer] = base;
 }
 
+// optional absorbers, one thread per (layer, column), into ws_extra (extra.hpp)
+enum { kExtraClouds = 0, kExtraCo = 1, kExtraH2 = 2, kNumExtra = 3 };
+
+__host__ __device__ inline bool has_extra(const rbx_absorbers &ab) { return ab.clouds || ab.co || ab.h2 >= 3; }
+
+__global__ __launch_bounds__(kThreads) void alpha_extra_kernel(AlphaArgs A) {
+    const int bpl = (A.nf + kThreads - 1) / kThreads;
+    const int layer = blockIdx.x / bpl;
+    const int col = (blockIdx.x % bpl) * kThreads + threadIdx.x;
+    if (col >= A.nf) return;
+    const int nl = A.nl;
+    const double P = A.P[layer], T = A.T[layer], f = A.freq[col];
+    const double ph2 = P * A.mix[RBX_X_H2 * nl + layer];
+    const double phe = P * A.mix[RBX_X_HE * nl + layer];
+    const size_t plane = static_cast<size_t>(nl) * A.nf;
+    double *out = A.ws_extra + static_cast<size_t>(layer) * A.nf + col;
+    if (A.abs.clouds) out[kExtraClouds * plane] = clouds_idp(A.cloud, nl, layer, T, f, A.abs.cloud_types);
+    if (A.abs.co)
+        out[kExtraCo * plane] = co_ddb(A.cat.blk[RBX_LINES_CO], P, T, ph2, phe, P * A.mix[RBX_X_CO * nl + layer], f,
+                                       A.abs.co_shape);
+    if (A.abs.h2 >= 3)
+        out[kExtraH2 * plane] = h2_orton(A.cat.blk[RBX_TABLE_H2_ORTON], T, ph2, phe,
+                                         P * A.mix[RBX_X_CH4 * nl + layer], f, A.abs.h2 - 3);
+}
+
 template <bool F32>
 __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int nsplit) {
     __shared__ SharedFwd sh;
@@ -805,7 +848,11 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
         tot += v * sc;
         if (A.by_c && ok) A.by_c[(static_cast<size_t>(c) * nl + layer) * A.nf + col] = v;
     };
-    if (A.abs.h2) emit(RBX_C_H2, h2_cia(L, A.abs.h2) * x);
+    // the total sums the constituents in name order: clouds, co, h2, h2o, h2s, nh3, ph3
+    const double *extra = A.ws_extra + o;
+    if (A.abs.clouds) emit(RBX_C_CLOUDS, extra[kExtraClouds * plane]);
+    if (A.abs.co) emit(RBX_C_CO, extra[kExtraCo * plane]);
+    if (A.abs.h2) emit(RBX_C_H2, A.abs.h2 <= 2 ? h2_cia(L, A.abs.h2) * x : extra[kExtraH2 * plane]);
     if (A.abs.h2o) {
         double base = 0.0;
         for (int p = 0; p < nsplit; ++p)
@@ -1053,7 +1100,7 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         Dual hs, sj[1];
         if (nh3_uses_hs(sel, L0.P)) {
             {
-                const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, 0), InvPrep<Dual>(L, 1)};
+                const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, nh3_low_set(sel)), InvPrep<Dual>(L, 1)};
                 lines(A.cat.blk[RBX_LINES_NH3_INV], prep, 2, 1, xn, hs, unused);
             }
             {
@@ -1100,10 +1147,15 @@ __global__ __launch_bounds__(kThreads) void alpha_tangent_kernel_f64(AlphaArgs A
 
 // check that the catalogue holds what the selected absorbers read
 static int check_catalog(const rbx_catalog *cat, const rbx_absorbers &ab, const char *who) {
-    RBX_CHECK_ARG(ab.h2 >= 0 && ab.h2 <= 2, "%s: bad h2 selector %d", who, ab.h2);
+    RBX_CHECK_ARG(ab.h2 >= 0 && ab.h2 <= 4, "%s: bad h2 selector %d", who, ab.h2);
+    RBX_CHECK_ARG(ab.co >= 0 && ab.co <= 1, "%s: bad co selector %d", who, ab.co);
+    RBX_CHECK_ARG(ab.co_shape >= 0 && ab.co_shape <= 3, "%s: bad co_shape %d", who, ab.co_shape);
+    RBX_CHECK_ARG(ab.clouds >= 0 && ab.clouds <= 1, "%s: bad clouds selector %d", who, ab.clouds);
+    RBX_CHECK_ARG(ab.cloud_types >= 0 && ab.cloud_types < (1 << RBX_NUM_CLOUDS), "%s: bad cloud_types %d", who,
+                  ab.cloud_types);
     RBX_CHECK_ARG(ab.h2o >= 0 && ab.h2o <= 1, "%s: bad h2o selector %d", who, ab.h2o);
     RBX_CHECK_ARG(ab.h2s >= 0 && ab.h2s <= 1, "%s: bad h2s selector %d", who, ab.h2s);
-    RBX_CHECK_ARG(ab.nh3 >= 0 && ab.nh3 <= 3, "%s: bad nh3 selector %d", who, ab.nh3);
+    RBX_CHECK_ARG(ab.nh3 >= 0 && ab.nh3 <= 5, "%s: bad nh3 selector %d", who, ab.nh3);
     RBX_CHECK_ARG(ab.ph3 >= 0 && ab.ph3 <= 1, "%s: bad ph3 selector %d", who, ab.ph3);
     struct Need {
         bool on;
@@ -1112,8 +1164,10 @@ static int check_catalog(const rbx_catalog *cat, const rbx_absorbers &ab, const 
     } needs[] = {{ab.h2o != 0, RBX_LINES_H2O, "h2o_bk"},
                  {ab.h2s != 0, RBX_LINES_H2S, "h2s_ddb"},
                  {ab.ph3 != 0, RBX_LINES_PH3, "ph3_jh"},
-                 {ab.nh3 == 1 || ab.nh3 == 2, RBX_LINES_NH3_INV, "nh3_hs"},
-                 {ab.nh3 == 1 || ab.nh3 == 3, RBX_LINES_NH3_SJS, "nh3_sjs"}};
+                 {nh3_has_hs(ab.nh3), RBX_LINES_NH3_INV, "nh3_hs / nh3_dbs"},
+                 {nh3_has_sjs(ab.nh3), RBX_LINES_NH3_SJS, "nh3_sjs"},
+                 {ab.co != 0, RBX_LINES_CO, "co_ddb"},
+                 {ab.h2 >= 3, RBX_TABLE_H2_ORTON, "h2_orton"}};
     for (const Need &n : needs) {
         if (n.on && cat->dev.blk[n.kind].n <= 0) {
             set_error("%s: absorber %s selected but its line block %d is empty", who, n.what, n.kind);
@@ -1124,10 +1178,12 @@ static int check_catalog(const rbx_catalog *cat, const rbx_absorbers &ab, const 
 }
 
 static int fill_args(AlphaArgs &A, const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,
-                     const double *T, const double *mix, int nl, const double *freq, const double *freq_nh3, int nf,
-                     int nf_nh3_low, const double *scale, void *out, double *by_c, const char *who) {
+                     const double *T, const double *mix, const double *cloud, int nl, const double *freq,
+                     const double *freq_nh3, int nf, int nf_nh3_low, const double *scale, void *out, double *by_c,
+                     const char *who) {
     RBX_CHECK_ARG(cat && absorbers, "%s: null catalogue or absorber selection", who);
     RBX_CHECK_ARG(P && T && mix && freq && out, "%s: null device pointer", who);
+    RBX_CHECK_ARG(!absorbers->clouds || cloud, "%s: clouds_idp selected but the cloud table is null", who);
     RBX_CHECK_ARG(nl > 0 && nf > 0, "%s: nl=%d nf=%d must be positive", who, nl, nf);
     RBX_CHECK_ARG(static_cast<long long>(nl) * ((nf + 63) / 64) < (1LL << 31), "%s: nl=%d x nf=%d too large", who,
                   nl, nf);
@@ -1139,6 +1195,7 @@ static int fill_args(AlphaArgs &A, const rbx_catalog *cat, const rbx_absorbers *
     A.P = P;
     A.T = T;
     A.mix = mix;
+    A.cloud = cloud;
     A.nl = nl;
     A.freq = freq;
     A.freq_nh3 = freq_nh3;
@@ -1194,7 +1251,7 @@ static int resident_blocks(int K, int precision, int &out) {
 // the block's own set-up), i.e. the grid's tail.
 static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
     const DevBlock *blk = A.cat.blk;
-    const bool hs = A.abs.nh3 == 1 || A.abs.nh3 == 2;
+    const bool hs = nh3_has_hs(A.abs.nh3);
     const int R = (A.abs.h2o ? rounds_of(blk[RBX_LINES_H2O].n) : 0) +
                   (A.abs.h2s ? rounds_of(blk[RBX_LINES_H2S].n) : 0) +
                   (hs ? rounds_of(blk[RBX_LINES_NH3_INV].n) + rounds_of(blk[RBX_LINES_NH3_ROT].n) +
@@ -1226,7 +1283,9 @@ static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
 
 static size_t alpha_ws_bytes(const AlphaArgs &A, int nsplit) {
     const size_t plane = static_cast<size_t>(A.nl) * A.nf;
-    return (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl) * sizeof(double);
+    const size_t extra = has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0;
+    return (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl + extra) *
+           sizeof(double);
 }
 
 template <int K>
@@ -1240,6 +1299,11 @@ static int launch_alpha(AlphaArgs &A, int precision, int nsplit, hipStream_t st)
     int rc = launch_status("alpha_lines_kernel");
     if (rc) return rc;
     dim3 g2(static_cast<unsigned>((A.nf + kThreads - 1) / kThreads * static_cast<long long>(A.nl)));
+    if (has_extra(A.abs)) {
+        hipLaunchKernelGGL(alpha_extra_kernel, g2, dim3(kThreads), 0, st, A);
+        rc = launch_status("alpha_extra_kernel");
+        if (rc) return rc;
+    }
     if (precision == RBX_FP32)
         hipLaunchKernelGGL((alpha_finish_kernel<true>), g2, dim3(kThreads), 0, st, A, nsplit);
     else
@@ -1281,12 +1345,13 @@ extern "C" int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorb
 }
 
 extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P, const double *T,
-                         const double *mix, int nl, const double *freq, const double *freq_nh3, int nf,
+                         const double *mix, const double *cloud, int nl, const double *freq, const double *freq_nh3,
+                         int nf,
                          int nf_nh3_low, const double *scale, int precision, void *alpha, double *by_constituent,
                          void *workspace, size_t workspace_bytes, void *stream) {
     RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha: bad precision %d", precision);
     rbx::AlphaArgs A;
-    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
+    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, cloud, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
                             by_constituent, "rbx_alpha");
     if (rc) return rc;
     int K = 1, nsplit = 1;
@@ -1299,6 +1364,7 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
     RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha: workspace not 8-byte aligned");
     A.ws = static_cast<double *>(workspace);
     A.ws_base = A.ws + static_cast<size_t>(rbx::kSlots) * nsplit * nl * nf;
+    A.ws_extra = A.ws_base + static_cast<size_t>(nsplit) * nl;
     hipStream_t st = static_cast<hipStream_t>(stream);
     switch (K) {
         case 4: return rbx::launch_alpha<4>(A, precision, nsplit, st);
@@ -1313,7 +1379,9 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
                                  double *alpha, double *dalpha, void *stream) {
     RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_tangent: bad precision %d", precision);
     rbx::AlphaArgs A;
-    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nl, freq, freq_nh3, nf, nf_nh3_low, scale, dalpha,
+    RBX_CHECK_ARG(absorbers && !rbx::has_extra(*absorbers),
+                  "rbx_alpha_tangent: co_ddb, clouds_idp and h2_orton have no tangent; select them off");
+    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nullptr, nl, freq, freq_nh3, nf, nf_nh3_low, scale, dalpha,
                             nullptr, "rbx_alpha_tangent");
     if (rc) return rc;
     A.value = alpha;

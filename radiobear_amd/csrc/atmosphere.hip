@@ -9,10 +9,11 @@ namespace rbx {
 
 // chemistry.py:7-46: gas constant, molar masses (g/mol) and microwave
 // refractivity coefficients of the species in the rows of mix (RBX_X_* order);
-// PH3 has no refractivity there (NULL), H2O's depends on T.
+// PH3 has no refractivity there (NULL), H2O's depends on T, and CO has neither
+// (ConstituentProperties gives it the NULL entries, 0, chemistry.py:168-200).
 constexpr double kRgas = 8.314462;
-__constant__ double kAmu[RBX_NUM_SPECIES] = {2.016, 4.003, 16.04, 17.030, 18.015, 34.076, 33.997};
-__constant__ double kRefr[RBX_NUM_SPECIES] = {124.43, 35.832, 413.0, 2700.0, 0.0, 2247.0, 0.0};
+__constant__ double kAmu[RBX_NUM_SPECIES] = {2.016, 4.003, 16.04, 17.030, 18.015, 34.076, 33.997, 0.0};
+__constant__ double kRefr[RBX_NUM_SPECIES] = {124.43, 35.832, 413.0, 2700.0, 0.0, 2247.0, 0.0, 0.0};
 
 __device__ inline double refractivity(const double *mix, size_t nlt, size_t l, double T) {
     double r = 0.0;

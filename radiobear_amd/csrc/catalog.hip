@@ -104,12 +104,29 @@ extern "C" int rbx_catalog_set_lines(rbx_catalog *cat, int kind, int n, const do
     for (int i = 0; i < n; ++i)
         RBX_CHECK_ARG(host_columns[0][i] > 0.0, "rbx_catalog_set_lines: line %d has centre %g <= 0",
                       i, host_columns[0][i]);
+    if (kind == RBX_TABLE_H2_ORTON && n > 0) {
+        // the kernels bracket a frequency by table rows i-1, i, i+1 and a temperature by a
+        // table interval: both grids must be ascending
+        RBX_CHECK_ARG(n >= 3, "rbx_catalog_set_lines: the Orton table needs >= 3 frequencies, got %d", n);
+        for (int i = 1; i < n; ++i)
+            RBX_CHECK_ARG(host_columns[0][i] > host_columns[0][i - 1],
+                          "rbx_catalog_set_lines: Orton table frequencies not ascending at %d", i);
+        for (int i = 1; i < RBX_ORTON_NTEMP; ++i)
+            RBX_CHECK_ARG(host_columns[2][i] > host_columns[2][i - 1] && host_columns[2][0] > 0.0,
+                          "rbx_catalog_set_lines: Orton table temperatures not positive and ascending");
+    }
     free_block(cat, kind);
     if (n == 0) return RBX_OK;
-    const size_t bytes = sizeof(double) * static_cast<size_t>(n) * ncol;
-    std::vector<double> packed(static_cast<size_t>(n) * ncol);
-    for (int c = 0; c < ncol; ++c)
-        std::memcpy(packed.data() + static_cast<size_t>(c) * n, host_columns[c], sizeof(double) * n);
+    size_t total = 0;
+    for (int c = 0; c < ncol; ++c) total += rbx::column_length(kind, c, n);
+    const size_t bytes = sizeof(double) * total;
+    std::vector<double> packed(total);
+    size_t at = 0;
+    for (int c = 0; c < ncol; ++c) {
+        const size_t len = rbx::column_length(kind, c, n);
+        std::memcpy(packed.data() + at, host_columns[c], sizeof(double) * len);
+        at += len;
+    }
     double *d = nullptr;
     hipError_t e = hipMalloc(&d, bytes);
     if (e != hipSuccess) {
@@ -124,9 +141,13 @@ extern "C" int rbx_catalog_set_lines(rbx_catalog *cat, int kind, int n, const do
     }
     cat->storage[kind] = d;
     rbx::DevBlock &b = cat->dev.blk[kind];
-    for (int c = 0; c < ncol; ++c) b.col[c] = d + static_cast<size_t>(c) * n;
+    at = 0;
+    for (int c = 0; c < ncol; ++c) {
+        b.col[c] = d + at;
+        at += rbx::column_length(kind, c, n);
+    }
     b.n = n;
-    b.wscale = block_wscale(kind, n, host_columns);
+    b.wscale = kind == RBX_TABLE_H2_ORTON ? 0.0 : block_wscale(kind, n, host_columns);
     return RBX_OK;
 }
 

@@ -6,7 +6,15 @@
 namespace rbx {
 
 constexpr int kMaxCols = 10;
-constexpr int kNumCols[RBX_NUM_LINE_KINDS] = {4, 6, 3, 4, 4, 6, 10};
+constexpr int kNumCols[RBX_NUM_LINE_KINDS] = {4, 6, 3, 4, 4, 6, 10, 3, 3};
+
+// values in column c of a block of n lines (n for line lists; the Orton table's
+// columns are its frequencies, its [6 n][RBX_ORTON_NTEMP] values and its temperatures)
+inline size_t column_length(int kind, int c, int n) {
+    if (kind != RBX_TABLE_H2_ORTON) return static_cast<size_t>(n);
+    if (c == 1) return static_cast<size_t>(6) * n * RBX_ORTON_NTEMP;
+    return c == 2 ? static_cast<size_t>(RBX_ORTON_NTEMP) : static_cast<size_t>(n);
+}
 
 // One block of lines as the kernels see it: SoA float64 columns in HBM.
 struct DevBlock {

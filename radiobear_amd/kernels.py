@@ -14,7 +14,7 @@ from . import _lib
 from ._lib import check, lib
 
 F_SPLIT = 30.0          # nh3_hs switches coefficient sets above 30 GHz (nh3_hs.py:84)
-NH3_MODELS = {'nh3_hs_sjs': 1, 'nh3_hs': 2, 'nh3_sjs': 3}
+NH3_MODELS = {'nh3_hs_sjs': 1, 'nh3_hs': 2, 'nh3_sjs': 3, 'nh3_dbs_sjs': 4, 'nh3_dbs': 5}
 
 
 def device():
@@ -57,9 +57,15 @@ class Catalog:
     def set_lines(self, kind, data):
         """Upload block ``kind`` from a dict holding its columns (see _lib.LINE_COLUMNS)."""
         names = _lib.LINE_COLUMNS[kind]
-        cols = [np.ascontiguousarray(np.asarray(data[n], dtype=np.float64)) for n in names]
+        cols = [np.ascontiguousarray(np.asarray(data[n], dtype=np.float64).ravel()) for n in names]
         n = len(cols[0])
-        if any(len(c) != n for c in cols):
+        if kind == _lib.TABLE_H2_ORTON:
+            nt = _lib.ORTON_NTEMP
+            if len(cols[1]) != 6 * n * nt or len(cols[2]) != nt:
+                raise ValueError('Orton table: {} frequencies need {} values and {} temperatures'.format(
+                    n, 6 * n * nt, nt))
+            self.orton_ftab = cols[0].copy()
+        elif any(len(c) != n for c in cols):
             raise ValueError('line block {}: columns of unequal length'.format(kind))
         arr = (ctypes.POINTER(ctypes.c_double) * len(cols))(
             *[c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in cols])
@@ -83,18 +89,23 @@ class Catalog:
             pass
 
 
-def absorbers_struct(constituent_alpha, h2state='e'):
-    """rbx_absorbers from a reference-style {'nh3': 'nh3_hs_sjs', ...} mapping."""
+def absorbers_struct(constituent_alpha, h2state='e', other_dict=None):
+    """rbx_absorbers from a reference-style {'nh3': 'nh3_hs_sjs', ...} mapping.
+
+    other_dict: the per-constituent options Alpha.setup_formalisms copies from the
+    config (alpha.py:93-102): other_dict['co']['coshape'] (default 'voigt',
+    co_ddb.py:37-40) and the clouds_idp flags in other_dict['clouds']."""
+    other_dict = other_dict or {}
     ab = _lib.Absorbers()
     for c, model in constituent_alpha.items():
         if model is None:
             continue
         if c == 'h2':
-            if model != 'h2_jj_ddb':
+            if model not in ('h2_jj_ddb', 'h2_orton'):
                 raise ValueError('unsupported h2 absorber {}'.format(model))
             if h2state not in ('e', 'n'):
                 raise ValueError('INVALID H2STATE {!r}'.format(h2state))
-            ab.h2 = 1 if h2state == 'e' else 2
+            ab.h2 = (1 if h2state == 'e' else 2) + (2 if model == 'h2_orton' else 0)
         elif c == 'h2o':
             if model != 'h2o_bk':
                 raise ValueError('unsupported h2o absorber {}'.format(model))
@@ -111,9 +122,34 @@ def absorbers_struct(constituent_alpha, h2state='e'):
             if model != 'ph3_jh':
                 raise ValueError('unsupported ph3 absorber {}'.format(model))
             ab.ph3 = 1
+        elif c == 'co':
+            if model != 'co_ddb':
+                raise ValueError('unsupported co absorber {}'.format(model))
+            ab.co = 1
+            shape = (other_dict.get('co') or {}).get('coshape', 'voigt')
+            ab.co_shape = _lib.CO_SHAPES.get(shape, 3)
+        elif c == 'clouds':
+            if model != 'clouds_idp':
+                raise ValueError('unsupported clouds absorber {}'.format(model))
+            ab.clouds = 1
+            flags = other_dict.get('clouds') or {}
+            ab.cloud_types = sum(1 << i for i, k in enumerate(_lib.CLOUD_FLAGS)
+                                 if k in flags and flags[k] is not None and flags[k] > 0.0)
         else:
             raise ValueError('unsupported constituent {}'.format(c))
     return ab
+
+
+def _check_orton_freqs(catalog, freqs):
+    """h2_orton brackets each frequency by the first tabulated one above it and the next
+    one up (h2_orton.py:77-92): the reference fails (IndexError) at or beyond the
+    second-to-last table frequency, and so does this."""
+    ft = getattr(catalog, 'orton_ftab', None)
+    if ft is None:
+        raise ValueError('h2_orton selected but the catalogue has no Orton table')
+    f = np.asarray(freqs, dtype=np.float64)
+    if np.any(f >= ft[-2]):
+        raise ValueError('h2_orton: frequencies must be below {} GHz (the table range)'.format(ft[-2]))
 
 
 def nh3_order(freqs):
@@ -125,8 +161,20 @@ def nh3_order(freqs):
 
 
 def mix_table(gas, C):
-    """(7, nl) mixing ratios in RBX_X_* order from a reference gas table."""
-    return np.stack([np.asarray(gas[C[s]], dtype=np.float64) for s in _lib.SPECIES])
+    """(8, nl) mixing ratios in RBX_X_* order from a reference gas table (a species
+    the table has no row for, such as CO in some gas files, is zero)."""
+    nl = np.asarray(gas).shape[1]
+    return np.stack([np.asarray(gas[C[s]], dtype=np.float64) if s in C else np.zeros(nl)
+                     for s in _lib.SPECIES])
+
+
+def cloud_table(cloud, Cl):
+    """(6, nl) cloud densities (g/cm^3) in RBX_CL_* order from a reference cloud table
+    laid out by Cl (config.Cl); a cloud the table has no row for is zero."""
+    cloud = np.asarray(cloud, dtype=np.float64)
+    nl = cloud.shape[1]
+    return np.stack([cloud[Cl[k]] if k in Cl and Cl[k] < cloud.shape[0] else np.zeros(nl)
+                     for k in _lib.CLOUDS])
 
 
 def _alpha_inputs(P, T, mix, freqs, scale):
@@ -149,16 +197,26 @@ def _alpha_inputs(P, T, mix, freqs, scale):
     return P, T, mix, fd, fn, nf_low, sc, nl
 
 
-def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by_constituent=False):
-    """alpha[nl][nf] (cm^-1) on the device; optionally also [5][nl][nf] per constituent."""
+def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by_constituent=False, cloud=None):
+    """alpha[nl][nf] (cm^-1) on the device; optionally also [7][nl][nf] per constituent
+    (RBX_C_* rows).  cloud: (6, nl) densities (cloud_table), needed by clouds_idp."""
     prec = _lib.PRECISIONS[precision]
     P, T, mix, fd, fn, nf_low, sc, nl = _alpha_inputs(P, T, mix, freqs, scale)
     nf = fd.shape[0]
+    cl = None
+    if absorbers.clouds:
+        if cloud is None:
+            raise ValueError('clouds_idp needs the cloud densities (cloud=)')
+        cl = as_device(cloud)
+        if cl.shape != (len(_lib.CLOUDS), nl):
+            raise ValueError('cloud must be ({}, {})'.format(len(_lib.CLOUDS), nl))
+    if absorbers.h2 >= 3:
+        _check_orton_freqs(catalog, freqs)
     out = torch.empty((nl, nf), dtype=torch.float32 if prec == _lib.FP32 else torch.float64, device=device())
     byc = torch.zeros((len(_lib.CONSTITUENTS), nl, nf), dtype=torch.float64, device=device()) \
         if by_constituent else None
     ws, nbytes = alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec)
-    check(lib().rbx_alpha(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), nl, _ptr(fd),
+    check(lib().rbx_alpha(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), _ptr(cl), nl, _ptr(fd),
                           _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(out), _ptr(byc), _ptr(ws), nbytes, _stream()))
     return (out, byc) if by_constituent else out
 
@@ -292,12 +350,14 @@ def jacobian(alpha_table, dalpha, T, hds, nsteps, tau_cut=0.0):
 
 
 # ------------------------------------------------------------- Jupiter lines
-def jupiter_catalog(raw, truncate_strength=None, truncate_freq=None, nh3_inversion=None, h2o=None):
+def jupiter_catalog(raw, truncate_strength=None, truncate_freq=None, nh3_inversion=None, h2o=None, co=False,
+                    orton=False):
     """Catalogue with the Jupiter line lists and the config's truncation.
 
     raw: lines.raw_catalog(); truncate_strength/freq: per-constituent dicts as
     config.truncate_* (defaults: h2s and ph3 at I0 > 1e-22); nh3_inversion /
-    h2o: optional replacement lists (synthetic workloads).
+    h2o: optional replacement lists (synthetic workloads); co / orton: also load
+    the CO lines (co_ddb) / the Orton CIA table (h2_orton).
     """
     from . import lines as L
     ts = {'h2s': 1e-22, 'ph3': 1e-22} if truncate_strength is None else truncate_strength
@@ -316,6 +376,10 @@ def jupiter_catalog(raw, truncate_strength=None, truncate_freq=None, nh3_inversi
     cat.set_lines(_lib.LINES_PH3, L.truncate_ph3(raw['ph3jh'], raw['PH3WGT'], ts.get('ph3'), tf.get('ph3')))
     h2o_lines = L.h2o_bk_lines() if h2o is None else h2o
     cat.set_lines(_lib.LINES_H2O, L.truncate_h2o(h2o_lines, ts.get('h2o'), tf.get('h2o')))
+    if co:
+        cat.set_lines(_lib.LINES_CO, L.load('co'))      # co_ddb reads no truncation (co_ddb.py:14-19)
+    if orton:
+        cat.set_lines(_lib.TABLE_H2_ORTON, L.orton_columns(L.load_orton()))
     return cat
 
 

@@ -1,9 +1,9 @@
 """Line catalogues shipped with the package (radiobear_amd/data).
 
-The five .npz files are the reference's line lists, unchanged in content and
-keys (constituents/nh3/ammonia.npz, nh3/nh3.npz, h2s/h2s.npz, ph3/ph3jh.npz,
-ph3/PH3WGT.npz), so that a user's own catalogue in the reference schema drops
-in.  The 15 H2O lines of h2o_bk live in ``h2o_bk_lines`` (the reference types
+The .npz line lists are the reference's, unchanged in content and keys
+(constituents/nh3/ammonia.npz, nh3/nh3.npz, h2s/h2s.npz, ph3/ph3jh.npz,
+ph3/PH3WGT.npz, co/co.npz), so that a user's own catalogue in the reference
+schema drops in; orton_H2.npz holds h2_orton's CIA tables (see orton_to_npz.py).  The 15 H2O lines of h2o_bk live in ``h2o_bk_lines`` (the reference types
 them in constituents/h2o/h2o_bk.py:23-49).
 """
 import os
@@ -22,6 +22,32 @@ def load(name):
 
 def raw_catalog():
     return {name: load(name) for name in CATALOGUES}
+
+
+def load_orton():
+    """Orton's H2 collision-induced absorption tables (h2_orton, constituents/h2/h2_orton.py:17-97),
+    as orton_to_npz.py converts constituents/h2/orton_H2.tables: header [nTemp, Tmax, Tmin],
+    ftab_invcm [nfreq] and logv [6 nfreq][nTemp] natural-log values."""
+    return load('orton_H2')
+
+
+ORTON_GHZ = 29.9792458   # cm^-1 -> GHz, h2_orton.py:11
+
+
+def orton_columns(table):
+    """The catalogue columns of the Orton table block: frequencies in GHz and the
+    temperature grid computed as h2_orton.readInputFiles does (h2_orton.py:27-40,
+    log-spaced from Tmin by repeated addition), and the log values row by row."""
+    import math
+    ntemp, tmax, tmin = int(table['header'][0]), float(table['header'][1]), float(table['header'][2])
+    lTmx, lTmn = math.log(tmax), math.log(tmin)
+    dlT = (lTmx - lTmn) / (ntemp - 1.0)
+    ta = [lTmn]
+    for i in range(ntemp - 1):
+        ta.append(ta[i] + dlT)
+    return {'ftab_ghz': np.asarray(table['ftab_invcm'], dtype=np.float64) * ORTON_GHZ,
+            'logv': np.ascontiguousarray(table['logv'], dtype=np.float64).ravel(),
+            'Ttab': np.array([math.exp(t) for t in ta])}
 
 
 def truncate(data, keys, strength=None, freq=None):

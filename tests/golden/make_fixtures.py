@@ -59,12 +59,12 @@ def load_module(constituent, name):
     return sys.modules[name]
 
 
-def call(mod, freqs, T, P, X, other, path, **kw):
+def call(mod, freqs, T, P, X, other, path, P_dict=None, **kw):
     kw.setdefault('truncate_freq', None)           # alpha.py:210-213 always passes both
     kw.setdefault('truncate_strength', None)
     with contextlib.redirect_stdout(io.StringIO()):
-        return np.asarray(mod.alpha(list(freqs), T, P, X, C, other, units='invcm', path=path,
-                                    verbose=False, **kw), dtype=np.float64)
+        return np.asarray(mod.alpha(list(freqs), T, P, X, C if P_dict is None else P_dict, other,
+                                    units='invcm', path=path, verbose=False, **kw), dtype=np.float64)
 
 
 def fixture_absorbers():
@@ -95,6 +95,69 @@ def fixture_absorbers():
     out['alpha_nh3_hs_unsorted'] = np.array([call(nh3_hs, FREQS[perm], T, P, gas_column(T, P), {}, p)
                                              for T, P in TP[:4]])
     np.savez_compressed(os.path.join(HERE, 'absorbers.npz'), **out)
+
+
+# cloud densities (g/cm^3) in the Jupiter cloud-table layout (config.par 'clouds' line)
+CL = {'Z': 0, 'T': 1, 'P': 2, 'SOLN': 3, 'H2O': 4, 'NH4SH': 5, 'NH3': 6, 'H2S': 7, 'CH4': 8, 'AR': 9,
+      'PH3': 10, 'DZ': 11}
+CLOUD_DENSITY = {'SOLN': 5e-4, 'H2O': 1e-4, 'NH4SH': 2e-5, 'NH3': 3e-5, 'H2S': 1e-5, 'CH4': 1e-5}
+CLOUD_FLAGS = ('water_p', 'ice_p', 'nh4sh_p', 'nh3ice_p', 'h2sice_p', 'ch4')
+# (T K, P bar) for CO (Voigt / VVW blend between 1e-3 and 0.1 bar), clouds (both sides
+# of 273 K) and h2_orton (below, inside and above its 40-400 K table)
+TP_CO = np.array([[60.0, 1e-4], [90.0, 1e-3], [110.0, 0.005], [120.0, 0.03], [140.0, 0.1],
+                  [165.0, 1.0], [400.0, 30.0], [900.0, 350.0]])
+TP_CLOUD = np.array([[110.0, 0.5], [250.0, 5.0], [272.9, 6.0], [273.0, 6.0], [300.0, 8.0],
+                     [350.0, 12.0], [600.0, 90.0]])
+TP_ORTON = np.array([[30.0, 0.01], [40.0, 0.05], [55.0, 0.1], [110.0, 0.5], [165.0, 1.0],
+                     [250.0, 5.0], [399.0, 30.0], [400.0, 35.0], [615.0, 100.0], [1500.0, 1200.0]])
+
+
+def fixture_other_absorbers():
+    """F6: the absorbers of the other planets' configs and the optional ones: nh3_dbs and
+    nh3_dbs_sjs (Saturn / Uranus / Neptune), co_ddb (Neptune) in its four line shapes,
+    clouds_idp with each cloud type alone and all together, h2_orton in both h2 states."""
+    out = {'freqs': FREQS, 'TP': TP, 'TP_co': TP_CO, 'TP_cloud': TP_CLOUD, 'TP_orton': TP_ORTON}
+    from radiobear.constituents.nh3 import nh3_dbs
+    p = os.path.join(CONST, 'nh3')
+    out['alpha_nh3_dbs'] = np.array([call(nh3_dbs, FREQS, T, P, gas_column(T, P), {}, p) for T, P in TP])
+    mod = load_module('nh3', 'nh3_dbs_sjs')
+    out['alpha_nh3_dbs_sjs'] = np.array([call(mod, FREQS, T, P, gas_column(T, P), {}, p) for T, P in TP])
+    # co_ddb: CO at 1e-6
+    mod = load_module('co', 'co_ddb')
+    for shape in ('voigt', 'vvw', 'diff', 'std'):
+        rows = []
+        for T, P in TP_CO:
+            X = gas_column(T, P)
+            X[C['CO']] = 1.0e-6
+            rows.append(call(mod, FREQS, T, P, X, {'coshape': shape}, os.path.join(CONST, 'co')))
+        out['alpha_co_' + shape] = np.array(rows)
+    # clouds_idp: the cloud column, one flag at a time, then all of them
+    mod = load_module('clouds', 'clouds_idp')
+    cases = [(f,) for f in CLOUD_FLAGS] + [CLOUD_FLAGS]
+    for i, flags in enumerate(cases):
+        other = {f: 1.0e-4 for f in flags}
+        rows = []
+        for T, P in TP_CLOUD:
+            X = np.zeros(len(CL))
+            X[CL['T']], X[CL['P']] = T, P
+            for k, v in CLOUD_DENSITY.items():
+                X[CL[k]] = v
+            rows.append(call(mod, FREQS, T, P, X, other, os.path.join(CONST, 'clouds'), P_dict=CL))
+        out['alpha_clouds_{}'.format(i)] = np.array(rows)
+    out['cloud_cases'] = np.array(['+'.join(c) for c in cases])
+    out['cloud_density'] = np.array([CLOUD_DENSITY[k] for k in ('H2O', 'SOLN', 'NH4SH', 'NH3', 'H2S', 'CH4')])
+    # h2_orton: called directly (its alpha() takes no truncate_* keywords)
+    load_module('h2', 'h2_jj')
+    mod = load_module('h2', 'h2_orton')
+    for state in ('e', 'n'):
+        rows = []
+        for T, P in TP_ORTON:
+            with contextlib.redirect_stdout(io.StringIO()):
+                a = mod.alpha(list(FREQS), T, P, gas_column(T, P), C, {'h2state': state, 'h2newset': True},
+                              units='invcm', path=os.path.join(CONST, 'h2'))
+            rows.append(np.asarray(a, dtype=np.float64))
+        out['alpha_h2_orton_' + state] = np.array(rows)
+    np.savez_compressed(os.path.join(HERE, 'other_absorbers.npz'), **out)
 
 
 def fixture_synthetic_lines():
@@ -208,7 +271,11 @@ def fixture_benchmark():
 
 
 if __name__ == '__main__':
+    if sys.argv[1:] == ['other']:
+        fixture_other_absorbers()
+        sys.exit(0)
     fixture_absorbers()
+    fixture_other_absorbers()
     fixture_synthetic_lines()
     fixture_geometry()
     fixture_benchmark()

@@ -58,7 +58,7 @@ def test_alpha_benchmark_layers(K, jcat, bench_atm, jupiter_lines, precision, to
     got = out.double().cpu().numpy().T
     assert rel(got, ref) < tol
     # per-constituent tables match the reference absorb columns (sorted h2, h2o, h2s, nh3, ph3)
-    b = byc.cpu().numpy()            # (5, nl, nf)
+    b = byc.cpu().numpy()            # (7, nl, nf): RBX_C_* rows, the first five these
     for c in range(5):
         r = saved[:, :, c]           # (nl, nf)
         m = np.abs(r) > 1e-30
@@ -113,7 +113,8 @@ def test_alpha_scale_by_constituent(K, jcat, bench_atm, jupiter_lines):
     aset = layers.jupiter_absorbers(jupiter_lines)
     scale = {'nh3': list(np.linspace(0.5, 2.0, nl)), 'h2o': list(np.linspace(3.0, 1.0, nl))}
     ref = layers.get_layers(list(FREQS), g, C, aset, scale=scale)
-    sc = np.ones((5, nl))
+    from radiobear_amd import _lib
+    sc = np.ones((len(_lib.CONSTITUENTS), nl))
     sc[3] = scale['nh3']
     sc[1] = scale['h2o']
     out = K.alpha(jcat, K.absorbers_struct(JUPITER_ALPHA), g[C['P']], g[C['T']], K.mix_table(g, C), FREQS,
@@ -339,7 +340,8 @@ def test_alpha_rescale_reuses_saved_constituents(K, jcat, bench_atm):
     ab = K.absorbers_struct(JUPITER_ALPHA, 'e')
     P, T, mix = g[C['P']], g[C['T']], K.mix_table(g, C)
     rng = np.random.default_rng(5)
-    scale = rng.uniform(0.3, 3.0, size=(5, g.shape[1]))
+    from radiobear_amd import _lib
+    scale = rng.uniform(0.3, 3.0, size=(len(_lib.CONSTITUENTS), g.shape[1]))
     ref = K.alpha(jcat, ab, P, T, mix, FREQS, 'fp64', scale=scale).cpu().numpy()
     _, byc = K.alpha(jcat, ab, P, T, mix, FREQS, 'fp64', by_constituent=True)
     absorb = np.transpose(byc.cpu().numpy(), (1, 2, 0))          # (nl, nf, nc), sorted constituents

@@ -45,9 +45,20 @@ def test_argument_errors_are_reported_without_gpu_work():
     assert lib.rbx_catalog_lines(h, 0, ctypes.byref(n)) == 0 and n.value == 0
     ab = kernels.absorbers_struct({'nh3': 'nh3_hs'})
     dummy = ctypes.c_void_p(8)
-    rc = lib.rbx_alpha(h, ctypes.byref(ab), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy, None,
+    rc = lib.rbx_alpha(h, ctypes.byref(ab), dummy, dummy, dummy, None, 3, dummy, None, 2, 0, None, 0, dummy, None,
                        dummy, 0, None)
     assert rc == -4 and 'nh3_hs' in _lib.last_error()             # empty inversion block
+    for sel, what in (({'co': 'co_ddb'}, 'co_ddb'), ({'h2': 'h2_orton'}, 'h2_orton')):
+        rc = lib.rbx_alpha(h, ctypes.byref(kernels.absorbers_struct(sel)), dummy, dummy, dummy, None, 3, dummy,
+                           None, 2, 0, None, 0, dummy, None, dummy, 0, None)
+        assert rc == -4 and what in _lib.last_error()             # its block is empty
+    cl = kernels.absorbers_struct({'clouds': 'clouds_idp'})
+    rc = lib.rbx_alpha(h, ctypes.byref(cl), dummy, dummy, dummy, None, 3, dummy, None, 2, 0, None, 0, dummy, None,
+                       dummy, 0, None)
+    assert rc == -1 and 'cloud table' in _lib.last_error()        # clouds need the cloud densities
+    rc = lib.rbx_alpha_tangent(h, ctypes.byref(cl), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy,
+                               dummy, None)
+    assert rc == -1 and 'no tangent' in _lib.last_error()
     # workspace: partial sums [4 slots][parts][nl][nf] + h2o base [parts][nl] doubles
     nb = ctypes.c_size_t(0)
     h2 = kernels.absorbers_struct({'h2': 'h2_jj_ddb'})
@@ -78,9 +89,24 @@ def test_nh3_order_and_absorber_selection():
     with pytest.raises(ValueError):
         kernels.absorbers_struct({'h2': 'h2_jj_ddb'}, h2state='x')
     with pytest.raises(ValueError):
-        kernels.absorbers_struct({'nh3': 'nh3_dbs'})
+        kernels.absorbers_struct({'nh3': 'nh3_bg'})
+    with pytest.raises(ValueError):
+        kernels.absorbers_struct({'hcn': 'hcn_ddb'})
     with pytest.raises(ValueError):
         kernels.planet_desc(1.0, 1.0, gtype='gravity')
+    # the other planets' NH3 models and the optional absorbers with their options
+    assert kernels.absorbers_struct({'nh3': 'nh3_dbs_sjs'}).nh3 == 4
+    assert kernels.absorbers_struct({'nh3': 'nh3_dbs'}).nh3 == 5
+    assert kernels.absorbers_struct({'h2': 'h2_orton'}, h2state='n').h2 == 4
+    co = kernels.absorbers_struct({'co': 'co_ddb'})
+    assert (co.co, co.co_shape) == (1, 0)                          # coshape defaults to 'voigt'
+    for shape, code in (('vvw', 1), ('diff', 2), ('std', 3), (' ', 3)):
+        assert kernels.absorbers_struct({'co': 'co_ddb'}, other_dict={'co': {'coshape': shape}}).co_shape == code
+    flags = {'water_p': 1e-4, 'ice_p': 1e-4, 'nh4sh_p': 0.0, 'nh3ice_p': 1e-4, 'h2sice_p': 1e-4, 'ch4_p': 1e-4}
+    cl = kernels.absorbers_struct({'clouds': 'clouds_idp'}, other_dict={'clouds': flags})
+    # bits in RBX_CL_* order: ice (H2O) 0, water (SOLN) 1, nh4sh 2, nh3 3, h2s 4, ch4 5; the
+    # module tests the key 'ch4', which Alpha never copies ('ch4_p'), so CH4 stays off
+    assert (cl.clouds, cl.cloud_types) == (1, 0b011011)
 
 
 def test_truncation_matches_oracle(raw_lines):
