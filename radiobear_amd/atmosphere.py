@@ -44,6 +44,20 @@ def jupiter_no_tweak(gas, cloud, C, Cl):
 BUILTIN_TWEAKS = {'JupiterNoTweak': jupiter_no_tweak}
 
 
+def _interp_linear(x, y, xnew, fill):
+    """scipy interp1d(kind='linear', bounds_error=False, fill_value=fill) on ascending x:
+    the bracketing segment's line, `fill` outside [x[0], x[-1]]."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    xnew = np.asarray(xnew, dtype=np.float64)
+    hi = np.clip(np.searchsorted(x, xnew, side='left'), 1, len(x) - 1)
+    lo = hi - 1
+    slope = (y[hi] - y[lo]) / (x[hi] - x[lo])
+    out = slope * (xnew - x[lo]) + y[lo]
+    out[(xnew < x[0]) | (xnew > x[-1])] = fill
+    return out
+
+
 class Atmosphere:
     """gas (nC, nl), cloud, property (nLP, nl) and config, as the reference's Atmosphere."""
 
@@ -84,9 +98,19 @@ class Atmosphere:
         return atm
 
     def std(self):
-        """The file pipeline: read gas, regrid, tweak, derived properties."""
+        """The file pipeline: read gas and cloud, regrid, tweak, derived properties
+        (atmosphere.py:66-120).  A cloud file that is not there gives a cloud-free table
+        (the reference stops with an IOError); cloud densities only matter to clouds_idp."""
         self.read_gas()
-        self.cloud = np.zeros((len(self.config.Cl), self.gas.shape[1]))
+        cpath = os.path.join(self.config.path, self.config.cloudFile[self.idnum]) if self.config.cloudFile else None
+        if cpath is not None and os.path.exists(cpath):
+            self.read_cloud(cpath)
+        else:
+            if self.verbose:
+                print('no cloud file {}: cloud densities are zero'.format(cpath))
+            Cl = self.config.Cl
+            self.cloud = np.zeros((len(Cl), self.gas.shape[1]))
+            self.cloud[Cl['P']] = self.gas[self.config.C['P']]
         self.regrid()
         self.tweak()
         self.compute_prop()
@@ -98,6 +122,22 @@ class Atmosphere:
         cfg = self.config
         if path is None:
             path = os.path.join(cfg.path, cfg.gasFile[self.idnum])
+        self.gas, cfg.C = self._read_table(path, cfg.C)
+        self._renorm_z()
+
+    def read_cloud(self, path=None):
+        """atm_base.readCloud (atm_base.py:80-118): the cloud table laid out by config.Cl."""
+        cfg = self.config
+        if path is None:
+            path = os.path.join(cfg.path, cfg.cloudFile[self.idnum])
+        self.cloud, cfg.Cl = self._read_table(path, cfg.Cl)
+        self._renorm_z('cloud')
+
+    @staticmethod
+    def _read_table(path, layout):
+        """Rows of the most common entry count, missing columns zero, P ascending
+        (flipped if the file runs deep to high); returns the table and its layout
+        renumbered in column order (atm_base.py:39-118)."""
         with open(path) as fp:
             lines = fp.readlines()
         counts = {}
@@ -110,29 +150,29 @@ class Atmosphere:
         if len(keys) > 1 and 2 * sum(keys[1:]) > keys[0]:
             raise ValueError("Not enough data lines in file: {}".format(keys))
         expected = inv[keys[0]]
-        nC = len(cfg.C)
+        nC = len(layout)
         rows = []
         for ln in lines:
             cv = utils.get_data_from(ln)
             if cv is None or len(cv) != expected:
                 continue
             rows.append([cv[n] if n < len(cv) else 0.0 for n in range(nC)])
-        gas = np.array(rows, dtype=np.float64).T
-        order = sorted(cfg.C, key=lambda k: cfg.C[k])
-        cfg.C = {k: i for i, k in enumerate(order)}
-        P = gas[cfg.C['P']]
+        tab = np.array(rows, dtype=np.float64).T
+        order = sorted(layout, key=lambda k: layout[k])
+        layout = {k: i for i, k in enumerate(order)}
+        P = tab[layout['P']]
         if not np.all(np.diff(P) > 0.0):
-            gas = np.fliplr(gas)
-            if not np.all(np.diff(gas[cfg.C['P']]) > 0.0):
+            tab = np.fliplr(tab)
+            if not np.all(np.diff(tab[layout['P']]) > 0.0):
                 raise ValueError("Pressure not monotonically increasing in {}.".format(path))
-        self.gas = np.ascontiguousarray(gas)
-        self._renorm_z()
+        return np.ascontiguousarray(tab), layout
 
-    def _renorm_z(self):
+    def _renorm_z(self, which='gas'):
         """atm_base._renorm_z: deepest z = 0, DZ in cm (atm_base.py:206-219)."""
-        C = self.config.C
-        self.gas[C['Z']] = self.gas[C['Z']] - self.gas[C['Z']][-1]
-        self.gas[C['DZ']] = np.append(0.0, np.abs(np.diff(self.gas[C['Z']])) * 1.0E5)
+        ind = self.config.C if which == 'gas' else self.config.Cl
+        tab = self.gas if which == 'gas' else self.cloud
+        tab[ind['Z']] = tab[ind['Z']] - tab[ind['Z']][-1]
+        tab[ind['DZ']] = np.append(0.0, np.abs(np.diff(tab[ind['Z']])) * 1.0E5)
 
     def regrid(self):
         """regrid.regrid with an integer regridType: logspace(pmin, pmax, n), linear in P.
@@ -169,7 +209,18 @@ class Atmosphere:
             slope = (y[hi] - y[lo]) / (P[hi] - P[lo])
             new[row] = slope * (Pgrid - P[lo]) + y[lo]
         self.gas = new
+        # the cloud table: linear in P, zero outside the cloud file's range (regrid.py:74-77)
+        Cl = cfg.Cl
+        Pc = self.cloud[Cl['P']]
+        cloud = np.zeros((self.cloud.shape[0], len(Pgrid)))
+        cloud[Cl['P']] = Pgrid
+        for key, row in Cl.items():
+            if key in ('P', 'DZ'):
+                continue
+            cloud[row] = _interp_linear(Pc, self.cloud[row], Pgrid, 0.0)
+        self.cloud = cloud
         self._renorm_z()
+        self._renorm_z('cloud')
         return 1
 
     def tweak(self):

@@ -26,6 +26,14 @@ class Brightness:
         self.precision = precision
         self.tau_cut = tau_cut
 
+    @staticmethod
+    def _no_doppler(config):
+        # the reference's Doppler branch calls Alpha.get_alpha, which does not exist, and
+        # announces itself as broken (brightness.py:79-90): with Doppler on it cannot run
+        if getattr(config, 'Doppler', False):
+            raise NotImplementedError('Doppler is not supported: the reference branch is broken '
+                                      '(brightness.py:79-90 calls a missing Alpha.get_alpha)')
+
     def _modes(self, alpha):
         prec = self.precision or ('fp32' if alpha.table.dtype == torch.float32 else 'fp64')
         tau_cut = self.tau_cut if self.tau_cut is not None else (50.0 if prec == 'fp32' else 0.0)
@@ -34,6 +42,7 @@ class Brightness:
     def start_paths(self, bs, atm, orientation=None):
         """Launch the ray paths of bs on a side stream (they do not depend on the
         absorption, so they run beside it); batch(..., paths=) joins them."""
+        self._no_doppler(self.config)
         disc = [isinstance(b, str) or (not np.isscalar(b) and isinstance(b[0], str)) for b in bs]
         rays = np.array([[0.0, 0.0] if d else [float(b[0]), float(b[1])] for b, d in zip(bs, disc)])
         main = torch.cuda.current_stream()
@@ -46,6 +55,7 @@ class Brightness:
     def batch(self, bs, freqs, atm, alpha, orientation=None, paths=None):
         """Tb (nray, nf) numpy float64 for the rays bs; 'disc' rays use the disc average.
         paths: what start_paths returned for these rays, or None to compute them here."""
+        self._no_doppler(alpha.config)
         if paths is None:
             paths = self.start_paths(bs, atm, orientation)
         disc, hds, ns, side = paths
@@ -73,6 +83,7 @@ class Brightness:
 
     def single(self, b, freqs, atm, alpha, orientation=None, taulimit=20.0):
         """Tb (list over freqs) for one ray, with the reference's side attributes."""
+        self._no_doppler(alpha.config)
         disc_average = utils.b_type(b).startswith('dis')
         if disc_average:
             b = [0.0, 0.0]

@@ -190,3 +190,70 @@ def test_planet_run_writes_result_files(tmp_path):
     assert any(t.startswith('# imgSize: ') for t in text) and any(t.startswith('# res:  ') for t in text)
     rows = [t for t in text if t and not t.startswith('#')]
     assert len(rows) == j.imSize[0]
+
+
+@pytest.mark.parametrize('coshape,with_clouds', [('voigt', False), ('std', True)])
+def test_planet_run_neptune_absorbers(jupiter_lines, coshape, with_clouds):
+    """Neptune's default absorbers (co_ddb + nh3_dbs_sjs and the shared set, default_config.json
+    'alpha') through Planet.run on a synthetic atmosphere with CO, against the oracle's
+    absorption + Brightness.single (<= 0.01 K).  The default coshape 'voigt' sums CO line
+    shapes without their strengths (co_ddb.py:86-87), which makes the atmosphere opaque at
+    its top (every Tb is T_cmb, as in the reference); 'std' weights them by strength and
+    is run with clouds_idp over a cloud deck."""
+    from oracle import extra_absorbers as E
+    from radiobear_amd import _lib, lines
+    from radiobear_amd.planet import Planet
+    sa = synth.jupiter_like(120)
+    sa.gas[C['CO']] = 1.0e-6
+    p = Planet('neptune', config_file=None, verbose=False, atmosphere=sa)
+    assert p.alpha[0].constituent['co'] == 'co_ddb' and p.alpha[0].constituent['nh3'] == 'nh3_dbs_sjs'
+    atm = p.atmos[0]
+    Cl = p.config.Cl
+    P = atm.gas[C['P']]
+    cloud = np.zeros((len(Cl), atm.gas.shape[1]))
+    cloud[Cl['P']] = P
+    cloud[Cl['SOLN']] = 4e-4 * np.exp(-((np.log(P) - 1.0) / 0.4) ** 2)
+    cloud[Cl['H2O']] = 1e-4 * np.exp(-((np.log(P) - 0.2) / 0.4) ** 2)
+    cloud[Cl['NH4SH']] = 2e-5 * np.exp(-((np.log(P) + 0.5) / 0.3) ** 2)
+    atm.cloud = cloud
+    p.config.coshape = coshape
+    if with_clouds:
+        p.config.constituent_alpha['clouds'] = 'clouds_idp'
+    p.alpha[0].setup_formalisms()
+    freqs = [1.0, 5.0, 22.0, 40.0, 115.0]
+    bs = [[0.0, 0.0], [0.3, 0.2], [0.7, -0.4], [0.95, 0.0]]
+    rv = p.run(freqs, b=bs, reuse_override='False')
+    # the oracle: per layer, the absorbers in name order summed as total_layer_alpha does
+    co = lines.load('co')
+    hs = {'h2s': lambda f, T, Pl, X: OA.h2s_ddb(f, T, Pl, X, C, {}, data=jupiter_lines['h2s'], units='invcm'),
+          'h2o': lambda f, T, Pl, X: OA.h2o_bk(f, T, Pl, X, C, {}, data=jupiter_lines['h2o'], units='invcm'),
+          'ph3': lambda f, T, Pl, X: OA.ph3_jh(f, T, Pl, X, C, {}, data=jupiter_lines['ph3'], units='invcm'),
+          'h2': lambda f, T, Pl, X: OA.h2_jj_ddb(f, T, Pl, X, C, {'h2state': 'e'}, units='invcm'),
+          'nh3': lambda f, T, Pl, X: OA.nh3_dbs_sjs(f, T, Pl, X, C, {}, data_hs=jupiter_lines['nh3_hs'],
+                                                     data_sjs=jupiter_lines['nh3_sjs'], units='invcm'),
+          'co': lambda f, T, Pl, X: E.co_ddb(f, T, Pl, X, C, {'coshape': coshape}, data=co, units='invcm')}
+    flags = {k: getattr(p.config, k) for k in ('water_p', 'ice_p', 'nh4sh_p', 'nh3ice_p', 'h2sice_p', 'ch4_p')}
+    nl = atm.gas.shape[1]
+    alpha = np.zeros((len(freqs), nl))
+    for l in range(nl):
+        T, Pl, X = atm.gas[C['T'], l], atm.gas[C['P'], l], atm.gas[:, l]
+        cols = {c: np.asarray(fn(freqs, T, Pl, X), dtype=np.float64) for c, fn in hs.items()}
+        if with_clouds:
+            cols['clouds'] = np.asarray(E.clouds_idp(freqs, T, Pl, cloud[:, l], Cl, flags, units='invcm'))
+        tot = np.zeros(len(freqs))
+        for c in sorted(cols):
+            tot = tot + cols[c]
+        alpha[:, l] = tot
+    prop = atm.property
+    ref = []
+    for b in bs:
+        ray = geometry.compute_ds(prop[p.config.LP['R']], prop[p.config.LP['N']], b, p.config.Req, p.config.Rpol,
+                                  p.config.orientation)
+        ref.append(rt.single(alpha, atm.gas[C['T']], ray.ds, ray.layer4ds))
+    assert rel(p.alpha[0].layers, alpha) < 1e-9
+    assert np.max(np.abs(np.asarray(rv.Tb, dtype=np.float64) - np.array(ref))) < 0.01
+    if with_clouds:
+        assert p.alpha[0].absorbers.cloud_types == 0b011111     # all on but CH4: 'ch4' is never copied
+        assert np.max(np.abs(np.asarray(rv.Tb, dtype=np.float64)[0] - 2.725)) > 10.0
+    else:
+        assert np.all(np.asarray(rv.Tb) == np.float32(2.725))

@@ -82,6 +82,45 @@ def test_atmosphere_pipeline_matches_oracle(bench_gas_text):
     assert abs(g[i1] - c.GM_ref / c.Req ** 2) < 1e-12          # GM normalised at p_ref
 
 
+def test_cloud_file_read_and_regrid(tmp_path, bench_gas_text):
+    """readCloud + the cloud regrid (atm_base.py:80-118, regrid.py:74-81, 97-118): the
+    cloud table on the gas grid, linear in P, zero outside the cloud file's range,
+    deepest z = 0, checked against scipy's interp1d as the reference calls it."""
+    from scipy.interpolate import interp1d
+    (tmp_path / 'gas.dat').write_text(bench_gas_text)
+    # a deck written deep to high (the reader flips it), 3 bar .. 0.05 bar, clouds near 1 bar
+    P = np.geomspace(3.0, 0.05, 40)
+    z = np.linspace(0.0, 120.0, 40)
+    T = 150.0 + 40.0 * np.log10(P)
+    rows = []
+    for i in range(40):
+        soln = 5e-4 * np.exp(-((np.log(P[i]) - 0.7) / 0.3) ** 2)
+        ice = 2e-4 * np.exp(-((np.log(P[i]) + 0.5) / 0.4) ** 2)
+        nh3 = 3e-5 * np.exp(-((np.log(P[i]) + 1.2) / 0.3) ** 2)
+        rows.append('{:8.2f} {:9.3f} {:11.4e} {:11.4e} {:11.4e} {:11.4e} {:11.4e} {:11.4e} {:11.4e} {:11.4e}'
+                    .format(z[i], T[i], P[i], soln, ice, 1e-6, nh3, 0.0, 0.0, 0.0))
+    (tmp_path / 'cloud.dat').write_text('   Z T P SOLN H2O NH4SH NH3 H2S CH4 AR\n' + '\n'.join(rows) + '\n')
+    par = (tmp_path / 'config.par')
+    par.write_text(open(os.path.join(GOLDEN, 'config_benchmark.par')).read()
+                   .replace('gasfile jupiter.benchmark_gas', 'gasfile gas.dat')
+                   .replace('cloudfile jupiter.benchmark_nocloud', 'cloudfile cloud.dat'))
+    c = config.planetConfig('jupiter', str(par))
+    a = atmosphere.Atmosphere('jupiter', config=c)
+    a.std()
+    gas, _ = pipeline.benchmark_atmosphere(bench_gas_text)
+    assert np.array_equal(a.gas, gas)                          # clouds do not touch the gas table
+    Cl = c.Cl
+    Pg = a.gas[c.C['P']]
+    assert np.array_equal(a.cloud[Cl['P']], Pg)
+    Pin = np.array([float(r.split()[2]) for r in rows])[::-1]     # as written (4 digits)
+    for key, col in (('SOLN', 3), ('H2O', 4), ('NH3', 6)):
+        src = np.array([float(r.split()[col]) for r in rows])[::-1]
+        want = interp1d(Pin, src, kind='linear', fill_value=0.0, bounds_error=False)(Pg)
+        assert np.array_equal(a.cloud[Cl[key]], want), key
+    assert a.cloud[Cl['Z']][-1] == 0.0                         # deepest z = 0 after regrid
+    assert np.max(a.cloud[Cl['SOLN']]) > 1e-4 and np.all(a.cloud[Cl['SOLN']][Pg > 3.0] == 0.0)
+
+
 def test_synthetic_atmosphere_generator():
     atm = synth.jupiter_like(200)
     P, T = atm.P, atm.T
@@ -134,3 +173,17 @@ def test_result_files_in_reference_formats(tmp_path):
     assert lines[2] == '# GHz  K@b  \tdisc'
     t = datetime.datetime(2024, 5, 6, 7, 8, 9)
     assert output_name('Jupiter', 'spectrum', '_', t, 'Out') == os.path.join('Out', 'Jupiter_spectrum_20240506_070809.dat')
+
+
+def test_doppler_is_refused_before_any_device_work():
+    """The reference's Doppler branch cannot run (brightness.py:79-90 calls a missing
+    Alpha.get_alpha); here a Doppler config is refused up front."""
+    from argparse import Namespace
+    from radiobear_amd.brightness import Brightness
+    c = config.planetConfig('jupiter')
+    c.Doppler = True
+    br = Brightness(config=c, verbose=False)
+    with pytest.raises(NotImplementedError):
+        br.batch([[0.0, 0.0]], [1.0], None, Namespace(config=c, table=None))
+    with pytest.raises(NotImplementedError):
+        br.single([0.0, 0.0], [1.0], None, Namespace(config=c, table=None))
