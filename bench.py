@@ -245,43 +245,34 @@ def kernel_times(prob, prec, steps):
     return {n: acc[n] / steps for n in names}
 
 
-def cpu_baseline(prob, args):
-    """The FP64 oracle on a bounded sample, extrapolated to the whole workload."""
-    from oracle import geometry, layers, pipeline, rt
-    from radiobear_amd import lines
-    w = prob.w
-    atm = prob.atm
-    raw = lines.raw_catalog()
-    jl = pipeline.jupiter_lines(raw)
-    inj = dict(jl['nh3_hs'])
-    for k in ('fo', 'Io', 'Eo', 'gammaNH3o'):
-        inj[k] = w['nh3_inv'][k]
-    jl['nh3_hs'] = inj
-    if w['h2o'] is not None:
-        jl['h2o'] = w['h2o']
-    aset = layers.jupiter_absorbers(jl)
-    C = atm.C
-    sample_layers = [0, atm.nl // 2]
-    t0 = time.perf_counter()
-    for l in sample_layers:
-        aset.absorb(list(prob.freqs), atm.T[l], atm.P[l], atm.gas[:, l], C)
-    t_layer = (time.perf_counter() - t0) / len(sample_layers)
-    # geometry + integration on sample rays (on planet), with a cheap stand-in alpha table
-    rng = np.random.default_rng(0)
-    onp = [i for i in range(prob.nray) if np.hypot(*prob.rays[i]) < 0.99]
-    pick = rng.choice(onp, size=min(3, len(onp)), replace=False)
-    alpha_tab = np.full((prob.nf, atm.nl), 1e-7)
-    t0 = time.perf_counter()
-    for i in pick:
-        ray = geometry.compute_ds(atm.R, atm.N, prob.rays[i], atm.Req, atm.Rpol)
-        rt.single(alpha_tab, atm.T, ray.ds, ray.layer4ds)
-    t_ray = (time.perf_counter() - t0) / len(pick)
-    total = t_layer * atm.nl + t_ray * prob.on_planet
-    sample = ('{} layers of alpha at full lines x freqs ({:.2f} s/layer) and {} rays of geometry + '
-              'integration at full freqs x steps ({:.2f} s/ray), extrapolated to {} layers and {} '
-              'on-planet rays').format(len(sample_layers), t_layer, len(pick), t_ray, atm.nl, prob.on_planet)
-    return {'value': prob.on_planet * prob.nf / total, 'unit': 'ray*freq Tb/s', 'cores': 1, 'kind': 'port',
-            'sample': sample, 'seconds_extrapolated': total}
+def cpu_workers():
+    """Host processes for the all-core CPU baseline: the cores this process may run on,
+    at most 16 (a one-GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_measure(args):
+    """Per-item oracle times on one core and on all cores (taken before the GPU is touched:
+    the all-core pool forks this process)."""
+    from oracle import cpu_sample
+    state = cpu_sample.setup(args.workload)
+    one = cpu_sample.single_core(state)
+    allc = cpu_sample.all_core(state, cpu_workers())
+    return one, allc
+
+
+def cpu_baseline(prob, meas):
+    """The FP64 oracle on a bounded sample, extrapolated to the whole workload: all cores
+    (the value) and one core."""
+    from oracle import cpu_sample
+    one, allc = meas
+    rec = cpu_sample.baseline(allc, prob.nl, prob.nf, prob.on_planet)
+    rec['single_core'] = cpu_sample.baseline(one, prob.nl, prob.nf, prob.on_planet)
+    return rec
 
 
 def _rows_rate(rb, reps, rebuild=False):
@@ -356,6 +347,10 @@ def main():
         print('bench.py measures one GPU only: this build has no multi-GPU path '
               '(every GPU box here has one GPU); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
+    cpu_meas = None
+    if not (args.no_cpu_baseline or args.copy_calibration):
+        log(args, 'cpu baseline (oracle sample, one core and all cores)...')
+        cpu_meas = cpu_measure(args)
     import torch
     from radiobear_amd import _lib
     if not torch.cuda.is_available():
@@ -436,10 +431,10 @@ def main():
         if tr:
             roof['traffic'] = tr['read'] + tr['write']
             roof['traffic_source'] = 'rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)'
-    cpu = None
-    if not args.no_cpu_baseline:
-        log(args, 'cpu baseline (oracle sample)...')
-        cpu = cpu_baseline(prob, args)
+            # the HBM side of the same kernel: measured bytes over its live duration, vs 8 TB/s
+            roof['hbm_GBps'] = roof['traffic'] / (kt32[dom] * 1e-3) / 1e9
+            roof['hbm_frac'] = roof['hbm_GBps'] / PEAK['hbm']
+    cpu = cpu_baseline(prob, cpu_meas) if cpu_meas is not None else None
     value = prob.on_planet * prob.nf / dt32
     line = {
         'metric': 'ray*freq Tb/s (FP32 batched forward; Jacobian rows/s and FP64 plain alongside)',
