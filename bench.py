@@ -183,9 +183,11 @@ def autotune(prob, prec, args):
         ka = at.tune('alpha_k', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
         ks = at.tune('alpha_split', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
         ki = at.tune('int_unroll', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
+        km = at.tune('int_map', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
     torch.cuda.synchronize()
     return {'alpha_k': ka['best'], 'alpha_split': ks['best'], 'alpha_parts': prob.alpha_parts(prec),
-            'int_unroll': ki['best'], 'alpha_k_ms': ka['ms'], 'alpha_split_ms': ks['ms'], 'integrate_ms': ki['ms']}
+            'int_unroll': ki['best'], 'int_map': km['best'], 'alpha_k_ms': ka['ms'], 'alpha_split_ms': ks['ms'],
+            'integrate_ms': ki['ms'], 'int_map_ms': km['ms']}
 
 
 def other_workloads(args):

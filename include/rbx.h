@@ -154,7 +154,12 @@ int rbx_device_count(int *count);
 #define RBX_TUNE_ALPHA_K     0   /* frequencies per lane of rbx_alpha: 1, 2 or 4 */
 #define RBX_TUNE_INT_UNROLL  1   /* path steps per load batch of rbx_integrate: 4 or 8 */
 #define RBX_TUNE_ALPHA_SPLIT 2   /* parts each layer's line sums are split into: 1..16 */
-#define RBX_NUM_TUNE_KEYS    3
+#define RBX_TUNE_INT_MAP     3   /* rbx_integrate block order: 1 frequency group major (each XCD
+                                    keeps a slice of the table for every ray), 2 ray major (each
+                                    XCD runs every frequency group of its rays: their ds rows are
+                                    read once per ray, the whole table lives in each XCD's L2);
+                                    built in: ray major for tables up to 3.5 MiB */
+#define RBX_NUM_TUNE_KEYS    4
 int rbx_tune(int key, int value);
 
 /* ---- line catalogue (device-resident, owned by the library) ---- */

@@ -36,6 +36,7 @@ CO_SHAPES = {'voigt': 0, 'vvw': 1, 'diff': 2}                         # any othe
 TUNE_ALPHA_K = 0
 TUNE_INT_UNROLL = 1
 TUNE_ALPHA_SPLIT = 2
+TUNE_INT_MAP = 3
 
 EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_tune', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',

@@ -13,7 +13,9 @@ Knobs:
 * ``alpha_split`` parts each layer's line sums are split into (0 = the
                   library's model of the grid tail, or 1, 2, 4, 8): more parts
                   give more, shorter blocks, so the last wave of blocks idles less;
-* ``int_unroll``  path steps per load batch of rbx_integrate (4, 8).
+* ``int_unroll``  path steps per load batch of rbx_integrate (4, 8);
+* ``int_map``     block order of rbx_integrate: 1 frequency group major, 2 ray major
+                  (every frequency group of a ray block on one XCD).
 """
 import json
 import os
@@ -22,8 +24,9 @@ import torch
 
 from . import _lib
 
-CANDIDATES = {'alpha_k': (1, 2, 4), 'alpha_split': (0, 1, 2, 4, 8), 'int_unroll': (4, 8)}
-_KEYS = {'alpha_k': _lib.TUNE_ALPHA_K, 'alpha_split': _lib.TUNE_ALPHA_SPLIT, 'int_unroll': _lib.TUNE_INT_UNROLL}
+CANDIDATES = {'alpha_k': (1, 2, 4), 'alpha_split': (0, 1, 2, 4, 8), 'int_unroll': (4, 8), 'int_map': (1, 2)}
+_KEYS = {'alpha_k': _lib.TUNE_ALPHA_K, 'alpha_split': _lib.TUNE_ALPHA_SPLIT, 'int_unroll': _lib.TUNE_INT_UNROLL,
+         'int_map': _lib.TUNE_INT_MAP}
 _cache = {}
 
 

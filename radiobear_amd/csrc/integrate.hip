@@ -9,7 +9,9 @@
 // Path point k is layer k (the reference's paths are ingress-only), so alpha
 // for a step is one coalesced row read alpha[k][f0..f0+63]; the ray's ds and
 // T[k] are wave-uniform scalar loads.  Blocks map frequency groups to XCDs
-// (block b -> group b % ngroups) so each XCD's L2 holds a slice of the table.
+// (block b -> group b % ngroups) so each XCD's L2 holds a slice of the table, or,
+// ray-major (rbx_tune RBX_TUNE_INT_MAP 2), every group of a ray block to one XCD so the
+// ray's ds row is fetched once and the whole table stays in that XCD's L2.
 #include "common.hpp"
 
 namespace rbx {
@@ -30,12 +32,29 @@ struct IntArgs {
     double tau_cut;
     double *Tb, *intW;
     int ngroups;
+    int ray_major;   // block order: 0 frequency group major, 1 ray major (XCD-aware)
     // several atmospheres laid end to end (null: one atmosphere of nl layers):
     // ray r uses rows [layer_off[a], layer_off[a + 1]) of alpha and T, a = ray_atm[r]
     const int *ray_atm, *layer_off;
     int natm;
     const int *order;   // wavefront w integrates ray order[w] (null: ray w)
 };
+
+constexpr int kXcds = 8;   // MI355X: blocks are dealt round-robin to 8 XCDs (b % 8)
+
+// The frequency group and ray block (kWaves rays) of this block.  Ray major: the
+// blocks of XCD x (b = x, x + 8, ...) run ngroups groups of ray block x, then of x + 8, ...
+__device__ inline void block_tile(const struct IntArgs &A, int &group, int &rblock) {
+    const int b = blockIdx.x;
+    if (A.ray_major) {
+        const int q = b / kXcds;
+        group = q % A.ngroups;
+        rblock = (q / A.ngroups) * kXcds + b % kXcds;
+    } else {
+        group = b % A.ngroups;
+        rblock = b / A.ngroups;
+    }
+}
 
 // The ray a wavefront integrates (-1: none) and its first layer row and layer
 // count; false when its atmosphere index or offsets are out of range.
@@ -89,8 +108,9 @@ template <class AT, int MODE, bool FAST, int U>
 __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     const int wave = threadIdx.x / kLanes;
     const int lane = threadIdx.x % kLanes;
-    const int group = blockIdx.x % A.ngroups;
-    const int ray = ray_of(A, (blockIdx.x / A.ngroups) * kWaves + wave);
+    int group, rblock;
+    block_tile(A, group, rblock);
+    const int ray = ray_of(A, rblock * kWaves + wave);
     if (ray < 0) return;
     const int j = group * kLanes + lane;
     const bool ok = j < A.nf;
@@ -328,6 +348,7 @@ static void fill_int_args(IntArgs &A, const void *alpha, int alpha_precision, co
     A.Tb = nullptr;
     A.intW = nullptr;
     A.ngroups = (nf + kLanes - 1) / kLanes;
+    A.ray_major = 0;
     A.ray_atm = nullptr;
     A.layer_off = nullptr;
     A.natm = 0;
@@ -336,8 +357,17 @@ static void fill_int_args(IntArgs &A, const void *alpha, int alpha_precision, co
 
 }  // namespace rbx
 
-static int launch_integrate(const rbx::IntArgs &A, void *stream) {
-    const long long blocks = static_cast<long long>(A.ngroups) * ((A.nray + rbx::kWaves - 1) / rbx::kWaves);
+static int launch_integrate(rbx::IntArgs A, void *stream) {
+    const long long rblocks = (A.nray + rbx::kWaves - 1) / rbx::kWaves;
+    // ray major pads the ray blocks to a multiple of the XCD count (padding blocks find no ray).
+    // Built-in choice: ray major while the whole table fits one XCD's L2 (4 MiB) beside the
+    // streamed ds rows (measured on the 800 x 1024 table: 2.7 ms ray major in FP32, 3.3 ms
+    // group major; the 6.5 MB FP64 table is faster group major).
+    const int forced = rbx::tune_get(RBX_TUNE_INT_MAP);
+    const double table = static_cast<double>(A.nl) * A.nf * (A.f32 ? 4.0 : 8.0);
+    A.ray_major = forced ? forced == 2 : table <= 3.5 * 1024 * 1024;
+    const long long blocks = static_cast<long long>(A.ngroups) *
+                             (A.ray_major ? (rblocks + rbx::kXcds - 1) / rbx::kXcds * rbx::kXcds : rblocks);
     RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate: too many blocks");
     const dim3 grid(static_cast<unsigned>(blocks)), block(rbx::kWaves * rbx::kLanes);
     hipStream_t st = static_cast<hipStream_t>(stream);

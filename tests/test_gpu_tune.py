@@ -105,6 +105,34 @@ def test_integrate_every_unroll(K, at, precision):
         assert np.max(np.abs(b[ok] - base[ok])) < 1e-12 * scale, u
 
 
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_integrate_every_block_order(K, at, precision):
+    """Frequency-group-major and ray-major (XCD-aware) block orders give bit-identical Tb:
+    only which block integrates which (ray, frequency group) changes.  Ray counts that are
+    not a multiple of 4 x 8 leave padding blocks with no ray."""
+    import torch
+    atm = synth.jupiter_like(90)
+    rng = np.random.default_rng(4)
+    nf = 130                                        # 3 frequency groups, the last partial
+    tab = torch.as_tensor(rng.uniform(1e-9, 2e-6, size=(atm.nl, nf)), device='cuda')
+    if precision == 'fp32':
+        tab = tab.float()
+    pl = K.planet_desc(atm.Req, atm.Rpol)
+    for nray in (1, 37, 200):
+        rays = synth.random_disc_rays(nray, 12 + nray)
+        hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
+        out = {}
+        try:
+            for m in at.CANDIDATES['int_map']:
+                at.set_knob('int_map', m)
+                out[m] = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=50.0).cpu().numpy()
+        finally:
+            at.set_knob('int_map', 0)
+        a, b = out[1], out[2]
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)]), nray
+
+
 def test_tune_picks_a_candidate_and_caches(K, at, tmp_path):
     calls = []
     rec = at.tune('int_unroll', 'test-shape', lambda: calls.append(1), reps=1, path=str(tmp_path / 't.json'))
