@@ -346,6 +346,8 @@ __device__ __forceinline__ void sum_staged_f32(const Staged32 &st, const int (&s
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
+                // (x - Ds)^2 + h in FP64: h < 0 when a Ben-Reuven line's coupling exceeds its
+                // centre frequency (zeta > f0 + delta), and the sum then cancels
                 const double d1 = x[k] - Dp[k].x, d2 = x[k] - Dp[k].y;
                 const float e1 = static_cast<float>(fma(d1, d1, hp[k].x));
                 const float e2 = static_cast<float>(fma(d2, d2, hp[k].y));
