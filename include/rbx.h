@@ -60,7 +60,9 @@ extern "C" {
 #define RBX_FP64        0   /* plain FP64 everywhere (IEEE division, FP64 exp) */
 #define RBX_FP32        1   /* FP32 line sums, compensated (FP64 outer accumulation);
                                FP64 detuning, tau prefix sum and geometry; exp(-tau)
-                               from an FP64 range reduction and an FP32 exp2 */
+                               from an FP64 range reduction and an FP32 exp2; point-ray
+                               weights and trapezoid sums in FP32, added into FP64 totals
+                               every few steps */
 
 /* line-block kinds and their columns (float64, one value per line) */
 #define RBX_LINES_NH3_INV  0  /* fo, Io, Eo, gammaNH3o                     (ammonia.npz) */

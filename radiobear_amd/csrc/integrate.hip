@@ -2,7 +2,8 @@
 //
 // Lane = frequency, wavefront = one ray x 64 consecutive frequencies.  The
 // step loop follows Brightness.single (brightness.py:53-117):
-//   dtau = (a0 + a1) ds/2, tau_{i+1} = tau_i + dtau          (FP64 prefix sum)
+//   dtau = (a0 + a1) ds/2, tau_{i+1} = tau_i + dtau          (FP64 prefix sum; in the
+//   FP32 mode the increments, the weights and the trapezoid sums are FP32)
 //   W_{i+1} = a1 e^{-tau} (point) or 2 a1 E2(tau) (disc), W_0 = 0
 //   intW += (W_{i+1} + W_i) ds/2, Tb_lyr += (T1 W_{i+1} + T0 W_i) ds/2
 //   Tb = T_cmb if Tb_lyr < T_cmb else Tb_lyr / intW
@@ -146,6 +147,98 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
         for (; i < n; ++i) st(static_cast<double>(al[(i + 1) * nf]), h[i], Tl[i + 1]);
     }
     const double intW = st.intW, num = st.num;
+    if (!ok) return;
+    const size_t o = static_cast<size_t>(ray) * A.nf + j;
+    double Tb;
+    if (!valid)
+        Tb = __builtin_nan("");
+    else if (S < 0 || num < kTcmb)
+        Tb = kTcmb;
+    else
+        Tb = num / intW;
+    A.Tb[o] = Tb;
+    if (A.intW) A.intW[o] = (S < 0) ? 0.0 : intW;
+}
+
+// ---- FP32 mode, point rays: FP32 weights and trapezoid sums ----
+// tau stays an FP64 prefix sum (its increments (a0 + a1) ds/2 are FP32), e^-tau
+// keeps the FP64 range reduction, and W, T W and the two trapezoid sums are FP32,
+// added into FP64 totals after every batch of U steps (blocked summation).  Each
+// wavefront first copies its ray's half steps and layer temperatures to LDS as
+// FP32 pairs (h_i, T_{i+1}), so a step reads them with one broadcast LDS load.
+__device__ __forceinline__ float exp_neg_f32(double tau) {
+    const double t = -tau * kLog2e;
+    double n = __builtin_rint(t);
+    n = n < -160.0 ? -160.0 : n;   // e^-tau < 2^-150 is 0 in FP32
+    const float r = static_cast<float>(t - n);
+    return __builtin_ldexpf(__builtin_amdgcn_exp2f(r), static_cast<int>(n));
+}
+
+template <int U>
+__global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs A) {
+    extern __shared__ float2 hT_all[];   // [kWaves][ld]
+    const int wave = threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
+    int group, rblock;
+    block_tile(A, group, rblock);
+    const int ray = ray_of(A, rblock * kWaves + wave);
+    if (ray < 0) return;
+    const int j = group * kLanes + lane;
+    const bool ok = j < A.nf;
+    const int jj = ok ? j : A.nf - 1;
+    int base, nlr;
+    const bool valid = ray_span(A, ray, base, nlr);
+    const int S = valid ? min(A.nsteps[ray], nlr - 1) : -1;   // never read past the ray's ds row or table
+    const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
+    const size_t nf = A.nf;
+    const float *al = static_cast<const float *>(A.alpha) + static_cast<size_t>(base) * nf + jj;
+    const double *Tl = A.T + base;
+    float2 *hT = hT_all + static_cast<size_t>(wave) * A.ld;
+    double tau = 0.0, intW = 0.0, num = 0.0;
+    if (S > 1) {
+        const int n = S - 1;
+        for (int k = lane; k < n; k += kLanes)
+            hT[k] = make_float2(static_cast<float>(h[k]), static_cast<float>(Tl[k + 1]));
+        wave_sync();
+        float a0 = al[0], W0 = 0.0f, TW0 = 0.0f;
+        int i = 0;
+        for (; i + U <= n; i += U) {
+            float ab[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) ab[u] = al[(i + u + 1) * nf];
+            float iW = 0.0f, nm = 0.0f;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float2 ht = hT[i + u];
+                tau += static_cast<double>((a0 + ab[u]) * ht.x);
+                const float W1 = ab[u] * exp_neg_f32(tau);
+                iW = fmaf(W1 + W0, ht.x, iW);
+                const float TW1 = ht.y * W1;
+                nm = fmaf(TW1 + TW0, ht.x, nm);
+                a0 = ab[u];
+                W0 = W1;
+                TW0 = TW1;
+            }
+            intW += static_cast<double>(iW);
+            num += static_cast<double>(nm);
+            if (A.tau_cut > 0.0 && __all(!ok || tau > A.tau_cut)) {
+                i = n;
+                break;
+            }
+        }
+        for (; i < n; ++i) {
+            const float a1 = al[(i + 1) * nf];
+            const float2 ht = hT[i];
+            tau += static_cast<double>((a0 + a1) * ht.x);
+            const float W1 = a1 * exp_neg_f32(tau);
+            intW += static_cast<double>((W1 + W0) * ht.x);
+            const float TW1 = ht.y * W1;
+            num += static_cast<double>((TW1 + TW0) * ht.x);
+            a0 = a1;
+            W0 = W1;
+            TW0 = TW1;
+        }
+    }
     if (!ok) return;
     const size_t o = static_cast<size_t>(ray) * A.nf + j;
     double Tb;
@@ -376,7 +469,13 @@ static int launch_integrate(rbx::IntArgs A, void *stream) {
     if (A.mode == 1) {
         if (A.f32) RBX_INT_LAUNCH(float, 1, false, 4); else RBX_INT_LAUNCH(double, 1, false, 4);
     } else if (A.f32) {
-        if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(float, 0, true, 4); else RBX_INT_LAUNCH(float, 0, true, 8); }
+        const size_t lds = sizeof(float2) * rbx::kWaves * static_cast<size_t>(A.ld);
+        if (A.fast_exp && lds <= 64 * 1024) {
+            if (u4)
+                hipLaunchKernelGGL(rbx::integrate_f32_kernel<4>, grid, block, lds, st, A);
+            else
+                hipLaunchKernelGGL(rbx::integrate_f32_kernel<8>, grid, block, lds, st, A);
+        } else if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(float, 0, true, 4); else RBX_INT_LAUNCH(float, 0, true, 8); }
         else { if (u4) RBX_INT_LAUNCH(float, 0, false, 4); else RBX_INT_LAUNCH(float, 0, false, 8); }
     } else {
         if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(double, 0, true, 4); else RBX_INT_LAUNCH(double, 0, true, 8); }

@@ -77,7 +77,8 @@ def test_alpha_every_split(K, at, raw_lines, jupiter_lines, precision, tol):
 @pytest.mark.parametrize('precision', ['fp64', 'fp32'])
 def test_integrate_every_unroll(K, at, precision):
     """4 and 8 path steps per load batch: the same Tb to rounding (the compiler may contract
-    multiply-adds differently in the two loop bodies) without the tau cut, and with the cut
+    multiply-adds differently in the two loop bodies; in FP32 mode the FP32 trapezoid sums
+    are added into the FP64 totals once per batch) without the tau cut, and with the cut
     (checked once per batch, so the two stop at different steps) within e^-50 of it."""
     import torch
     atm = synth.jupiter_like(120)
@@ -101,8 +102,9 @@ def test_integrate_every_unroll(K, at, precision):
     for u, (a, b) in res.items():
         assert np.array_equal(np.isnan(a), ~ok)
         scale = np.max(np.abs(base[ok]))
-        assert np.max(np.abs(a[ok] - base[ok])) < 1e-13 * scale, u
-        assert np.max(np.abs(b[ok] - base[ok])) < 1e-12 * scale, u
+        tol = 1e-13 if precision == 'fp64' else 2e-6
+        assert np.max(np.abs(a[ok] - base[ok])) < tol * scale, u
+        assert np.max(np.abs(b[ok] - base[ok])) < max(tol, 1e-12) * scale, u
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'fp32'])
