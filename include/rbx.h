@@ -153,7 +153,8 @@ int rbx_device_count(int *count);
 
 /* Launch-parameter overrides (no reference counterpart; used by the autotuner):
  * applies to every later call in the process; value 0 restores the built-in choice. */
-#define RBX_TUNE_ALPHA_K     0   /* frequencies per lane of rbx_alpha: 1, 2 or 4 */
+#define RBX_TUNE_ALPHA_K     0   /* frequencies per lane of rbx_alpha: 1, 2 or 4 (built in: 2 from
+                                    256 frequencies up, else 1) */
 #define RBX_TUNE_INT_UNROLL  1   /* path steps per load batch of rbx_integrate: 4 or 8 */
 #define RBX_TUNE_ALPHA_SPLIT 2   /* parts each layer's line sums are split into: 1..16 */
 #define RBX_TUNE_INT_MAP     3   /* rbx_integrate block order: 1 frequency group major (each XCD
