@@ -278,7 +278,40 @@ struct Staged32 {
     alignas(16) double D[2][kThreads + 4];   // Ds
     alignas(16) double h[2][kThreads + 4];
     alignas(16) float2 ac[2][kThreads + 4];  // a, c (scaled)
+    alignas(16) float2 dh[2][kThreads + 4];  // Ds, h rounded to FP32 (the FP32 fast path)
+    // per set and column wavefront: bit j = staged lines [j kSub, (j + 1) kSub) are all
+    // FP32-safe for that wavefront's columns (byte w written by staging wavefront w)
+    unsigned int fast[2][kThreads / 64];
 };
+
+// Column range [lo, hi] of x = f^2 (var 0) and of x for NH3 (var 1) per wavefront of a block.
+struct WaveRanges {
+    double lo[2][kThreads / 64], hi[2][kThreads / 64];
+};
+
+// FP32 detuning is safe for a line against columns x in [lo, hi] when its shifted centre
+// Ds lies below 0.6 lo or above hi / 0.6: then |x - Ds| >= (x + |Ds|) / 4 at every column,
+// so (float)x - (float)Ds carries at most about five FP32 roundings relative to x - Ds.
+// A line with h < 0 also needs (x - Ds)^2 >= 4 |h| there, so that (x - Ds)^2 + h does not
+// cancel.  An empty range (lo = +inf) is safe; NaN coefficients are not.
+__device__ inline bool fp32_safe(double Ds, double h, double lo, double hi) {
+    double dmin;
+    if (Ds <= 0.6 * lo)
+        dmin = lo - Ds;
+    else if (0.6 * Ds >= hi)
+        dmin = Ds - hi;
+    else
+        return false;
+    return h >= 0.0 || dmin * dmin >= -4.0 * h;
+}
+
+// bit j set when bits [8 j, 8 j + 8) of m are all set (kSub = 8 lines per chunk)
+__device__ inline unsigned int full_chunks(unsigned long long m) {
+    unsigned int b = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b |= ((m >> (8 * j)) & 0xFFull) == 0xFFull ? 1u << j : 0u;
+    return b;
+}
 
 template <class S>
 __device__ inline void shifted_denominator(const CoefT<S> &q, S &Ds, S &h) {
@@ -319,42 +352,64 @@ __device__ __forceinline__ void line_coefs(const Prep *prep, int need, const Dev
 template <int K, bool UNI>
 __device__ __forceinline__ void sum_staged_f32(const Staged32 &st, const int (&set)[K], int cntp,
                                                const double (&x)[K], const float (&xf)[K], double (&acc)[K]) {
+    // chunks of kSub lines that are FP32-safe for this wavefront's columns take the all-FP32
+    // path (the same sums with (float)x - (float)Ds and d^2 + h in FP32); the mask is
+    // wavefront-uniform, so the choice is a scalar branch per chunk
+    const unsigned int fast =
+        UNI ? __builtin_amdgcn_readfirstlane(st.fast[set[0]][threadIdx.x / 64]) : 0u;
     for (int j0 = 0; j0 < cntp; j0 += kSub) {
         float p[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) p[k] = 0.0f;
-        // two lines over one denominator: n1/e1 + n2/e2 = (n1 e2 + n2 e1)/(e1 e2),
-        // one reciprocal per pair (denominators stay within 1e-17..1e27 here).  The
-        // mixed-set path (one wavefront per layer at most) reads K times as much
-        // and is not unrolled, to stay within the uniform path's registers.
-        constexpr int kPairUnroll = UNI ? kSub / 2 : 1;
-#pragma unroll kPairUnroll
-        for (int u = 0; u < kSub; u += 2) {
-            double2 Dp[K], hp[K];
-            float4 qp[K];
+        if (UNI && ((fast >> (j0 / kSub)) & 1u)) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (UNI && k > 0) {
-                    Dp[k] = Dp[0];
-                    hp[k] = hp[0];
-                    qp[k] = qp[0];
-                } else {   // pairs of lines as 16-byte LDS reads
-                    Dp[k] = reinterpret_cast<const double2 *>(st.D[set[k]])[(j0 + u) / 2];
-                    hp[k] = reinterpret_cast<const double2 *>(st.h[set[k]])[(j0 + u) / 2];
-                    qp[k] = reinterpret_cast<const float4 *>(st.ac[set[k]])[(j0 + u) / 2];
+            for (int u = 0; u < kSub; u += 2) {
+                const float4 dp = reinterpret_cast<const float4 *>(st.dh[set[0]])[(j0 + u) / 2];
+                const float4 qp = reinterpret_cast<const float4 *>(st.ac[set[0]])[(j0 + u) / 2];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const float d1 = xf[k] - dp.x, d2 = xf[k] - dp.z;
+                    const float e1 = fmaf(d1, d1, dp.y), e2 = fmaf(d2, d2, dp.w);
+                    const float n1 = fmaf(qp.x, xf[k], qp.y);
+                    const float n2 = fmaf(qp.z, xf[k], qp.w);
+                    const float num = fmaf(n2, e1, n1 * e2);
+                    p[k] = fmaf(num, __builtin_amdgcn_rcpf(e1 * e2), p[k]);
                 }
             }
+        } else {
+            // two lines over one denominator: n1/e1 + n2/e2 = (n1 e2 + n2 e1)/(e1 e2),
+            // one reciprocal per pair (denominators stay within 1e-17..1e27 here).  The
+            // mixed-set path (one wavefront per layer at most) reads K times as much
+            // and is not unrolled, to stay within the uniform path's registers.
+            constexpr int kPairUnroll = UNI ? kSub / 2 : 1;
+#pragma unroll kPairUnroll
+            for (int u = 0; u < kSub; u += 2) {
+                double2 Dp[K], hp[K];
+                float4 qp[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                // (x - Ds)^2 + h in FP64: h < 0 when a Ben-Reuven line's coupling exceeds its
-                // centre frequency (zeta > f0 + delta), and the sum then cancels
-                const double d1 = x[k] - Dp[k].x, d2 = x[k] - Dp[k].y;
-                const float e1 = static_cast<float>(fma(d1, d1, hp[k].x));
-                const float e2 = static_cast<float>(fma(d2, d2, hp[k].y));
-                const float n1 = fmaf(qp[k].x, xf[k], qp[k].y);
-                const float n2 = fmaf(qp[k].z, xf[k], qp[k].w);
-                const float num = fmaf(n2, e1, n1 * e2);
-                p[k] = fmaf(num, __builtin_amdgcn_rcpf(e1 * e2), p[k]);
+                for (int k = 0; k < K; ++k) {
+                    if (UNI && k > 0) {
+                        Dp[k] = Dp[0];
+                        hp[k] = hp[0];
+                        qp[k] = qp[0];
+                    } else {   // pairs of lines as 16-byte LDS reads
+                        Dp[k] = reinterpret_cast<const double2 *>(st.D[set[k]])[(j0 + u) / 2];
+                        hp[k] = reinterpret_cast<const double2 *>(st.h[set[k]])[(j0 + u) / 2];
+                        qp[k] = reinterpret_cast<const float4 *>(st.ac[set[k]])[(j0 + u) / 2];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    // (x - Ds)^2 + h in FP64: h < 0 when a Ben-Reuven line's coupling exceeds its
+                    // centre frequency (zeta > f0 + delta), and the sum then cancels
+                    const double d1 = x[k] - Dp[k].x, d2 = x[k] - Dp[k].y;
+                    const float e1 = static_cast<float>(fma(d1, d1, hp[k].x));
+                    const float e2 = static_cast<float>(fma(d2, d2, hp[k].y));
+                    const float n1 = fmaf(qp[k].x, xf[k], qp[k].y);
+                    const float n2 = fmaf(qp[k].z, xf[k], qp[k].w);
+                    const float num = fmaf(n2, e1, n1 * e2);
+                    p[k] = fmaf(num, __builtin_amdgcn_rcpf(e1 * e2), p[k]);
+                }
             }
         }
 #pragma unroll
@@ -387,7 +442,7 @@ __device__ __forceinline__ void sum_staged_f64(const Staged64 &st, const int (&s
 template <int K, bool F32, class Prep>
 __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, bool active, const int (&set)[K],
                          const double *scale, const double (&x)[K], double (&acc)[K], SharedFwd &sh, double &base,
-                         int lb, int le) {
+                         int lb, int le, const WaveRanges &rng, int var) {
     float xf[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) xf[k] = static_cast<float>(x[k]);
@@ -399,6 +454,7 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, bool act
         const int i = l0 + threadIdx.x;
         const int cnt = min(kThreads, le - l0);
         const int cntp = (cnt + kSub - 1) / kSub * kSub;
+        unsigned int safe[2] = {0u, 0u};   // FP32 mode: bit w = safe for wavefront w's columns
         if (static_cast<int>(threadIdx.x) < cntp) {
             CoefT<double> qs[2] = {{0.0, 0.0, 0.0, 0.0, 1.0}, {0.0, 0.0, 0.0, 0.0, 1.0}};   // padding: 0
             if (i < le) line_coefs(prep, need, B, i, base, qs);
@@ -413,12 +469,30 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, bool act
                     sh.s32.h[s][threadIdx.x] = h;
                     sh.s32.ac[s][threadIdx.x] =
                         make_float2(static_cast<float>(q.a * scale[s]), static_cast<float>(q.c * scale[s]));
+                    sh.s32.dh[s][threadIdx.x] = make_float2(static_cast<float>(Ds), static_cast<float>(h));
+#pragma unroll
+                    for (int w = 0; w < kThreads / 64; ++w)
+                        safe[s] |= fp32_safe(Ds, h, rng.lo[var][w], rng.hi[var][w]) ? 1u << w : 0u;
                 } else {
                     sh.s64.D[s][threadIdx.x] = q.D;
                     sh.s64.a[s][threadIdx.x] = q.a;
                     sh.s64.c[s][threadIdx.x] = q.c;
                     sh.s64.g[s][threadIdx.x] = q.g;
                     sh.s64.G[s][threadIdx.x] = q.G;
+                }
+            }
+        }
+        if constexpr (F32) {
+            // chunk masks: staging wavefront sw holds lines [64 sw, 64 sw + 64) = chunks
+            // [8 sw, 8 sw + 8); lanes past cntp report unsafe (those chunks are not read)
+            const int sw = threadIdx.x / 64;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                if (!(need & (1 << s))) continue;
+#pragma unroll
+                for (int w = 0; w < kThreads / 64; ++w) {
+                    const unsigned int b = full_chunks(__ballot((safe[s] >> w) & 1u));
+                    if (threadIdx.x % 64 == 0) reinterpret_cast<unsigned char *>(&sh.s32.fast[s][w])[sw] = b;
                 }
             }
         }
@@ -703,6 +777,36 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         xn[k] = fn * fn;
     }
 
+    // each wavefront's column range of x and of x for NH3 (the FP32 fast path's test)
+    __shared__ WaveRanges rng;
+    if constexpr (F32) {
+        double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (!ok[k]) continue;
+            lo[0] = fmin(lo[0], x[k]);
+            hi[0] = fmax(hi[0], x[k]);
+            lo[1] = fmin(lo[1], xn[k]);
+            hi[1] = fmax(hi[1], xn[k]);
+        }
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                lo[v] = fmin(lo[v], __shfl_xor(lo[v], m));
+                hi[v] = fmax(hi[v], __shfl_xor(hi[v], m));
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                rng.lo[v][wave] = lo[v];
+                rng.hi[v][wave] = hi[v];
+            }
+        }
+        __syncthreads();
+    }
+
     const DevBlock *blk = A.cat.blk;
     const LinePlan plan(A, L.P);
     const int *nr = plan.nr;
@@ -739,7 +843,7 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         H2oPrep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_H2O].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_H2O], &prep, 1, active, one, s, x, a2, sh, base, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_H2O], &prep, 1, active, one, s, x, a2, sh, base, lb, le, rng, 0);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = a2[k] / s[0];
         did_h2o = true;
@@ -749,7 +853,7 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         H2sPrep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_H2S].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_H2S], &prep, 1, active, one, s, x, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_H2S], &prep, 1, active, one, s, x, a2, sh, unused, lb, le, rng, 0);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = a2[k] / s[0];
     }
@@ -759,7 +863,7 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         const double s[2] = {F32 ? pow2_scale(prep[0].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0,
                              F32 ? pow2_scale(prep[1].pref * blk[RBX_LINES_NH3_INV].wscale) : 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_NH3_INV], prep, need_inv, active, set, s, xn, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_NH3_INV], prep, need_inv, active, set, s, xn, a2, sh, unused, lb, le, rng, 1);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[set[k]];
     }
@@ -767,7 +871,7 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         RotPrep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_NH3_ROT].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_NH3_ROT], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_NH3_ROT], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le, rng, 1);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
     }
@@ -775,7 +879,7 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         V2Prep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_NH3_V2].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_NH3_V2], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_NH3_V2], &prep, 1, active, one, s, xn, a2, sh, unused, lb, le, rng, 1);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] += a2[k] / s[0];
     }
@@ -784,7 +888,7 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 3 : 4) void alpha_lines_kernel(A
         Ph3Prep<double> prep(L);
         const double s[2] = {F32 ? pow2_scale(prep.pref * blk[RBX_LINES_PH3].wscale) : 1.0, 1.0};
         double a2[K] = {};
-        line_sum<K, F32>(blk[RBX_LINES_PH3], &prep, 1, active, one, s, x, a2, sh, unused, lb, le);
+        line_sum<K, F32>(blk[RBX_LINES_PH3], &prep, 1, active, one, s, x, a2, sh, unused, lb, le, rng, 0);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = a2[k] / s[0];
     }

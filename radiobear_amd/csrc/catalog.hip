@@ -1,4 +1,5 @@
 // Error handling, version query and the line catalogue of librbx.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -121,10 +122,22 @@ extern "C" int rbx_catalog_set_lines(rbx_catalog *cat, int kind, int n, const do
     for (int c = 0; c < ncol; ++c) total += rbx::column_length(kind, c, n);
     const size_t bytes = sizeof(double) * total;
     std::vector<double> packed(total);
+    // the line lists summed by the forward line-sum kernel are stored by ascending
+    // centre frequency (stable): neighbouring staged lines then sit at similar
+    // distances from a wavefront's columns, which its FP32 fast path relies on
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    if (rbx::sorted_by_centre(kind))
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return host_columns[0][a] < host_columns[0][b]; });
     size_t at = 0;
     for (int c = 0; c < ncol; ++c) {
         const size_t len = rbx::column_length(kind, c, n);
-        std::memcpy(packed.data() + at, host_columns[c], sizeof(double) * len);
+        if (rbx::sorted_by_centre(kind)) {
+            for (int i = 0; i < n; ++i) packed[at + i] = host_columns[c][order[i]];
+        } else {
+            std::memcpy(packed.data() + at, host_columns[c], sizeof(double) * len);
+        }
         at += len;
     }
     double *d = nullptr;

@@ -16,6 +16,13 @@ inline size_t column_length(int kind, int c, int n) {
     return c == 2 ? static_cast<size_t>(RBX_ORTON_NTEMP) : static_cast<size_t>(n);
 }
 
+// line lists the forward line-sum kernel stages (every column has one value per line):
+// stored sorted by centre frequency, column 0
+inline bool sorted_by_centre(int kind) {
+    return kind == RBX_LINES_NH3_INV || kind == RBX_LINES_NH3_ROT || kind == RBX_LINES_NH3_V2 ||
+           kind == RBX_LINES_H2S || kind == RBX_LINES_PH3 || kind == RBX_LINES_H2O;
+}
+
 // One block of lines as the kernels see it: SoA float64 columns in HBM.
 struct DevBlock {
     const double *col[kMaxCols];

@@ -28,9 +28,11 @@ def at(K):
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
 def test_alpha_every_tile_width(K, at, raw_lines, jupiter_lines, precision, tol):
-    """1, 2 and 4 frequencies per lane: identical tables (the per-frequency line order does
-    not depend on the tile width), all matching the oracle.  330 freqs straddle the
-    30 GHz regime switch inside a tile; 41 layers leave a partial last tile."""
+    """1, 2 and 4 frequencies per lane, all matching the oracle.  FP64: identical tables (the
+    per-frequency line order does not depend on the tile width).  FP32: a chunk of 8 staged
+    lines takes the all-FP32 path when it is far from every column of the wavefront, so the
+    choice depends on the tile's frequency range; tables agree to 1e-6.  330 freqs straddle
+    the 30 GHz regime switch inside a tile; 41 layers leave a partial last tile."""
     atm = synth.jupiter_like(41)
     h2o = synth.h2o_lines(300, 5)
     cat = K.jupiter_catalog(raw_lines, h2o=h2o)
@@ -46,7 +48,10 @@ def test_alpha_every_tile_width(K, at, raw_lines, jupiter_lines, precision, tol)
         got = out[k].T
         assert np.max(np.abs(got - ref) / np.abs(ref)) < tol, k
     for k in out:
-        assert np.array_equal(out[k], out[1]), k
+        if precision == 'fp64':
+            assert np.array_equal(out[k], out[1]), k
+        else:
+            assert np.max(np.abs(out[k] - out[1]) / np.abs(out[1])) < 1e-6, k
 
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
