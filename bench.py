@@ -116,11 +116,6 @@ class Problem:
                                         K._ptr(self.mix), None, self.nl, K._ptr(self.f), None, self.nf, self.nf_low,
                                         None, prec, K._ptr(out), None, K._ptr(self.ws), nbytes, K._stream()))
 
-    def alpha_parts(self, prec):
-        """Parts per layer of the line sums under the current launch settings."""
-        _, nbytes = self.K.alpha_workspace(self.cat, self.ab, self.nl, self.nf, self.nf_low, prec, reuse=self.ws)
-        return nbytes // 8 // (4 * self.nl * self.nf + self.nl)
-
     def launch_paths(self):
         import ctypes
         from radiobear_amd import _lib
@@ -172,7 +167,7 @@ class Problem:
 def autotune(prob, prec, args):
     """Pick the launch shapes of rbx_alpha / rbx_integrate for this shape (outside the timed region)."""
     import torch
-    from radiobear_amd import autotune as at
+    from radiobear_amd import _lib, autotune as at
     if args.no_tune:
         at.reset()
         return None
@@ -180,12 +175,16 @@ def autotune(prob, prec, args):
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         prob.launch_paths()
+        # FP32: far-line series (1 off, 2 on), then the tile width and split for that path
+        kf = at.tune('alpha_series', key, lambda: prob.launch_alpha(prec), path=args.tune_cache) \
+            if prec == _lib.FP32 else None
         ka = at.tune('alpha_k', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
         ks = at.tune('alpha_split', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
         ki = at.tune('int_unroll', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
         km = at.tune('int_map', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
     torch.cuda.synchronize()
-    return {'alpha_k': ka['best'], 'alpha_split': ks['best'], 'alpha_parts': prob.alpha_parts(prec),
+    return {'alpha_path': 'far-line series' if kf and kf['best'] == 2 else 'direct line sums',
+            'alpha_series_ms': kf['ms'] if kf else None, 'alpha_k': ka['best'], 'alpha_split': ks['best'],
             'int_unroll': ki['best'], 'int_map': km['best'], 'alpha_k_ms': ka['ms'], 'alpha_split_ms': ks['ms'],
             'integrate_ms': ki['ms'], 'int_map_ms': km['ms']}
 

@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 3
+#define RBX_ABI_VERSION 4
 
 /* status codes */
 #define RBX_OK          0
@@ -162,7 +162,11 @@ int rbx_device_count(int *count);
                                     XCD runs every frequency group of its rays: their ds rows are
                                     read once per ray, the whole table lives in each XCD's L2);
                                     built in: ray major for tables up to 3.5 MiB */
-#define RBX_NUM_TUNE_KEYS    4
+#define RBX_TUNE_ALPHA_SERIES 4  /* FP32 rbx_alpha: 1 sums every line at every frequency; 2 sums
+                                    per wavefront tile only the lines whose poles lie within a
+                                    factor 4 of the tile's frequencies and adds the others as 12-term
+                                    power series (1e-7 of their magnitude); built in: 2 */
+#define RBX_NUM_TUNE_KEYS    5
 int rbx_tune(int key, int value);
 
 /* ---- line catalogue (device-resident, owned by the library) ---- */

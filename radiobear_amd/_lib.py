@@ -7,7 +7,7 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'librbx.so')
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 FP64 = 0
 FP32 = 1
@@ -37,6 +37,7 @@ TUNE_ALPHA_K = 0
 TUNE_INT_UNROLL = 1
 TUNE_ALPHA_SPLIT = 2
 TUNE_INT_MAP = 3
+TUNE_ALPHA_SERIES = 4
 
 EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_tune', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',

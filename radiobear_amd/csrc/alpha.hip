@@ -99,6 +99,8 @@ struct CoefT {
     S D, a, c, g, G;
 };
 
+constexpr int kFarStreams = 7;   // near-mask streams: h2o, h2s, inversion set 0, set 1, rot, v2, ph3
+
 struct AlphaArgs {
     DevCatalog cat;
     rbx_absorbers abs;
@@ -116,6 +118,15 @@ struct AlphaArgs {
     double *ws;        // forward: partial line sums [kSlots][nsplit][nl][nf]
     double *ws_base;   // forward: partial h2o base terms [nsplit][nl]
     double *ws_extra;  // forward: optional absorbers [kNumExtra][nl][nf] (when any is selected)
+    // FP32 far-line series (far.hpp): near-line masks [nl][tiles_total][far_words], moments
+    // [nl][far_split][tiles_total][kFarMom], h2o base terms [far_split][nl]
+    unsigned int *near;
+    double2 *near_dh;   // staged lines per layer and stream line [nl][far_words * 32]: Ds, h
+    float2 *near_ac;    // and the scaled a, c
+    double *far_mom, *far_base;
+    double *far_scale;  // FP32 line-weight scale per layer, segment and coefficient set [nl][kSegs][2]
+    int far_words, far_split;
+    int far_off[kFarStreams];   // first mask word of each stream
 };
 
 // ---- per-layer coefficient sets: each derives (D, a, c, g, G) of one line ----
@@ -349,15 +360,15 @@ __device__ __forceinline__ void line_coefs(const Prep *prep, int need, const Dev
 // set[k]: the coefficient set column k reads (the nh3_hs inversion lines have a
 // <= 30 GHz and a > 30 GHz set; every other block has one).  UNI: every column of
 // the wavefront reads the same set, so one LDS read serves all K columns.
-template <int K, bool UNI>
-__device__ __forceinline__ void sum_staged_f32(const Staged32 &st, const int (&set)[K], int cntp,
+template <int K, bool UNI, class ST = Staged32>
+__device__ __forceinline__ void sum_staged_f32(const ST &st, const int (&set)[K], int jb, int je,
                                                const double (&x)[K], const float (&xf)[K], double (&acc)[K]) {
     // chunks of kSub lines that are FP32-safe for this wavefront's columns take the all-FP32
     // path (the same sums with (float)x - (float)Ds and d^2 + h in FP32); the mask is
     // wavefront-uniform, so the choice is a scalar branch per chunk
     const unsigned int fast =
         UNI ? __builtin_amdgcn_readfirstlane(st.fast[set[0]][threadIdx.x / 64]) : 0u;
-    for (int j0 = 0; j0 < cntp; j0 += kSub) {
+    for (int j0 = jb; j0 < je; j0 += kSub) {
         float p[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) p[k] = 0.0f;
@@ -500,9 +511,9 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, bool act
         if (active) {
             if constexpr (F32) {
                 if (uni)
-                    sum_staged_f32<K, true>(sh.s32, set, cntp, x, xf, acc);
+                    sum_staged_f32<K, true>(sh.s32, set, 0, cntp, x, xf, acc);
                 else
-                    sum_staged_f32<K, false>(sh.s32, set, cntp, x, xf, acc);
+                    sum_staged_f32<K, false>(sh.s32, set, 0, cntp, x, xf, acc);
             } else {
                 if (uni)
                     sum_staged_f64<K, true>(sh.s64, set, cntp, x, acc);
@@ -984,6 +995,8 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
         static_cast<double *>(A.alpha)[o] = tot;
 }
 
+#include "far.hpp"
+
 // ---- tangent kernel: alpha and dalpha[3][nl][nf] = d alpha / d(T, q_NH3, q_H2O) ----
 //
 // Work item = (layer, tile of 64 consecutive columns), one per wavefront; the
@@ -1390,11 +1403,56 @@ static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
     return RBX_OK;
 }
 
-static size_t alpha_ws_bytes(const AlphaArgs &A, int nsplit) {
+// FP32 far-line series (far.hpp): rbx_tune(RBX_TUNE_ALPHA_SERIES) 1 off, 2 on; built in on
+static bool series_on(int precision) {
+    return precision == RBX_FP32 && tune_get(RBX_TUNE_ALPHA_SERIES) != 1;
+}
+
+// near-mask streams: one bit per line of every staging round of each selected segment
+static void far_layout(AlphaArgs &A) {
+    const DevBlock *blk = A.cat.blk;
+    const bool hs = nh3_has_hs(A.abs.nh3);
+    const int n[kFarStreams] = {A.abs.h2o ? blk[RBX_LINES_H2O].n : 0,     A.abs.h2s ? blk[RBX_LINES_H2S].n : 0,
+                                hs ? blk[RBX_LINES_NH3_INV].n : 0,        hs ? blk[RBX_LINES_NH3_INV].n : 0,
+                                hs ? blk[RBX_LINES_NH3_ROT].n : 0,        hs ? blk[RBX_LINES_NH3_V2].n : 0,
+                                A.abs.ph3 ? blk[RBX_LINES_PH3].n : 0};
+    int at = 0;
+    for (int s = 0; s < kFarStreams; ++s) {
+        A.far_off[s] = at;
+        at += rounds_of(n[s]) * (kThreads / 32);
+    }
+    A.far_words = at;
+}
+
+// parts per layer of the classification pass: rbx_tune(RBX_TUNE_ALPHA_SPLIT), else enough
+// blocks (layers x tile groups x parts) to fill the device a few times over
+static int far_split(const AlphaArgs &A) {
+    const DevBlock *blk = A.cat.blk;
+    const bool hs = nh3_has_hs(A.abs.nh3);
+    const int R = (A.abs.h2o ? rounds_of(blk[RBX_LINES_H2O].n) : 0) +
+                  (A.abs.h2s ? rounds_of(blk[RBX_LINES_H2S].n) : 0) +
+                  (hs ? rounds_of(blk[RBX_LINES_NH3_INV].n) + rounds_of(blk[RBX_LINES_NH3_ROT].n) +
+                            rounds_of(blk[RBX_LINES_NH3_V2].n)
+                      : 0) +
+                  (A.abs.ph3 ? rounds_of(blk[RBX_LINES_PH3].n) : 0);
+    const int forced = tune_get(RBX_TUNE_ALPHA_SPLIT);
+    if (forced > 0) return std::min(forced, kMaxSplit);
+    const long long items = static_cast<long long>(A.nl) * ((A.tiles_total + kFarTiles - 1) / kFarTiles);
+    const long long want = (4096 + items - 1) / items;
+    return static_cast<int>(std::max(1LL, std::min<long long>(want, std::min(kMaxSplit, std::max(R, 1)))));
+}
+
+static size_t alpha_ws_bytes(const AlphaArgs &A, int nsplit, bool series) {
     const size_t plane = static_cast<size_t>(A.nl) * A.nf;
     const size_t extra = has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0;
-    return (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl + extra) *
-           sizeof(double);
+    size_t b = (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl + extra) *
+               sizeof(double);
+    if (series)   // (+16: the staged lines start 16-byte aligned)
+        b += 16 + static_cast<size_t>(A.nl) * A.far_words * 32 * (sizeof(double2) + sizeof(float2)) +
+             (static_cast<size_t>(A.nl) * A.far_split * A.tiles_total * kFarMom +
+              static_cast<size_t>(A.far_split) * A.nl + static_cast<size_t>(A.nl) * kSegs * 2) * sizeof(double) +
+             static_cast<size_t>(A.nl) * A.tiles_total * A.far_words * sizeof(unsigned int);
+    return b;
 }
 
 template <int K>
@@ -1420,13 +1478,45 @@ static int launch_alpha(AlphaArgs &A, int precision, int nsplit, hipStream_t st)
     return launch_status("alpha_finish_kernel");
 }
 
-// workspace of one rbx_alpha call: the tile width and split it will use
-static int plan_alpha(AlphaArgs &A, int precision, int &K, int &nsplit, size_t &bytes) {
+// the series path: classification pass, near sums per tile, then the same finish (one part)
+template <int K>
+static int launch_series(AlphaArgs &A, hipStream_t st) {
+    const long long groups = (A.tiles_total + kFarTiles - 1) / kFarTiles;
+    dim3 g1(static_cast<unsigned>(groups * A.nl), static_cast<unsigned>(A.far_split));
+    hipLaunchKernelGGL(alpha_far_kernel, g1, dim3(kThreads), 0, st, A);
+    int rc = launch_status("alpha_far_kernel");
+    if (rc) return rc;
+    dim3 g2(static_cast<unsigned>(static_cast<long long>(A.tiles_total) * A.nl));
+    hipLaunchKernelGGL((alpha_near_kernel<K>), g2, dim3(kThreads), 0, st, A);
+    rc = launch_status("alpha_near_kernel");
+    if (rc) return rc;
+    dim3 g3(static_cast<unsigned>((A.nf + kThreads - 1) / kThreads * static_cast<long long>(A.nl)));
+    if (has_extra(A.abs)) {
+        hipLaunchKernelGGL(alpha_extra_kernel, g3, dim3(kThreads), 0, st, A);
+        rc = launch_status("alpha_extra_kernel");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL((alpha_finish_kernel<true>), g3, dim3(kThreads), 0, st, A, 1);
+    return launch_status("alpha_finish_kernel");
+}
+
+// workspace of one rbx_alpha call: the tile width, split and path it will use
+static int plan_alpha(AlphaArgs &A, int precision, int &K, int &nsplit, bool &series, size_t &bytes) {
     K = alpha_tile_k(A.nf);
     set_wave_tiles(A, K);
-    int rc = alpha_split(A, K, precision, nsplit);
-    if (rc) return rc;
-    bytes = alpha_ws_bytes(A, nsplit);
+    series = series_on(precision);
+    if (series) {
+        far_layout(A);
+        series = A.far_words > 0;   // no line lists selected: the plain path (one part)
+    }
+    if (series) {
+        A.far_split = far_split(A);
+        nsplit = 1;
+    } else {
+        int rc = alpha_split(A, K, precision, nsplit);
+        if (rc) return rc;
+    }
+    bytes = alpha_ws_bytes(A, nsplit, series);
     return RBX_OK;
 }
 
@@ -1450,7 +1540,8 @@ extern "C" int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorb
     A.nf = nf;
     A.nf_low = nf_nh3_low;
     int K = 1, nsplit = 1;
-    return rbx::plan_alpha(A, precision, K, nsplit, *bytes);
+    bool series = false;
+    return rbx::plan_alpha(A, precision, K, nsplit, series, *bytes);
 }
 
 extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P, const double *T,
@@ -1464,8 +1555,9 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
                             by_constituent, "rbx_alpha");
     if (rc) return rc;
     int K = 1, nsplit = 1;
+    bool series = false;
     size_t need = 0;
-    rc = rbx::plan_alpha(A, precision, K, nsplit, need);
+    rc = rbx::plan_alpha(A, precision, K, nsplit, series, need);
     if (rc) return rc;
     RBX_CHECK_ARG(workspace, "rbx_alpha: null workspace (rbx_alpha_workspace_size gives its size)");
     RBX_CHECK_ARG(workspace_bytes >= need, "rbx_alpha: workspace of %zu bytes, this call needs %zu", workspace_bytes,
@@ -1475,6 +1567,23 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
     A.ws_base = A.ws + static_cast<size_t>(rbx::kSlots) * nsplit * nl * nf;
     A.ws_extra = A.ws_base + static_cast<size_t>(nsplit) * nl;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (series) {
+        const size_t plane = static_cast<size_t>(nl) * nf;
+        const size_t lines = static_cast<size_t>(nl) * A.far_words * 32;
+        const uintptr_t at = reinterpret_cast<uintptr_t>(
+            A.ws_extra + (rbx::has_extra(A.abs) ? static_cast<size_t>(rbx::kNumExtra) * plane : 0));
+        A.near_dh = reinterpret_cast<double2 *>((at + 15) / 16 * 16);
+        A.near_ac = reinterpret_cast<float2 *>(A.near_dh + lines);
+        A.far_mom = reinterpret_cast<double *>(A.near_ac + lines);
+        A.far_base = A.far_mom + static_cast<size_t>(nl) * A.far_split * A.tiles_total * rbx::kFarMom;
+        A.far_scale = A.far_base + static_cast<size_t>(A.far_split) * nl;
+        A.near = reinterpret_cast<unsigned int *>(A.far_scale + static_cast<size_t>(nl) * rbx::kSegs * 2);
+        switch (K) {
+            case 4: return rbx::launch_series<4>(A, st);
+            case 2: return rbx::launch_series<2>(A, st);
+            default: return rbx::launch_series<1>(A, st);
+        }
+    }
     switch (K) {
         case 4: return rbx::launch_alpha<4>(A, precision, nsplit, st);
         case 2: return rbx::launch_alpha<2>(A, precision, nsplit, st);

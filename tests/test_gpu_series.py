@@ -1,0 +1,113 @@
+"""FP32 far-line series (RBX_TUNE_ALPHA_SERIES): the per-tile near sums plus the series of the
+far lines' moments against the direct FP32 sums over every line and against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import layers
+from radiobear_amd import synth
+
+pytestmark = pytest.mark.gpu
+C = synth.GAS_ROWS
+JUPITER_ALPHA = {'h2': 'h2_jj_ddb', 'h2o': 'h2o_bk', 'h2s': 'h2s_ddb', 'nh3': 'nh3_hs_sjs', 'ph3': 'ph3_jh'}
+
+
+@pytest.fixture(scope='module')
+def K():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no HIP device')
+    from radiobear_amd import kernels
+    return kernels
+
+
+@pytest.fixture
+def at(K):
+    from radiobear_amd import autotune
+    yield autotune
+    autotune.reset()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def both_paths(K, at, cat, sel, atm, f, **kw):
+    out = {}
+    for mode in (1, 2):
+        at.set_knob('alpha_series', mode)
+        r = K.alpha(cat, K.absorbers_struct(sel), atm.P, atm.T, K.mix_table(atm.gas, C), f, 'fp32', **kw)
+        out[mode] = [t.double().cpu().numpy() for t in r] if isinstance(r, tuple) else r.double().cpu().numpy()
+    return out[1], out[2]
+
+
+@pytest.mark.parametrize('tile_k', [1, 2, 4])
+def test_series_matches_direct_and_oracle(K, at, raw_lines, jupiter_lines, tile_k):
+    """330 log-spaced frequencies 1-200 GHz (a tile straddles the 30 GHz switch, the last tile
+    is partial), 41 layers to 100 bar, 300 synthetic H2O lines: series within 1e-6 of the
+    direct FP32 sums at every tile width, both within 1e-5 of the FP64 oracle."""
+    atm = synth.jupiter_like(41)
+    h2o = synth.h2o_lines(300, 5)
+    cat = K.jupiter_catalog(raw_lines, h2o=h2o)
+    lines = dict(jupiter_lines)
+    lines['h2o'] = h2o
+    f = np.logspace(0.0, np.log10(200.0), 330)
+    at.set_knob('alpha_k', tile_k)
+    direct, series = both_paths(K, at, cat, JUPITER_ALPHA, atm, f)
+    ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(lines)).T
+    assert rel(series, direct) < 1e-6
+    assert rel(series, ref) < 1e-5
+    assert rel(direct, ref) < 1e-5
+
+
+def test_series_per_constituent_and_deep_layers(K, at, raw_lines, jupiter_lines):
+    """Layers to 3000 bar (nh3_hs blended into nh3_sjs, then off), every constituent's row."""
+    atm = synth.jupiter_like(37, pmax=3000.0)
+    h2o = synth.h2o_lines(700, 9)
+    cat = K.jupiter_catalog(raw_lines, h2o=h2o)
+    f = np.logspace(np.log10(0.5), np.log10(250.0), 190)
+    (da, dbyc), (sa, sbyc) = both_paths(K, at, cat, JUPITER_ALPHA, atm, f, by_constituent=True)
+    assert rel(sa, da) < 1e-6
+    for c in range(dbyc.shape[0]):
+        if np.any(dbyc[c] != 0.0):
+            assert rel(sbyc[c], dbyc[c]) < 1e-6, c
+
+
+def test_series_unordered_request_and_extreme_frequencies(K, at, raw_lines, jupiter_lines):
+    """A shuffled request (the NH3 columns are permuted, so tiles hold scattered x ranges)
+    with frequencies from 0.05 to 2000 GHz: lines fall below, above and across tiles."""
+    atm = synth.jupiter_like(23)
+    cat = K.jupiter_catalog(raw_lines)
+    rng = np.random.default_rng(3)
+    f = rng.permutation(np.concatenate([np.geomspace(0.05, 2000.0, 150), [22.2, 23.7, 30.0, 30.01]]))
+    direct, series = both_paths(K, at, cat, JUPITER_ALPHA, atm, f)
+    ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(jupiter_lines)).T
+    assert rel(series, direct) < 1e-6
+    assert rel(series, ref) < 1e-5
+
+
+@pytest.mark.parametrize('nf', [1, 63, 65])
+def test_series_small_requests(K, at, raw_lines, jupiter_lines, nf):
+    atm = synth.jupiter_like(9)
+    cat = K.jupiter_catalog(raw_lines)
+    f = np.linspace(1.5, 95.0, nf)
+    direct, series = both_paths(K, at, cat, JUPITER_ALPHA, atm, f)
+    assert rel(series, direct) < 1e-6
+
+
+def test_series_large_workload_sample(K, at):
+    """The 800-layer, 25753-line, 0.3-300 GHz workload's catalogue on 40 of its layers and
+    all 1024 frequencies: series within 1e-6 of the direct FP32 sums."""
+    from radiobear_amd import lines
+    w = synth.workload('c3_large')
+    atm = w['atm']
+    keep = np.arange(0, atm.nl, 20)
+    cat = K.jupiter_catalog(lines.raw_catalog(), nh3_inversion=w['nh3_inv'], h2o=w['h2o'])
+    at.set_knob('alpha_series', 1)
+    sel = K.absorbers_struct(JUPITER_ALPHA)
+    P, T, mix = atm.P[keep], atm.T[keep], K.mix_table(atm.gas[:, keep], C)
+    direct = K.alpha(cat, sel, P, T, mix, w['freqs'], 'fp32').double().cpu().numpy()
+    at.set_knob('alpha_series', 2)
+    series = K.alpha(cat, sel, P, T, mix, w['freqs'], 'fp32').double().cpu().numpy()
+    assert rel(series, direct) < 1e-6
