@@ -132,13 +132,41 @@ __device__ __forceinline__ void with_prep(const AlphaArgs &A, const LayerT<doubl
 
 // ---- pass 1: classification, moments, near masks ----
 
+constexpr int kFarPairs = kFarTiles + 1;   // (tile, coefficient set) pairs: one tile may need both sets
+
 struct FarShared {
-    struct {
-        double a[2][kThreads], c[2][kThreads], Ds[2][kThreads], P0[2][kThreads], rs[2][kThreads], rb[2][kThreads];
-    } st;
-    double mom[kFarTiles][2][kFarMom];   // per tile and thread-group half
+    union {
+        struct {   // one staging round, per coefficient set and line
+            double a[2][kThreads], c[2][kThreads], Ds[2][kThreads], P0[2][kThreads];
+            float2 r[2][kThreads];   // pole moduli (smaller, larger)
+        } st;
+        double red[kThreads][6];     // accumulator reduction, six terms at a time
+    };
+    double mom[kFarTiles][kFarMom];
     double lo[kFarTiles][kFarRanges], hi[kFarTiles][kFarRanges];
-    double red[kThreads / 64];
+    unsigned int maskw[kFarPairs][kThreads / 32];
+    double bsum[kThreads / 64];
+};
+
+// the (tile, set) pairs of a segment in a group of tiles, in order: every tile of the
+// group, with each inversion set its columns use (one set for the other segments)
+struct FarPairs {
+    int n;
+    int j[kFarPairs], set[kFarPairs];
+    __device__ FarPairs(const AlphaArgs &A, int g, int ntiles, bool inv) {
+        n = 0;
+        for (int jj = 0; jj < ntiles; ++jj) {
+            int c0, c1;
+            tile_cols(A, g * kFarTiles + jj, c0, c1);
+            const int ts = inv ? tile_sets(A, c0, c1) : 1;
+            for (int v = 0; v < 2; ++v)
+                if ((ts >> v) & 1) {
+                    j[n] = jj;
+                    set[n] = v;
+                    ++n;
+                }
+        }
+    }
 };
 
 __global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
@@ -160,17 +188,12 @@ __global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
             }
         }
     }
-    for (int e = tid; e < kFarTiles * 2 * kFarMom; e += kThreads) (&sh.mom[0][0][0])[e] = 0.0;
-
-    // this thread's (tile, half) group and lane within it
-    const int grp = tid >> 4, j = grp >> 1, v = grp & 1, s16 = tid & 15;
-    const int tile = g * kFarTiles + j;
-    int c0 = 0, c1 = 0, gc0 = 0, gc1 = 0;
-    if (tile < A.tiles_total) tile_cols(A, tile, c0, c1);
+    for (int e = tid; e < kFarTiles * kFarMom; e += kThreads) (&sh.mom[0][0])[e] = 0.0;
+    int gc0, gc1;
     tile_cols(A, g * kFarTiles, gc0, gc1);
     gc1 = min(A.nf, (g + 1) * kFarTiles * A.tile_width);
-    const int tsets = tile < A.tiles_total ? tile_sets(A, c0, c1) : 0;
     const int gsets = tile_sets(A, gc0, gc1);   // inversion sets any tile of the group needs
+    const int ntiles = min(kFarTiles, A.tiles_total - g * kFarTiles);
     const double q = kFarRatio;
     __syncthreads();
 
@@ -186,15 +209,15 @@ __global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
         const DevBlock &B = A.cat.blk[kSegKind[seg]];
         const bool inv = seg == 2;
         const int need = inv ? gsets : 1;
-        // both inversion sets in one group: half v of a tile's threads takes set v;
-        // otherwise the two halves split the round's lines
-        const bool by_set = inv && need == 3;
-        const int myset = by_set ? v : (inv ? (need == 2 ? 1 : 0) : 0);
+        // thread groups of tpp consecutive threads, one per (tile, set) pair; each takes
+        // every tpp-th line of a round
+        const FarPairs pairs(A, g, ntiles, inv);
+        const int tpp = kThreads / pairs.n;
+        const int myp = tid / tpp, sidx = tid % tpp;
+        const bool active = myp < pairs.n;
+        const int myj = active ? pairs.j[myp] : 0, myset = active ? pairs.set[myp] : 0;
         const int rv = kSegVar[seg] == 0 ? 0 : (inv ? 2 + myset : 1);
-        const bool mine = inv ? ((tsets >> myset) & 1) != 0 : tile < A.tiles_total;
-        const double lo = sh.lo[j][rv], hi = sh.hi[j][rv];
-        unsigned int *mask = A.near + (static_cast<size_t>(layer) * A.tiles_total + tile) * A.far_words +
-                             A.far_off[kSegStream[seg] + (inv ? myset : 0)];
+        const float lo_q = static_cast<float>(q * sh.lo[myj][rv]), hi_q = static_cast<float>(sh.hi[myj][rv] / q);
         double accB[kFarTerms], accA[kFarTerms];
 #pragma unroll
         for (int k = 0; k < kFarTerms; ++k) accB[k] = accA[k] = 0.0;
@@ -229,21 +252,19 @@ __global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
                         sh.st.c[s][tid] = qs[s].c;
                         sh.st.Ds[s][tid] = Ds;
                         sh.st.P0[s][tid] = fma(Ds, Ds, h);
-                        sh.st.rs[s][tid] = rs;
-                        sh.st.rb[s][tid] = rb;
+                        sh.st.r[s][tid] = make_float2(static_cast<float>(rs), static_cast<float>(rb));
                     }
+                    if (tid < kFarPairs * (kThreads / 32)) (&sh.maskw[0][0])[tid] = 0u;
                 }
                 __syncthreads();
-                // phase 2: group (tile j, half v) classifies 16 consecutive lines per step
-                const int steps = by_set ? kThreads / 16 : kThreads / 32;
-                unsigned int lo16 = 0u;
-                for (int st = 0; st < steps; ++st) {
-                    const int i = by_set ? st * 16 + s16 : st * 32 + v * 16 + s16;
-                    bool near = false;
-                    if (mine && i < cnt) {
-                        const double a = sh.st.a[myset][i], c = sh.st.c[myset][i];
-                        const double Ds2 = 2.0 * sh.st.Ds[myset][i], P0 = sh.st.P0[myset][i];
-                        if (sh.st.rb[myset][i] <= q * lo) {
+                // phase 2: classify against the pair's tile; far lines add their series
+                // coefficients, near lines set their bit
+                if (active) {
+                    for (int i = sidx; i < cnt; i += tpp) {
+                        const float2 rr = sh.st.r[myset][i];
+                        if (rr.y <= lo_q) {
+                            const double a = sh.st.a[myset][i], c = sh.st.c[myset][i];
+                            const double Ds2 = 2.0 * sh.st.Ds[myset][i], P0 = sh.st.P0[myset][i];
                             double m0 = a, m1 = fma(Ds2, a, c);
                             accB[0] += m0;
                             accB[1] += m1;
@@ -254,8 +275,9 @@ __global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
                                 m0 = m1;
                                 m1 = m2;
                             }
-                        } else if (q * sh.st.rs[myset][i] >= hi) {
-                            const double iP = 1.0 / P0;
+                        } else if (rr.x >= hi_q) {
+                            const double a = sh.st.a[myset][i], c = sh.st.c[myset][i];
+                            const double Ds2 = 2.0 * sh.st.Ds[myset][i], iP = 1.0 / sh.st.P0[myset][i];
                             double n0 = c * iP, n1 = fma(Ds2, n0, a) * iP;
                             accA[0] += n0;
                             accA[1] += n1;
@@ -266,41 +288,38 @@ __global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
                                 n0 = n1;
                                 n1 = n2;
                             }
-                        } else {
-                            near = true;   // also NaN coefficients: summed directly
+                        } else {   // near (also NaN coefficients: summed directly)
+                            atomicOr(&sh.maskw[myp][i >> 5], 1u << (i & 31));
                         }
-                    }
-                    const unsigned long long m = __ballot(near);
-                    if (by_set) {   // this group's 16 bits; words of 32 lines every other step
-                        const unsigned int chunk = static_cast<unsigned int>(m >> (16 * ((tid & 63) >> 4))) & 0xFFFFu;
-                        if (st & 1) {
-                            if (mine && s16 == 0) mask[l0 / 32 + st / 2] = lo16 | (chunk << 16);
-                        } else {
-                            lo16 = chunk;
-                        }
-                    } else if (mine && s16 == 0 && v == 0) {   // halves 0 and 1: bits 0-15 and 16-31
-                        mask[l0 / 32 + st] = static_cast<unsigned int>(m >> (32 * (j & 1)));
                     }
                 }
                 __syncthreads();
+                if (tid < pairs.n * (kThreads / 32)) {   // the round's 8 mask words of each pair
+                    const int p = tid / (kThreads / 32), w = tid % (kThreads / 32);
+                    const int stream = kSegStream[seg] + (inv ? pairs.set[p] : 0);
+                    A.near[(static_cast<size_t>(layer) * A.tiles_total + g * kFarTiles + pairs.j[p]) * A.far_words +
+                           A.far_off[stream] + l0 / 32 + w] = sh.maskw[p][w];
+                }
             }
         });
-        // the segment's moments: reduce over the group's 16 lanes, add into the tile's set
+        // the segment's moments: each pair's threads' accumulators summed through LDS
+        // (the staging area), six terms at a time, into the pair's tile and set
 #pragma unroll
-        for (int k = 0; k < kFarTerms; ++k) {
+        for (int part6 = 0; part6 < 2 * kFarTerms / 6; ++part6) {
 #pragma unroll
-            for (int mm = 8; mm > 0; mm >>= 1) {
-                accB[k] += __shfl_xor(accB[k], mm);
-                accA[k] += __shfl_xor(accA[k], mm);
+            for (int e = 0; e < 6; ++e) {
+                const int t = part6 * 6 + e;
+                sh.red[tid][e] = t < kFarTerms ? accB[t] : accA[t - kFarTerms];
             }
-        }
-        if (s16 == 0 && mine) {
-            double *mo = sh.mom[j][v] + (kSegFarSet[seg] + (inv ? myset : 0)) * 2 * kFarTerms;
-#pragma unroll
-            for (int k = 0; k < kFarTerms; ++k) {
-                mo[k] += accB[k];
-                mo[kFarTerms + k] += accA[k];
+            __syncthreads();
+            if (tid < pairs.n * 6) {
+                const int p = tid / 6, e = tid % 6;
+                double sum = 0.0;
+                for (int k = 0; k < tpp; ++k) sum += sh.red[p * tpp + k][e];
+                const int ss = kSegFarSet[seg] + (inv ? pairs.set[p] : 0);
+                sh.mom[pairs.j[p]][ss * 2 * kFarTerms + part6 * 6 + e] += sum;
             }
+            __syncthreads();
         }
     }
     __syncthreads();
@@ -308,16 +327,15 @@ __global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
         const int jj = e / kFarMom, r = e % kFarMom;
         const int t = g * kFarTiles + jj;
         if (t < A.tiles_total)
-            A.far_mom[((static_cast<size_t>(layer) * nsplit + part) * A.tiles_total + t) * kFarMom + r] =
-                sh.mom[jj][0][r] + sh.mom[jj][1][r];
+            A.far_mom[((static_cast<size_t>(layer) * nsplit + part) * A.tiles_total + t) * kFarMom + r] = sh.mom[jj][r];
     }
     // the h2o base term of this part's lines (every group derives them; group 0 writes)
     for (int m = 32; m > 0; m >>= 1) base += __shfl_xor(base, m);
-    if (lane == 0) sh.red[wave] = base;
+    if (lane == 0) sh.bsum[wave] = base;
     __syncthreads();
     if (tid == 0 && g == 0) {
         double b = 0.0;
-        for (int w = 0; w < kThreads / 64; ++w) b += sh.red[w];
+        for (int w = 0; w < kThreads / 64; ++w) b += sh.bsum[w];
         A.far_base[static_cast<size_t>(part) * A.nl + layer] = b;
     }
 }
