@@ -418,7 +418,9 @@ def main():
         achieved = flop / (kt32[dom] * 1e-3) / 1e12
         roof = {'kernel': dom, 'bound': 'valu', 'achieved': achieved, 'peak': PEAK['fp32'], 'unit': 'TFLOP/s',
                 'frac': achieved / PEAK['fp32'], 'traffic': None,
-                'algorithmic_flop_per_launch': flop}
+                'algorithmic_flop_per_launch': flop,
+                'basis': 'FLOP of the direct sum over every (line, layer, frequency); the far-line series path '
+                         'sums about a third of those pairs and adds the rest as per-tile power series'}
     else:
         steps = float(np.sum(np.maximum(ns - 1, 0)))
         flop = steps * prob.nf * FLOP_RT_STEP

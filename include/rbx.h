@@ -164,15 +164,19 @@ int rbx_device_count(int *count);
                                     built in: ray major for tables up to 3.5 MiB */
 #define RBX_TUNE_ALPHA_SERIES 4  /* FP32 rbx_alpha: 1 sums every line at every frequency; 2 sums
                                     per wavefront tile only the lines whose poles lie within a
-                                    factor 4 of the tile's frequencies and adds the others as 12-term
-                                    power series (1e-7 of their magnitude); built in: 2 */
+                                    factor 4 of the tile's x = f^2 range and adds the others as
+                                    12-term power series (at most 2e-6 of a far line's value, tables
+                                    within 3e-7 of FP64 as the direct sums); built in: 2 (needs a
+                                    workspace of ~24 B per line and layer more) */
 #define RBX_NUM_TUNE_KEYS    5
 int rbx_tune(int key, int value);
 
 /* ---- line catalogue (device-resident, owned by the library) ---- */
 int rbx_catalog_create(rbx_catalog **out);
 /* Replace block `kind` with n lines; host_columns[i] is a HOST array of n
- * doubles for column i of that kind (see RBX_LINES_*).  n = 0 clears it. */
+ * doubles for column i of that kind (see RBX_LINES_*).  n = 0 clears it.  The
+ * line lists the forward sums stage (NH3 inversion, rotational, v2, H2S, PH3,
+ * H2O) are stored sorted by centre frequency (column 0, stable). */
 int rbx_catalog_set_lines(rbx_catalog *cat, int kind, int n, const double *const *host_columns);
 int rbx_catalog_lines(const rbx_catalog *cat, int kind, int *n);
 int rbx_catalog_destroy(rbx_catalog *cat);

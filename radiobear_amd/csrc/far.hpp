@@ -10,8 +10,11 @@
 //   * every |p| >= hi / kFarRatio ("above"): term = sum_k n_k x^k, with
 //     n_0 = c / P0, n_1 = (a + 2 Ds n_0) / P0, n_k = (2 Ds n_{k-1} - n_{k-2}) / P0
 // (the recurrences match powers of x in term * denominator = a x + c).  kFarTerms = 12
-// terms leave at most kFarRatio^12 / (1 - kFarRatio) ~ 8e-8 of each far line's
-// magnitude; the FP32 line sums themselves carry ~3e-7.  On 800 layers x 1024
+// terms leave ~8e-8 of a far line's value for two well-separated poles and at most
+// 2.2e-6 at the ratio limit for a near-double pole of a Gross line (a = 0: 11 terms
+// carry it, with coefficients growing like (k + 1) p^k); far lines are small next to
+// the near ones, and the tables stay within 3e-7 of FP64 like the direct FP32 sums.
+// On 800 layers x 1024
 // frequencies (0.3-300 GHz, 128-column tiles) about 30 % of the (line, tile) pairs
 // are neither: only those are summed per column.
 //
