@@ -161,17 +161,16 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
 }
 
 // ---- FP32 mode, point rays: FP32 weights and trapezoid sums ----
-// tau stays an FP64 prefix sum (its increments (a0 + a1) ds/2 are FP32), e^-tau
-// keeps the FP64 range reduction, and W, T W and the two trapezoid sums are FP32,
-// added into FP64 totals after every batch of U steps (blocked summation).  Each
-// wavefront first copies its ray's half steps and layer temperatures to LDS as
-// FP32 pairs (h_i, T_{i+1}), so a step reads them with one broadcast LDS load.
+// tau stays an FP64 prefix sum (its increments (a0 + a1) ds/2 are FP32), and W, T W
+// and the two trapezoid sums are FP32, added into FP64 totals after every batch of U
+// steps (blocked summation).  e^-tau is the hardware exp2 of t = -tau log2(e) rounded
+// to FP32: the rounding moves the exponent by at most |t| 2^-24, so a weight moves by
+// at most tau e^-tau 2^-24 ~ 2e-8 of its alpha (largest at tau = 1), below the FP32
+// sums' own error; past t = -150 the result is 0.  Each wavefront first copies its
+// ray's half steps and layer temperatures to LDS as FP32 pairs (h_i, T_{i+1}), so a
+// step reads them with one broadcast LDS load.
 __device__ __forceinline__ float exp_neg_f32(double tau) {
-    const double t = -tau * kLog2e;
-    double n = __builtin_rint(t);
-    n = n < -160.0 ? -160.0 : n;   // e^-tau < 2^-150 is 0 in FP32
-    const float r = static_cast<float>(t - n);
-    return __builtin_ldexpf(__builtin_amdgcn_exp2f(r), static_cast<int>(n));
+    return __builtin_amdgcn_exp2f(static_cast<float>(-tau * kLog2e));
 }
 
 template <int U>
