@@ -166,9 +166,13 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
 // steps (blocked summation).  e^-tau is the hardware exp2 of t = -tau log2(e) rounded
 // to FP32: the rounding moves the exponent by at most |t| 2^-24, so a weight moves by
 // at most tau e^-tau 2^-24 ~ 2e-8 of its alpha (largest at tau = 1), below the FP32
-// sums' own error; past t = -150 the result is 0.  Each wavefront first copies its
+// sums' own error; past t = -150 the result is 0.  Within a batch of U steps the tau
+// increments add up in FP32 on the batch's FP64 starting tau (U roundings of a sum of U
+// terms, < 1e-6 relative to the batch's own increment).  Each wavefront first copies its
 // ray's half steps and layer temperatures to LDS as FP32 pairs (h_i, T_{i+1}), so a
 // step reads them with one broadcast LDS load.
+constexpr float kLog2ef = 1.44269504f;
+
 __device__ __forceinline__ float exp_neg_f32(double tau) {
     return __builtin_amdgcn_exp2f(static_cast<float>(-tau * kLog2e));
 }
@@ -206,11 +210,15 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
 #pragma unroll
             for (int u = 0; u < U; ++u) ab[u] = al[(i + u + 1) * nf];
             float iW = 0.0f, nm = 0.0f;
+            // the batch's tau increments add up in FP32 (dt) on the FP64 tau at its start,
+            // and the exponent is t0 - dt log2(e) with t0 = -tau log2(e) rounded once
+            const float t0 = static_cast<float>(-tau * kLog2e);
+            float dt = 0.0f;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const float2 ht = hT[i + u];
-                tau += static_cast<double>((a0 + ab[u]) * ht.x);
-                const float W1 = ab[u] * exp_neg_f32(tau);
+                dt = fmaf(a0 + ab[u], ht.x, dt);
+                const float W1 = ab[u] * __builtin_amdgcn_exp2f(fmaf(dt, -kLog2ef, t0));
                 iW = fmaf(W1 + W0, ht.x, iW);
                 const float TW1 = ht.y * W1;
                 nm = fmaf(TW1 + TW0, ht.x, nm);
@@ -218,6 +226,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
                 W0 = W1;
                 TW0 = TW1;
             }
+            tau += static_cast<double>(dt);
             intW += static_cast<double>(iW);
             num += static_cast<double>(nm);
             if (A.tau_cut > 0.0 && __all(!ok || tau > A.tau_cut)) {
