@@ -129,7 +129,7 @@ class Problem:
         K = self.K
         a = self.alpha32 if prec == _lib.FP32 else self.alpha64
         Tb = self.Tb if prec == _lib.FP32 else self.Tb64
-        tau_cut = 50.0 if prec == _lib.FP32 else 0.0
+        tau_cut = K.FP32_TAU_CUT if prec == _lib.FP32 else 0.0
         _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
                                             self.ld, K._ptr(self.ns), self.nray, 0, prec, tau_cut, K._ptr(Tb), None,
                                             K._stream()))

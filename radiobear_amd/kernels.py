@@ -299,6 +299,12 @@ def raypaths(planet, R, N, b, units='half_cm'):
     return ds, ns
 
 
+# FP32-mode optical-depth cut: a ray's integration stops once tau > 30 at every frequency of a
+# wavefront (batch granularity); what is left weighs < e^-30 ~ 1e-13 of the sums, below FP32
+# resolution (on 800 layers x 1024 frequencies it moves Tb by 4e-14 K and saves 17 %)
+FP32_TAU_CUT = 30.0
+
+
 def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False):
     """Tb (nray, nf) float64 from an alpha table (nl, nf) and ray paths."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
