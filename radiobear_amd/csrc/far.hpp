@@ -172,7 +172,9 @@ struct FarPairs {
     }
 };
 
-__global__ __launch_bounds__(kThreads, 2) void alpha_far_kernel(AlphaArgs A) {
+// three blocks per CU (168 VGPRs, a few spills) run the pass in 1.00 ms against 1.18 ms with two
+// blocks and no spills; four (128 VGPRs) gain 3 % more for twice the spill traffic
+__global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
     __shared__ FarShared sh;
     const int ngroups = (A.tiles_total + kFarTiles - 1) / kFarTiles;
     const int layer = blockIdx.x / ngroups, g = blockIdx.x % ngroups;
