@@ -164,8 +164,8 @@ int rbx_device_count(int *count);
                                     built in: ray major for tables up to 3.5 MiB */
 #define RBX_TUNE_ALPHA_SERIES 4  /* FP32 rbx_alpha: 1 sums every line at every frequency; 2 sums
                                     per wavefront tile only the lines whose poles lie within a
-                                    factor 4 of the tile's x = f^2 range and adds the others as
-                                    12-term power series (at most 2e-6 of a far line's value, tables
+                                    factor 3 of the tile's x = f^2 range and adds the others as
+                                    16-term power series (at most 8e-7 of a far line's value, tables
                                     within 3e-7 of FP64 as the direct sums); built in: 2 (needs a
                                     workspace of ~24 B per line and layer more) */
 #define RBX_NUM_TUNE_KEYS    5

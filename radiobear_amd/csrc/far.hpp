@@ -9,14 +9,14 @@
 //     m_0 = a, m_1 = c + 2 Ds a, m_k = 2 Ds m_{k-1} - P0 m_{k-2};
 //   * every |p| >= hi / kFarRatio ("above"): term = sum_k n_k x^k, with
 //     n_0 = c / P0, n_1 = (a + 2 Ds n_0) / P0, n_k = (2 Ds n_{k-1} - n_{k-2}) / P0
-// (the recurrences match powers of x in term * denominator = a x + c).  kFarTerms = 12
-// terms leave ~8e-8 of a far line's value for two well-separated poles and at most
-// 2.2e-6 at the ratio limit for a near-double pole of a Gross line (a = 0: 11 terms
-// carry it, with coefficients growing like (k + 1) p^k); far lines are small next to
-// the near ones, and the tables stay within 3e-7 of FP64 like the direct FP32 sums.
-// On 800 layers x 1024
-// frequencies (0.3-300 GHz, 128-column tiles) about 30 % of the (line, tile) pairs
-// are neither: only those are summed per column.
+// (the recurrences match powers of x in term * denominator = a x + c).  kFarTerms = 16
+// terms at kFarRatio = 1/3 leave ~4e-8 of a far line's value for two well-separated
+// poles and at most 8e-7 at the ratio limit for a near-double pole of a Gross line
+// (a = 0: 15 terms carry it, with coefficients growing like (k + 1) p^k); far lines are
+// small next to the near ones, and the tables stay within 3e-7 of FP64 like the direct
+// FP32 sums.  On 800 layers x 1024 frequencies (0.3-300 GHz, 128-column tiles) about
+// 30 % of the (line, tile) pairs are neither: only those are summed per column.  (12
+// terms at 1/4 leave more near pairs, 33 %, and were 2 % slower overall.)
 //
 // alpha_far_kernel (block = layer x group of kFarTiles tiles x part of the layer's
 // staging rounds): each thread derives one line's coefficients per round (once per
@@ -31,11 +31,13 @@
 // wavefronts in order and the series of the summed moments, and writes the same
 // per-slot partials the finish kernel reads (one part).
 
-constexpr int kFarTerms = 12;
-constexpr double kFarRatio = 0.25;
+constexpr int kFarTerms = 16;
+constexpr double kFarRatio = 1.0 / 3.0;
 constexpr int kFarTiles = 8;                          // tiles per alpha_far_kernel block
 constexpr int kFarSets = 6;                           // h2o, h2s, inversion set 0, set 1, rot + v2, ph3
 constexpr int kFarMom = kFarSets * 2 * kFarTerms;     // moments per (layer, part, tile)
+constexpr int kRedChunk = 8;                          // accumulator terms reduced per LDS pass
+static_assert((2 * kFarTerms) % kRedChunk == 0, "series terms must fill whole reduction passes");
 constexpr int kSegStream[kSegs] = {0, 1, 2, 4, 5, 6};
 constexpr int kSegFarSet[kSegs] = {0, 1, 2, 4, 4, 5};
 constexpr int kSegVar[kSegs] = {0, 0, 1, 1, 1, 0};    // columns' x (0) or x for NH3 (1)
@@ -143,7 +145,7 @@ struct FarShared {
             double a[2][kThreads], c[2][kThreads], Ds[2][kThreads], P0[2][kThreads];
             float2 r[2][kThreads];   // pole moduli (smaller, larger)
         } st;
-        double red[kThreads][6];     // accumulator reduction, six terms at a time
+        double red[kThreads][kRedChunk];   // accumulator reduction, kRedChunk terms at a time
     };
     double mom[kFarTiles][kFarMom];
     double lo[kFarTiles][kFarRanges], hi[kFarTiles][kFarRanges];
@@ -308,21 +310,21 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
             }
         });
         // the segment's moments: each pair's threads' accumulators summed through LDS
-        // (the staging area), six terms at a time, into the pair's tile and set
+        // (the staging area), kRedChunk terms at a time, into the pair's tile and set
 #pragma unroll
-        for (int part6 = 0; part6 < 2 * kFarTerms / 6; ++part6) {
+        for (int part = 0; part < 2 * kFarTerms / kRedChunk; ++part) {
 #pragma unroll
-            for (int e = 0; e < 6; ++e) {
-                const int t = part6 * 6 + e;
+            for (int e = 0; e < kRedChunk; ++e) {
+                const int t = part * kRedChunk + e;
                 sh.red[tid][e] = t < kFarTerms ? accB[t] : accA[t - kFarTerms];
             }
             __syncthreads();
-            if (tid < pairs.n * 6) {
-                const int p = tid / 6, e = tid % 6;
+            if (tid < pairs.n * kRedChunk) {
+                const int p = tid / kRedChunk, e = tid % kRedChunk;
                 double sum = 0.0;
                 for (int k = 0; k < tpp; ++k) sum += sh.red[p * tpp + k][e];
                 const int ss = kSegFarSet[seg] + (inv ? pairs.set[p] : 0);
-                sh.mom[pairs.j[p]][ss * 2 * kFarTerms + part6 * 6 + e] += sum;
+                sh.mom[pairs.j[p]][ss * 2 * kFarTerms + part * kRedChunk + e] += sum;
             }
             __syncthreads();
         }

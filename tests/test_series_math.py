@@ -1,9 +1,9 @@
 """The far-line series the FP32 absorption path adds for lines away from a tile
 (radiobear_amd/csrc/far.hpp): for term(x) = (a x + c) / (x^2 - 2 Ds x + P0),
 
-    below (|poles| <= x/4):  term = sum_k m_k x^-(k+1),  m_0 = a, m_1 = c + 2 Ds a,
+    below (|poles| <= x/3):  term = sum_k m_k x^-(k+1),  m_0 = a, m_1 = c + 2 Ds a,
                              m_k = 2 Ds m_{k-1} - P0 m_{k-2}
-    above (|poles| >= 4 x):  term = sum_k n_k x^k,       n_0 = c / P0, n_1 = (a + 2 Ds n_0) / P0,
+    above (|poles| >= 3 x):  term = sum_k n_k x^k,       n_0 = c / P0, n_1 = (a + 2 Ds n_0) / P0,
                              n_k = (2 Ds n_{k-1} - n_{k-2}) / P0
 
 Pinned against the partial-fraction closed form (term = sum_i r_i / (x - p_i) with the poles
@@ -13,8 +13,8 @@ random lines of every canonical form: Ben-Reuven (G = 0, complex or real poles),
 and Van Vleck-Weisskopf (G > 0)."""
 import numpy as np
 
-TERMS = 12
-RATIO = 0.25
+TERMS = 16        # the kernels' kFarTerms and kFarRatio
+RATIO = 1.0 / 3.0
 
 
 def below(a, c, Ds, P0):
@@ -88,13 +88,13 @@ def test_recurrences_match_partial_fractions():
 
 
 def test_series_sums_match_direct_evaluation():
-    """Twelve terms at the kernels' ratio.  The tail is sum_i r_i p_i^12 / (x^12 (x - p_i)); for
-    two close poles (narrow lines, or real poles near a double root) the residues cancel and the
-    combined coefficients grow like (k + 1) p^k, and a Gross line (a = 0, so m_0 = 0) keeps only
-    11 terms: the tail is then at most sum_{k>=11} (k + 1) 0.25^k = 3.9e-6 of c / x^2, i.e.
-    2.2e-6 of the line's own value at the ratio limit (8e-8 for two well-separated poles)."""
+    """Sixteen terms at the kernels' ratio 1/3.  The tail is sum_i r_i p_i^16 / (x^16 (x - p_i));
+    for two close poles (narrow lines, or real poles near a double root) the residues cancel and
+    the combined coefficients grow like (k + 1) p^k, and a Gross line (a = 0, so m_0 = 0) keeps
+    only 15 terms: the tail is then at most sum_{k>=15} (k + 1) (1/3)^k = 1.7e-6 of c / x^2, i.e.
+    7.7e-7 of the line's own value at the ratio limit (4e-8 for two well-separated poles)."""
     rng = np.random.default_rng(11)
-    bound = RATIO ** (TERMS - 1) * (TERMS / (1 - RATIO) + RATIO / (1 - RATIO) ** 2)   # 3.9e-6
+    bound = RATIO ** (TERMS - 1) * (TERMS / (1 - RATIO) + RATIO / (1 - RATIO) ** 2)   # 1.7e-6
     for a, c, D, g, G in random_lines(rng, 300):
         Ds, h, P0 = shifted(D, g, G)
         p = np.abs(np.roots([1.0, -2.0 * Ds, P0]))
