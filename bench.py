@@ -129,7 +129,7 @@ class Problem:
         K = self.K
         a = self.alpha32 if prec == _lib.FP32 else self.alpha64
         Tb = self.Tb if prec == _lib.FP32 else self.Tb64
-        tau_cut = K.FP32_TAU_CUT if prec == _lib.FP32 else 0.0
+        tau_cut = K.FP32_TAU_CUT if prec == _lib.FP32 else K.FP64_TAU_CUT
         _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
                                             self.ld, K._ptr(self.ns), self.nray, 0, prec, tau_cut, K._ptr(Tb), None,
                                             K._stream()))
@@ -388,7 +388,11 @@ def main():
                              'fp32_vs_fp64_max_abs_dTb_K': float(np.max(np.abs(Tb32[ok] - Tb64[ok]))),
                              'fp32_vs_fp64_max_rel_dalpha': float(np.max(np.abs(a32 - a64) / np.abs(a64))),
                              'nan_rays': int(np.sum(np.isnan(Tb64).any(axis=1)))}
-        # the tau cut (e^-50) changes Tb below FP64 resolution: show it costs nothing in accuracy
+        # the tau cuts (FP64 e^-50, FP32 e^-30) change Tb below each mode's resolution: show it
+        prob.launch_integrate_nocut(FP64)
+        torch.cuda.synchronize()
+        d64 = np.abs(prob.Tb.cpu().numpy()[ok] - Tb64[ok])
+        res['fp64_plain']['tau_cut_max_abs_dTb_K'] = float(np.max(d64))
         autotune(prob, FP32, args)
         prob.launch_integrate(FP32)
         cut = prob.Tb.clone()

@@ -36,7 +36,7 @@ class Brightness:
 
     def _modes(self, alpha):
         prec = self.precision or ('fp32' if alpha.table.dtype == torch.float32 else 'fp64')
-        tau_cut = self.tau_cut if self.tau_cut is not None else (kernels.FP32_TAU_CUT if prec == 'fp32' else 0.0)
+        tau_cut = self.tau_cut if self.tau_cut is not None else (kernels.FP32_TAU_CUT if prec == 'fp32' else kernels.FP64_TAU_CUT)
         return prec, tau_cut
 
     def start_paths(self, bs, atm, orientation=None):

@@ -79,6 +79,17 @@ __device__ inline bool ray_span(const IntArgs &A, int ray, int &base, int &nl) {
     return base >= 0 && end <= A.nl && nl >= 2 && nl - 1 <= A.ld;
 }
 
+// The reference carries a NaN half step or temperature through tau and the sums to a NaN Tb
+// (a ray that misses a shell gets a NaN ds, raypath.py:190-209): an optical-depth cut must not
+// skip one, so the cut is off for a ray whose path (steps [0, n)) holds one.  Frequency
+// independent; one pass of the wavefront over the ray's rows.
+__device__ inline bool path_has_nan(const double *h, const double *Tl, int n) {
+    bool bad = false;
+    for (int k = static_cast<int>(threadIdx.x % kLanes); k < n; k += kLanes)
+        bad = bad || __builtin_isnan(h[k]) || __builtin_isnan(Tl[k + 1]);
+    return __any(bad);
+}
+
 __device__ inline double load_alpha(const void *p, int f32, size_t i) {
     return f32 ? static_cast<double>(static_cast<const float *>(p)[i]) : static_cast<const double *>(p)[i];
 }
@@ -126,6 +137,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     Step<MODE, FAST> st;
     if (S > 1) {
         const int n = S - 1;
+        const double cut = (A.tau_cut > 0.0 && !path_has_nan(h, Tl, n)) ? A.tau_cut : 0.0;
         st.a0 = static_cast<double>(al[0]);
         int i = 0;
         for (; i + U <= n; i += U) {
@@ -139,7 +151,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) st(static_cast<double>(ab[u]), hb[u], Tb[u]);
-            if (A.tau_cut > 0.0 && __all(!ok || st.tau > A.tau_cut)) {
+            if (cut > 0.0 && __all(!ok || st.tau > cut)) {
                 i = n;
                 break;
             }
@@ -200,8 +212,12 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
     double tau = 0.0, intW = 0.0, num = 0.0;
     if (S > 1) {
         const int n = S - 1;
-        for (int k = lane; k < n; k += kLanes)
+        bool bad = false;   // a NaN half step or temperature: no cut (path_has_nan)
+        for (int k = lane; k < n; k += kLanes) {
             hT[k] = make_float2(static_cast<float>(h[k]), static_cast<float>(Tl[k + 1]));
+            bad = bad || __builtin_isnan(h[k]) || __builtin_isnan(Tl[k + 1]);
+        }
+        const double cut = (A.tau_cut > 0.0 && !__any(bad)) ? A.tau_cut : 0.0;
         wave_sync();
         float a0 = al[0], W0 = 0.0f, TW0 = 0.0f;
         int i = 0;
@@ -229,7 +245,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
             tau += static_cast<double>(dt);
             intW += static_cast<double>(iW);
             num += static_cast<double>(nm);
-            if (A.tau_cut > 0.0 && __all(!ok || tau > A.tau_cut)) {
+            if (cut > 0.0 && __all(!ok || tau > cut)) {
                 i = n;
                 break;
             }
@@ -302,6 +318,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
     double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
     int last = 0;   // last path point reached
     if (S > 1) {
+        const double cut = (A.tau_cut > 0.0 && !path_has_nan(h, Tl, S - 1)) ? A.tau_cut : 0.0;
         double a0 = load_alpha(alpha, A.f32, jj);
         for (int i = 0; i < S - 1; ++i) {
             const double hh = h[i];
@@ -315,8 +332,8 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
             W0 = W1;
             TW0 = TW1;
             last = i + 1;
-            if (A.tau_cut > 0.0 && (i & 7) == 7) {
-                if (__all(!ok || tau > A.tau_cut)) break;
+            if (cut > 0.0 && (i & 7) == 7) {
+                if (__all(!ok || tau > cut)) break;
             }
         }
     }

@@ -303,6 +303,8 @@ def raypaths(planet, R, N, b, units='half_cm'):
 # wavefront (batch granularity); what is left weighs < e^-30 ~ 1e-13 of the sums, below FP32
 # resolution (on 800 layers x 1024 frequencies it moves Tb by 4e-14 K and saves 17 %)
 FP32_TAU_CUT = 30.0
+# FP64 mode: tau > 50 (what is left weighs < e^-50 ~ 2e-22 of the sums, below FP64 resolution)
+FP64_TAU_CUT = 50.0
 
 
 def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False):

@@ -34,7 +34,7 @@ class RetrievalBatch:
         if precision not in _lib.PRECISIONS:
             raise ValueError('precision must be fp32 or fp64')
         self.precision = precision
-        self.tau_cut = (K.FP32_TAU_CUT if precision == 'fp32' else 0.0) if tau_cut is None else float(tau_cut)
+        self.tau_cut = (K.FP32_TAU_CUT if precision == 'fp32' else K.FP64_TAU_CUT) if tau_cut is None else float(tau_cut)
         self.catalog = catalog
         self.absorbers = absorbers
         self.freqs = np.asarray(freqs, dtype=np.float64)
