@@ -166,8 +166,9 @@ int rbx_device_count(int *count);
                                     per wavefront tile only the lines whose poles lie within a
                                     factor 3 of the tile's x = f^2 range and adds the others as
                                     16-term power series (at most 8e-7 of a far line's value, tables
-                                    within 3e-7 of FP64 as the direct sums); built in: 2 (needs a
-                                    workspace of ~24 B per line and layer more) */
+                                    within 3e-7 of FP64 as the direct sums); built in: 2 while its
+                                    workspace (~24 B per line and layer more) stays under 4 GiB,
+                                    else 1 */
 #define RBX_NUM_TUNE_KEYS    5
 int rbx_tune(int key, int value);
 

@@ -1404,6 +1404,8 @@ static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
 }
 
 // FP32 far-line series (far.hpp): rbx_tune(RBX_TUNE_ALPHA_SERIES) 1 off, 2 on; built in on
+// while its workspace stays under kSeriesMaxWorkspace
+constexpr size_t kSeriesMaxWorkspace = size_t(4) << 30;
 static bool series_on(int precision) {
     return precision == RBX_FP32 && tune_get(RBX_TUNE_ALPHA_SERIES) != 1;
 }
@@ -1512,7 +1514,11 @@ static int plan_alpha(AlphaArgs &A, int precision, int &K, int &nsplit, bool &se
     if (series) {
         A.far_split = far_split(A);
         nsplit = 1;
-    } else {
+        // the staged lines take 24 B per line and layer: past kSeriesMaxWorkspace the built-in
+        // choice is the direct sums (rbx_tune(RBX_TUNE_ALPHA_SERIES, 2) forces the series)
+        if (tune_get(RBX_TUNE_ALPHA_SERIES) != 2 && alpha_ws_bytes(A, 1, true) > kSeriesMaxWorkspace) series = false;
+    }
+    if (!series) {
         int rc = alpha_split(A, K, precision, nsplit);
         if (rc) return rc;
     }

@@ -111,3 +111,25 @@ def test_series_large_workload_sample(K, at):
     at.set_knob('alpha_series', 2)
     series = K.alpha(cat, sel, P, T, mix, w['freqs'], 'fp32').double().cpu().numpy()
     assert rel(series, direct) < 1e-6
+
+
+def test_series_workspace_cap(K, at, raw_lines):
+    """The series path stores every line's staged form per layer (24 B): for a request whose
+    workspace would pass 4 GiB the built-in choice is the direct sums; forcing it still works.
+    (Sizes from rbx_alpha_workspace_size only: nothing this large is allocated.)"""
+    import ctypes
+    from radiobear_amd import _lib
+    cat = K.jupiter_catalog(raw_lines)
+    ab = K.absorbers_struct(JUPITER_ALPHA)
+
+    def size(nl, mode):
+        at.set_knob('alpha_series', mode)
+        nb = ctypes.c_size_t(0)
+        _lib.check(_lib.lib().rbx_alpha_workspace_size(cat.handle, ctypes.byref(ab), nl, 1024, 512, _lib.FP32,
+                                                       ctypes.byref(nb)))
+        return nb.value
+
+    big = 400000
+    assert size(big, 2) > 4 << 30                  # the series workspace of this request
+    assert size(big, 0) == size(big, 1)            # built in: the direct sums
+    assert size(40, 0) == size(40, 2) != size(40, 1)  # below the cap: the series
