@@ -133,3 +133,26 @@ def test_series_workspace_cap(K, at, raw_lines):
     assert size(big, 2) > 4 << 30                  # the series workspace of this request
     assert size(big, 0) == size(big, 1)            # built in: the direct sums
     assert size(40, 0) == size(40, 2) != size(40, 1)  # below the cap: the series
+
+
+@pytest.mark.parametrize('band', [(1.0, 25.0), (35.0, 400.0)])
+def test_series_one_inversion_set(K, at, raw_lines, band):
+    """Requests entirely below or above the 30 GHz switch: one coefficient set of the inversion
+    lines is never used (its stream is neither written nor read)."""
+    atm = synth.jupiter_like(17)
+    cat = K.jupiter_catalog(raw_lines)
+    f = np.geomspace(band[0], band[1], 300)
+    direct, series = both_paths(K, at, cat, JUPITER_ALPHA, atm, f)
+    assert rel(series, direct) < 1e-6
+
+
+@pytest.mark.parametrize('sel', [{'nh3': 'nh3_hs'}, {'h2o': 'h2o_bk', 'h2': 'h2_jj_ddb'}, {'ph3': 'ph3_jh'},
+                                 {'h2s': 'h2s_ddb', 'nh3': 'nh3_dbs'}])
+def test_series_absorber_subsets(K, at, raw_lines, sel):
+    """Absorber selections that leave line segments empty."""
+    atm = synth.jupiter_like(13)
+    h2o = synth.h2o_lines(200, 3)
+    cat = K.jupiter_catalog(raw_lines, h2o=h2o)
+    f = np.geomspace(0.5, 300.0, 257)
+    direct, series = both_paths(K, at, cat, sel, atm, f)
+    assert rel(series, direct) < 1e-6
