@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 
-CANDIDATES = {'alpha_k': (1, 2, 4), 'alpha_split': (0, 1, 2, 4, 8), 'int_unroll': (4, 8), 'int_map': (1, 2),
+CANDIDATES = {'alpha_k': (1, 2, 4), 'alpha_split': (0, 1, 2, 3, 4, 6, 8), 'int_unroll': (4, 8), 'int_map': (1, 2),
               'alpha_series': (1, 2)}
 _KEYS = {'alpha_k': _lib.TUNE_ALPHA_K, 'alpha_split': _lib.TUNE_ALPHA_SPLIT, 'int_unroll': _lib.TUNE_INT_UNROLL,
          'int_map': _lib.TUNE_INT_MAP, 'alpha_series': _lib.TUNE_ALPHA_SERIES}
