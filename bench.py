@@ -175,9 +175,8 @@ def autotune(prob, prec, args):
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         prob.launch_paths()
-        # FP32: far-line series (1 off, 2 on), then the tile width and split for that path
-        kf = at.tune('alpha_series', key, lambda: prob.launch_alpha(prec), path=args.tune_cache) \
-            if prec == _lib.FP32 else None
+        # far-line series (1 off, 2 on), then the tile width and split for that path
+        kf = at.tune('alpha_series', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
         ka = at.tune('alpha_k', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
         ks = at.tune('alpha_split', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
         ki = at.tune('int_unroll', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
@@ -192,8 +191,10 @@ def autotune(prob, prec, args):
 def other_workloads(args):
     """The nadir-ray case (workload 1: 150 layers, 2000 NH3 lines, 20 freqs, FP64 plain; launch
     bound, so its absolute time is the figure) and the image case (workload 2: 400 layers,
-    10000 lines, 512 freqs, 64x64 pixels; FP64 and FP32), one graph-replayed step each."""
-    from radiobear_amd import _lib
+    10000 lines, 512 freqs, 64x64 pixels; FP64 and FP32), one graph-replayed step each, with the
+    library's built-in launch choices (the knobs tuned for workload 3 are cleared first)."""
+    from radiobear_amd import _lib, autotune as at
+    at.reset()
     out = {}
     for name, precs in (('c1_plain_fp64', ('fp64',)), ('c2_image', ('fp64', 'fp32'))):
         prob = Problem(name)

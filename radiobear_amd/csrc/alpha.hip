@@ -121,8 +121,9 @@ struct AlphaArgs {
     // FP32 far-line series (far.hpp): near-line masks [nl][tiles_total][far_words], moments
     // [nl][far_split][tiles_total][kFarMom], h2o base terms [far_split][nl]
     unsigned int *near;
-    double2 *near_dh;   // staged lines per layer and stream line [nl][far_words * 32]: Ds, h
+    double2 *near_dh;   // FP32 staged lines per layer and stream line [nl][far_words * 32]: Ds, h
     float2 *near_ac;    // and the scaled a, c
+    double *near_c64;   // FP64: D, a, c, g, G planes [5][nl * far_words * 32]
     double *far_mom, *far_base;
     double *far_scale;  // FP32 line-weight scale per layer, segment and coefficient set [nl][kSegs][2]
     int far_words, far_split;
@@ -428,10 +429,10 @@ __device__ __forceinline__ void sum_staged_f32(const ST &st, const int (&set)[K]
     }
 }
 
-template <int K, bool UNI>
-__device__ __forceinline__ void sum_staged_f64(const Staged64 &st, const int (&set)[K], int cntp,
+template <int K, bool UNI, class ST = Staged64>
+__device__ __forceinline__ void sum_staged_f64(const ST &st, const int (&set)[K], int jb, int je,
                                                const double (&x)[K], double (&acc)[K]) {
-    for (int j = 0; j < cntp; ++j) {
+    for (int j = jb; j < je; ++j) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int m = UNI ? set[0] : set[k];
@@ -516,9 +517,9 @@ __device__ void line_sum(const DevBlock &B, const Prep *prep, int need, bool act
                     sum_staged_f32<K, false>(sh.s32, set, 0, cntp, x, xf, acc);
             } else {
                 if (uni)
-                    sum_staged_f64<K, true>(sh.s64, set, cntp, x, acc);
+                    sum_staged_f64<K, true>(sh.s64, set, 0, cntp, x, acc);
                 else
-                    sum_staged_f64<K, false>(sh.s64, set, cntp, x, acc);
+                    sum_staged_f64<K, false>(sh.s64, set, 0, cntp, x, acc);
             }
         }
         __syncthreads();
@@ -1403,11 +1404,21 @@ static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
     return RBX_OK;
 }
 
-// FP32 far-line series (far.hpp): rbx_tune(RBX_TUNE_ALPHA_SERIES) 1 off, 2 on; built in on
-// while its workspace stays under kSeriesMaxWorkspace
+// far-line series (far.hpp): rbx_tune(RBX_TUNE_ALPHA_SERIES) 1 off, 2 on; built in on for
+// requests of at least kSeriesMinTiles wavefront tiles while the workspace stays under
+// kSeriesMaxWorkspace.  (With fewer tiles few (line, tile) pairs are far and the extra pass
+// costs more than it saves: one 20-frequency tile of 150 layers took 0.37 ms per step against
+// 0.21 ms with the direct sums.)
 constexpr size_t kSeriesMaxWorkspace = size_t(4) << 30;
-static bool series_on(int precision) {
-    return precision == RBX_FP32 && tune_get(RBX_TUNE_ALPHA_SERIES) != 1;
+constexpr int kSeriesMinTiles = 4;
+static bool series_on(const AlphaArgs &A) {
+    const int t = tune_get(RBX_TUNE_ALPHA_SERIES);
+    return t == 2 || (t != 1 && A.tiles_total >= kSeriesMinTiles);
+}
+
+// bytes of one line's staged form per layer: FP32 (Ds, h) + scaled (a, c); FP64 (D, a, c, g, G)
+static size_t staged_line_bytes(int precision) {
+    return precision == RBX_FP32 ? sizeof(double2) + sizeof(float2) : 5 * sizeof(double);
 }
 
 // near-mask streams: one bit per line of every staging round of each selected segment
@@ -1444,13 +1455,13 @@ static int far_split(const AlphaArgs &A) {
     return static_cast<int>(std::max(1LL, std::min<long long>(want, std::min(kMaxSplit, std::max(R, 1)))));
 }
 
-static size_t alpha_ws_bytes(const AlphaArgs &A, int nsplit, bool series) {
+static size_t alpha_ws_bytes(const AlphaArgs &A, int precision, int nsplit, bool series) {
     const size_t plane = static_cast<size_t>(A.nl) * A.nf;
     const size_t extra = has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0;
     size_t b = (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl + extra) *
                sizeof(double);
     if (series)   // (+16: the staged lines start 16-byte aligned)
-        b += 16 + static_cast<size_t>(A.nl) * A.far_words * 32 * (sizeof(double2) + sizeof(float2)) +
+        b += 16 + static_cast<size_t>(A.nl) * A.far_words * 32 * staged_line_bytes(precision) +
              (static_cast<size_t>(A.nl) * A.far_split * A.tiles_total * kFarMom +
               static_cast<size_t>(A.far_split) * A.nl + static_cast<size_t>(A.nl) * kSegs * 2) * sizeof(double) +
              static_cast<size_t>(A.nl) * A.tiles_total * A.far_words * sizeof(unsigned int);
@@ -1481,15 +1492,15 @@ static int launch_alpha(AlphaArgs &A, int precision, int nsplit, hipStream_t st)
 }
 
 // the series path: classification pass, near sums per tile, then the same finish (one part)
-template <int K>
+template <int K, bool F32>
 static int launch_series(AlphaArgs &A, hipStream_t st) {
     const long long groups = (A.tiles_total + kFarTiles - 1) / kFarTiles;
     dim3 g1(static_cast<unsigned>(groups * A.nl), static_cast<unsigned>(A.far_split));
-    hipLaunchKernelGGL(alpha_far_kernel, g1, dim3(kThreads), 0, st, A);
+    hipLaunchKernelGGL(alpha_far_kernel<F32>, g1, dim3(kThreads), 0, st, A);
     int rc = launch_status("alpha_far_kernel");
     if (rc) return rc;
     dim3 g2(static_cast<unsigned>(static_cast<long long>(A.tiles_total) * A.nl));
-    hipLaunchKernelGGL((alpha_near_kernel<K>), g2, dim3(kThreads), 0, st, A);
+    hipLaunchKernelGGL((alpha_near_kernel<K, F32>), g2, dim3(kThreads), 0, st, A);
     rc = launch_status("alpha_near_kernel");
     if (rc) return rc;
     dim3 g3(static_cast<unsigned>((A.nf + kThreads - 1) / kThreads * static_cast<long long>(A.nl)));
@@ -1498,15 +1509,20 @@ static int launch_series(AlphaArgs &A, hipStream_t st) {
         rc = launch_status("alpha_extra_kernel");
         if (rc) return rc;
     }
-    hipLaunchKernelGGL((alpha_finish_kernel<true>), g3, dim3(kThreads), 0, st, A, 1);
+    hipLaunchKernelGGL((alpha_finish_kernel<F32>), g3, dim3(kThreads), 0, st, A, 1);
     return launch_status("alpha_finish_kernel");
+}
+
+template <int K>
+static int launch_series(AlphaArgs &A, int precision, hipStream_t st) {
+    return precision == RBX_FP32 ? launch_series<K, true>(A, st) : launch_series<K, false>(A, st);
 }
 
 // workspace of one rbx_alpha call: the tile width, split and path it will use
 static int plan_alpha(AlphaArgs &A, int precision, int &K, int &nsplit, bool &series, size_t &bytes) {
     K = alpha_tile_k(A.nf);
     set_wave_tiles(A, K);
-    series = series_on(precision);
+    series = series_on(A);
     if (series) {
         far_layout(A);
         series = A.far_words > 0;   // no line lists selected: the plain path (one part)
@@ -1514,15 +1530,17 @@ static int plan_alpha(AlphaArgs &A, int precision, int &K, int &nsplit, bool &se
     if (series) {
         A.far_split = far_split(A);
         nsplit = 1;
-        // the staged lines take 24 B per line and layer: past kSeriesMaxWorkspace the built-in
-        // choice is the direct sums (rbx_tune(RBX_TUNE_ALPHA_SERIES, 2) forces the series)
-        if (tune_get(RBX_TUNE_ALPHA_SERIES) != 2 && alpha_ws_bytes(A, 1, true) > kSeriesMaxWorkspace) series = false;
+        // the staged lines take 24 B (FP32) or 40 B (FP64) per line and layer: past
+        // kSeriesMaxWorkspace the built-in choice is the direct sums (rbx_tune(RBX_TUNE_ALPHA_SERIES,
+        // 2) forces the series)
+        if (tune_get(RBX_TUNE_ALPHA_SERIES) != 2 && alpha_ws_bytes(A, precision, 1, true) > kSeriesMaxWorkspace)
+            series = false;
     }
     if (!series) {
         int rc = alpha_split(A, K, precision, nsplit);
         if (rc) return rc;
     }
-    bytes = alpha_ws_bytes(A, nsplit, series);
+    bytes = alpha_ws_bytes(A, precision, nsplit, series);
     return RBX_OK;
 }
 
@@ -1556,7 +1574,7 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
                          int nf_nh3_low, const double *scale, int precision, void *alpha, double *by_constituent,
                          void *workspace, size_t workspace_bytes, void *stream) {
     RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha: bad precision %d", precision);
-    rbx::AlphaArgs A;
+    rbx::AlphaArgs A{};
     int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, cloud, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
                             by_constituent, "rbx_alpha");
     if (rc) return rc;
@@ -1578,16 +1596,24 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
         const size_t lines = static_cast<size_t>(nl) * A.far_words * 32;
         const uintptr_t at = reinterpret_cast<uintptr_t>(
             A.ws_extra + (rbx::has_extra(A.abs) ? static_cast<size_t>(rbx::kNumExtra) * plane : 0));
-        A.near_dh = reinterpret_cast<double2 *>((at + 15) / 16 * 16);
-        A.near_ac = reinterpret_cast<float2 *>(A.near_dh + lines);
-        A.far_mom = reinterpret_cast<double *>(A.near_ac + lines);
+        const uintptr_t lines_at = (at + 15) / 16 * 16;
+        double *after;
+        if (precision == RBX_FP32) {
+            A.near_dh = reinterpret_cast<double2 *>(lines_at);
+            A.near_ac = reinterpret_cast<float2 *>(A.near_dh + lines);
+            after = reinterpret_cast<double *>(A.near_ac + lines);
+        } else {
+            A.near_c64 = reinterpret_cast<double *>(lines_at);
+            after = A.near_c64 + 5 * lines;
+        }
+        A.far_mom = after;
         A.far_base = A.far_mom + static_cast<size_t>(nl) * A.far_split * A.tiles_total * rbx::kFarMom;
         A.far_scale = A.far_base + static_cast<size_t>(A.far_split) * nl;
         A.near = reinterpret_cast<unsigned int *>(A.far_scale + static_cast<size_t>(nl) * rbx::kSegs * 2);
         switch (K) {
-            case 4: return rbx::launch_series<4>(A, st);
-            case 2: return rbx::launch_series<2>(A, st);
-            default: return rbx::launch_series<1>(A, st);
+            case 4: return rbx::launch_series<4>(A, precision, st);
+            case 2: return rbx::launch_series<2>(A, precision, st);
+            default: return rbx::launch_series<1>(A, precision, st);
         }
     }
     switch (K) {
@@ -1602,7 +1628,7 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
                                  const double *freq_nh3, int nf, int nf_nh3_low, const double *scale, int precision,
                                  double *alpha, double *dalpha, void *stream) {
     RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_tangent: bad precision %d", precision);
-    rbx::AlphaArgs A;
+    rbx::AlphaArgs A{};
     RBX_CHECK_ARG(absorbers && !rbx::has_extra(*absorbers),
                   "rbx_alpha_tangent: co_ddb, clouds_idp and h2_orton have no tangent; select them off");
     int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nullptr, nl, freq, freq_nh3, nf, nf_nh3_low, scale, dalpha,

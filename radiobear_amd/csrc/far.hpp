@@ -16,7 +16,9 @@
 // small next to the near ones, and the tables stay within 3e-7 of FP64 like the direct
 // FP32 sums.  On 800 layers x 1024 frequencies (0.3-300 GHz, 128-column tiles) about
 // 30 % of the (line, tile) pairs are neither: only those are summed per column.  (12
-// terms at 1/4 leave more near pairs, 33 %, and were 2 % slower overall.)
+// terms at 1/4 leave more near pairs, 33 %, and were 2 % slower overall.)  FP64 mode uses
+// the same 16 terms at ratio 1/8 (at most ~4e-13 of a far line's value) and sums its near
+// lines in FP64 from their (D, a, c, g, G), as the direct FP64 sums do.
 //
 // alpha_far_kernel (block = layer x group of kFarTiles tiles x part of the layer's
 // staging rounds): each thread derives one line's coefficients per round (once per
@@ -24,7 +26,8 @@
 // tile each, add the far lines' series coefficients into per-tile moments (FP64,
 // in registers, reduced per segment) and write one bit per near line into the tile's
 // near mask.  The h2o base term is summed here.
-// It also stores every line's staged FP32-mode form (Ds, h; scaled a, c) per layer.
+// It also stores every line's staged form per layer: FP32 mode (Ds, h; scaled a, c),
+// FP64 mode (D, a, c, g, G).
 // alpha_near_kernel (block = tile x layer, all four wavefronts on the tile's
 // columns): lists the tile's near lines from its masks, gathers and sums them as the
 // line-sum kernel does with the staged lines split between the wavefronts, adds the
@@ -32,7 +35,10 @@
 // per-slot partials the finish kernel reads (one part).
 
 constexpr int kFarTerms = 16;
-constexpr double kFarRatio = 1.0 / 3.0;
+// pole ratio of a far line: 1/3 in FP32 mode, 1/8 in FP64 mode (at most 4e-13 of a far
+// line's value there, against the 1e-9 FP64 parity bar)
+template <bool F32>
+__host__ __device__ constexpr double far_ratio() { return F32 ? 1.0 / 3.0 : 0.125; }
 constexpr int kFarTiles = 8;                          // tiles per alpha_far_kernel block
 constexpr int kFarSets = 6;                           // h2o, h2s, inversion set 0, set 1, rot + v2, ph3
 constexpr int kFarMom = kFarSets * 2 * kFarTerms;     // moments per (layer, part, tile)
@@ -176,6 +182,7 @@ struct FarPairs {
 
 // three blocks per CU (168 VGPRs, a few spills) run the pass in 1.00 ms against 1.18 ms with two
 // blocks and no spills; four (128 VGPRs) gain 3 % more for twice the spill traffic
+template <bool F32>
 __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
     __shared__ FarShared sh;
     const int ngroups = (A.tiles_total + kFarTiles - 1) / kFarTiles;
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
     gc1 = min(A.nf, (g + 1) * kFarTiles * A.tile_width);
     const int gsets = tile_sets(A, gc0, gc1);   // inversion sets any tile of the group needs
     const int ntiles = min(kFarTiles, A.tiles_total - g * kFarTiles);
-    const double q = kFarRatio;
+    const double q = far_ratio<F32>();
     __syncthreads();
 
     const LinePlan plan(A, L.P);
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
         for (int k = 0; k < kFarTerms; ++k) accB[k] = accA[k] = 0.0;
 
         with_prep(A, L, seg, [&](const auto *prep, double s0, double s1) {
-            const double scale[2] = {s0, s1};
+            const double scale[2] = {F32 ? s0 : 1.0, F32 ? s1 : 1.0};
             if (g == 0 && tid < 2)   // (every part that has the segment stores the same value)
                 A.far_scale[(static_cast<size_t>(layer) * kSegs + seg) * 2 + tid] = scale[tid];
             for (int r = a0; r < b0; ++r) {
@@ -251,9 +258,18 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
                             const size_t o = static_cast<size_t>(layer) * A.far_words * 32 +
                                              static_cast<size_t>(A.far_off[kSegStream[seg] + (inv ? s : 0)]) * 32 +
                                              l0 + tid;
-                            A.near_dh[o] = make_double2(Ds, h);
-                            A.near_ac[o] = make_float2(static_cast<float>(qs[s].a * scale[s]),
-                                                       static_cast<float>(qs[s].c * scale[s]));
+                            if constexpr (F32) {
+                                A.near_dh[o] = make_double2(Ds, h);
+                                A.near_ac[o] = make_float2(static_cast<float>(qs[s].a * scale[s]),
+                                                           static_cast<float>(qs[s].c * scale[s]));
+                            } else {
+                                const size_t pl = static_cast<size_t>(A.nl) * A.far_words * 32;
+                                A.near_c64[o] = qs[s].D;
+                                A.near_c64[pl + o] = qs[s].a;
+                                A.near_c64[2 * pl + o] = qs[s].c;
+                                A.near_c64[3 * pl + o] = qs[s].g;
+                                A.near_c64[4 * pl + o] = qs[s].G;
+                            }
                         }
                         sh.st.a[s][tid] = qs[s].a;
                         sh.st.c[s][tid] = qs[s].c;
@@ -359,9 +375,14 @@ struct WaveStage {
     unsigned int fast[2][kThreads / 64];
 };
 
-template <int K>
+// FP64 mode: the (D, a, c, g, G) the direct FP64 sums read (sum_staged_f64)
+struct WaveStage64 {
+    double D[2][kWaveBatch], a[2][kWaveBatch], c[2][kWaveBatch], g[2][kWaveBatch], G[2][kWaveBatch];
+};
+
+template <int K, bool F32>
 struct NearShared {
-    WaveStage st[kThreads / 64];
+    std::conditional_t<F32, WaveStage, WaveStage64> st[kThreads / 64];
     union {
         unsigned short list[kListWords * 32];   // a window's near lines
         double red[kThreads / 64][K][64];       // the wavefronts' partial sums at a slot's end
@@ -391,14 +412,16 @@ __device__ inline double far_eval(const double *mom, double x) {
 
 // near lines of one segment and coefficient set v, listed from the tile's mask words; each
 // wavefront sums its own batches of kWaveBatch listed lines (batches w, w + 4, ...), staged
-// in its own LDS and fetched one batch ahead into registers
-template <int K>
-__device__ void near_sum(int n, const double2 *stream_dh, const float2 *stream_ac, int v, bool mixed, bool uni,
-                         const unsigned int *mask, int nwords, int var, const int (&set)[K], const double (&x)[K],
-                         const float (&xf)[K], double (&acc)[K], NearShared<K> &sh) {
+// in its own LDS and fetched one batch ahead into registers.  `so` = the stream's first line
+// in the staged-line arrays.
+template <int K, bool F32>
+__device__ void near_sum(const AlphaArgs &A, int n, size_t so, int v, bool mixed, bool uni, const unsigned int *mask,
+                         int nwords, int var, const int (&set)[K], const double (&x)[K], const float (&xf)[K],
+                         double (&acc)[K], NearShared<K, F32> &sh) {
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    WaveStage &st = sh.st[wave];
+    auto &st = sh.st[wave];
     const double lo = sh.rng.lo[var][0], hi = sh.rng.hi[var][0];
+    const size_t pl = static_cast<size_t>(A.nl) * A.far_words * 32;   // FP64 coefficient plane
     for (int w0 = 0; w0 < nwords; w0 += kListWords) {
         const int wi = w0 + tid;
         const unsigned int bits = (tid < kListWords && wi < nwords) ? mask[wi] : 0u;
@@ -420,54 +443,88 @@ __device__ void near_sum(int n, const double2 *stream_dh, const float2 *stream_a
         for (unsigned int b = bits; b != 0u; b &= b - 1u)
             sh.list[pos++] = static_cast<unsigned short>(tid * 32 + __builtin_ctz(b));
         __syncthreads();
-        // the staged form pass 1 stored for listed line e (past the list: a zero term)
-        auto fetch = [&](int e, double2 &dh, float2 &ac) {
+        // the staged form pass 1 stored for listed line e (past the list: a zero term):
+        // FP32 (Ds, h, a s, c s), FP64 (D, a, c, g, G)
+        auto fetch = [&](int e, double (&c5)[5], float2 &ac) {
             const int i = e < total ? w0 * 32 + sh.list[e] : n;
-            dh = make_double2(0.0, 1.0);
+            c5[0] = 0.0; c5[1] = 0.0; c5[2] = 0.0; c5[3] = 0.0; c5[4] = 1.0;
             ac = make_float2(0.0f, 0.0f);
             if (i < n) {
-                dh = stream_dh[i];
-                ac = stream_ac[i];
+                if constexpr (F32) {
+                    const double2 dh = A.near_dh[so + i];
+                    c5[0] = dh.x;
+                    c5[1] = dh.y;
+                    ac = A.near_ac[so + i];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) c5[m] = A.near_c64[m * pl + so + i];
+                }
+            }
+            if constexpr (F32) {
+                if (!(i < n)) c5[1] = 1.0;   // padding h
             }
         };
         const int nb = (total + kWaveBatch - 1) / kWaveBatch;
-        double2 dh;
+        double c5[5];
         float2 ac;
-        if (wave < nb) fetch(wave * kWaveBatch + lane, dh, ac);
+        if (wave < nb) fetch(wave * kWaveBatch + lane, c5, ac);
         for (int b = wave; b < nb; b += kThreads / 64) {
             const int cntp = (min(kWaveBatch, total - b * kWaveBatch) + kSub - 1) / kSub * kSub;
-            st.D[v][lane] = dh.x;
-            st.h[v][lane] = dh.y;
-            st.ac[v][lane] = ac;
-            st.dh[v][lane] = make_float2(static_cast<float>(dh.x), static_cast<float>(dh.y));
-            if (mixed) {   // the other set's columns add nothing
-                st.D[1 - v][lane] = 0.0;
-                st.h[1 - v][lane] = 1.0;
-                st.ac[1 - v][lane] = make_float2(0.0f, 0.0f);
-                st.dh[1 - v][lane] = make_float2(0.0f, 1.0f);
+            if constexpr (F32) {
+                st.D[v][lane] = c5[0];
+                st.h[v][lane] = c5[1];
+                st.ac[v][lane] = ac;
+                st.dh[v][lane] = make_float2(static_cast<float>(c5[0]), static_cast<float>(c5[1]));
+                if (mixed) {   // the other set's columns add nothing
+                    st.D[1 - v][lane] = 0.0;
+                    st.h[1 - v][lane] = 1.0;
+                    st.ac[1 - v][lane] = make_float2(0.0f, 0.0f);
+                    st.dh[1 - v][lane] = make_float2(0.0f, 1.0f);
+                }
+                const unsigned int fb = full_chunks(__ballot(fp32_safe(c5[0], c5[1], lo, hi)));
+                if (lane == 0) st.fast[v][wave] = fb;
+            } else {
+                st.D[v][lane] = c5[0];
+                st.a[v][lane] = c5[1];
+                st.c[v][lane] = c5[2];
+                st.g[v][lane] = c5[3];
+                st.G[v][lane] = c5[4];
+                if (mixed) {
+                    st.D[1 - v][lane] = 0.0;
+                    st.a[1 - v][lane] = 0.0;
+                    st.c[1 - v][lane] = 0.0;
+                    st.g[1 - v][lane] = 0.0;
+                    st.G[1 - v][lane] = 1.0;
+                }
             }
-            const unsigned int fb = full_chunks(__ballot(fp32_safe(dh.x, dh.y, lo, hi)));
-            if (lane == 0) st.fast[v][wave] = fb;
             wave_sync();
             const int bn = b + kThreads / 64;
-            double2 dh2 = make_double2(0.0, 1.0);
+            double c5n[5] = {0.0, 1.0, 0.0, 0.0, 1.0};
             float2 ac2 = make_float2(0.0f, 0.0f);
-            if (bn < nb) fetch(bn * kWaveBatch + lane, dh2, ac2);
-            if (uni)
-                sum_staged_f32<K, true>(st, set, 0, cntp, x, xf, acc);
-            else
-                sum_staged_f32<K, false>(st, set, 0, cntp, x, xf, acc);
+            if (bn < nb) fetch(bn * kWaveBatch + lane, c5n, ac2);
+            if constexpr (F32) {
+                if (uni)
+                    sum_staged_f32<K, true>(st, set, 0, cntp, x, xf, acc);
+                else
+                    sum_staged_f32<K, false>(st, set, 0, cntp, x, xf, acc);
+            } else {
+                if (uni)
+                    sum_staged_f64<K, true>(st, set, 0, cntp, x, acc);
+                else
+                    sum_staged_f64<K, false>(st, set, 0, cntp, x, acc);
+            }
             wave_sync();
-            dh = dh2;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) c5[m] = c5n[m];
             ac = ac2;
         }
         __syncthreads();   // every wavefront is done with this window's list
     }
 }
 
-template <int K>
+template <int K, bool F32>
 __global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(AlphaArgs A) {
-    __shared__ NearShared<K> sh;
+    __shared__ NearShared<K, F32> sh;
     // layer fastest: blocks are dealt to the XCDs in turn, so each XCD gets every tile (the
     // tiles' near fractions differ several-fold) rather than one tile of every layer
     const int layer = blockIdx.x % A.nl, tile = blockIdx.x / A.nl;
@@ -574,11 +631,10 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(Al
                 const unsigned int *mk = masks + A.far_off[stream];
                 const size_t so = static_cast<size_t>(layer) * A.far_words * 32 + static_cast<size_t>(A.far_off[stream]) * 32;
                 if (inv)
-                    near_sum<K>(B.n, A.near_dh + so, A.near_ac + so, v, mixed, uni_set, mk, nwords, var, set, xn, xnf,
-                                a2, sh);
+                    near_sum<K, F32>(A, B.n, so, v, mixed, uni_set, mk, nwords, var, set, xn, xnf, a2, sh);
                 else
-                    near_sum<K>(B.n, A.near_dh + so, A.near_ac + so, 0, false, true, mk, nwords, var, one,
-                                var ? xn : x, var ? xnf : xf, a2, sh);
+                    near_sum<K, F32>(A, B.n, so, 0, false, true, mk, nwords, var, one, var ? xn : x, var ? xnf : xf,
+                                     a2, sh);
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {

@@ -28,11 +28,13 @@ def at(K):
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
 def test_alpha_every_tile_width(K, at, raw_lines, jupiter_lines, precision, tol):
-    """1, 2 and 4 frequencies per lane, all matching the oracle.  FP64: identical tables (the
-    per-frequency line order does not depend on the tile width).  FP32: a chunk of 8 staged
-    lines takes the all-FP32 path when it is far from every column of the wavefront, so the
-    choice depends on the tile's frequency range; tables agree to 1e-6.  330 freqs straddle
-    the 30 GHz regime switch inside a tile; 41 layers leave a partial last tile."""
+    """1, 2 and 4 frequencies per lane, all matching the oracle, with the direct line sums and
+    with the far-line series.  FP64 direct sums: identical tables (the per-frequency line order
+    does not depend on the tile width).  FP32: a chunk of 8 staged lines takes the all-FP32 path
+    when it is far from every column of the wavefront, and with the series which lines are far
+    depends on the tile's frequency range: tables agree to 1e-6 (FP32) and 1e-11 (FP64 series).
+    330 freqs straddle the 30 GHz regime switch inside a tile; 41 layers leave a partial last
+    tile."""
     atm = synth.jupiter_like(41)
     h2o = synth.h2o_lines(300, 5)
     cat = K.jupiter_catalog(raw_lines, h2o=h2o)
@@ -40,18 +42,21 @@ def test_alpha_every_tile_width(K, at, raw_lines, jupiter_lines, precision, tol)
     lines['h2o'] = h2o
     f = np.logspace(np.log10(1.0), np.log10(200.0), 330)
     ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(lines))
-    out = {}
-    for k in at.CANDIDATES['alpha_k']:
-        at.set_knob('alpha_k', k)
-        out[k] = K.alpha(cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), f,
-                         precision).double().cpu().numpy()
-        got = out[k].T
-        assert np.max(np.abs(got - ref) / np.abs(ref)) < tol, k
-    for k in out:
-        if precision == 'fp64':
-            assert np.array_equal(out[k], out[1]), k
-        else:
-            assert np.max(np.abs(out[k] - out[1]) / np.abs(out[1])) < 1e-6, k
+    for series in (1, 2):
+        at.set_knob('alpha_series', series)
+        out = {}
+        for k in at.CANDIDATES['alpha_k']:
+            at.set_knob('alpha_k', k)
+            out[k] = K.alpha(cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), f,
+                             precision).double().cpu().numpy()
+            got = out[k].T
+            assert np.max(np.abs(got - ref) / np.abs(ref)) < tol, (series, k)
+        for k in out:
+            if precision == 'fp64' and series == 1:
+                assert np.array_equal(out[k], out[1]), k
+            else:
+                same = 1e-11 if precision == 'fp64' else 1e-6
+                assert np.max(np.abs(out[k] - out[1]) / np.abs(out[1])) < same, (series, k)
 
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
