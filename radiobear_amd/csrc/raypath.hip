@@ -81,6 +81,26 @@ __device__ inline Surf ellipse(const Geo &G, double a, const LatLng &q) {
     return s;
 }
 
+// the shell quantities at a path point v that depend only on its latitude: f, the radius of
+// the unit-equatorial-radius ellipse there (a shell of equatorial radius a has a f), and the
+// shell normal n.  With X = x^2 + z^2, Y = y^2 and sin lat = y/|v|, cos lat = sqrt(X)/|v|,
+// ellipse() reduces to f = sqrt((X + ratio^2 Y) / (X + Y)) and
+// n = (ratio x, y, ratio z) / sqrt(ratio^2 X + Y) (the same values up to rounding, with one
+// division and two square roots in parallel instead of latlng's and ellipse's chain).  y == 0
+// takes the 1e-6 rad latitude substitution through latlng.
+__device__ inline void shell_at(const Geo &G, V3 v, double &f, V3 &n) {
+    if (v.y == 0.0) {
+        const Surf s = ellipse(G, 1.0, latlng(v));
+        f = s.rmag;
+        n = s.n;
+        return;
+    }
+    const double X = v.x * v.x + v.z * v.z, Y = v.y * v.y, r2 = G.ratio * G.ratio;
+    f = sqrt(fma(r2, Y, X) / (X + Y));
+    const double inv = 1.0 / sqrt(fma(r2, X, Y));
+    n = {G.ratio * v.x * inv, v.y * inv, G.ratio * v.z * inv};
+}
+
 struct RayArgs {
     Geo G;
     const double *R, *N;
@@ -164,8 +184,8 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
     const V3 edge = rNorm * mat(G.M, V3{b0, b1, zq});
 
     // ---- compute_ds: start on the top shell at the edge's latitude ----
-    LatLng q = latlng(edge);
-    Surf geo = ellipse(G, rNorm, q);
+    const LatLng q = latlng(edge);
+    const Surf geo = ellipse(G, rNorm, q);
     V3 s = mat(G.M, V3{0.0, 0.0, -1.0});
     V3 r = geo.r;
     V3 n = geo.n;
@@ -176,10 +196,10 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
     const double cm_half = (A.units == 1);
     int layer = 0;
     int count = 0;
+    double rNow = geo.rmag;                           // shell layer at the current point
+    double rNext = ellipse(G, Rl[1], q).rmag;         // shell layer + 1 at the same latitude (raypath.py:182)
     while (true) {
         s = nratio * s + (nratio * c - ct) * n;      // raypath.py:179-180 (ingress)
-        const double rNow = geo.rmag;
-        const double rNext = ellipse(G, Rl[layer + 1], q).rmag;
         const double rdots = dot(r, s);
         const double disc = rdots * rdots + rNext * rNext - rNow * rNow;
         double ds = -rdots - sqrt(disc);             // NaN when the shell is missed
@@ -188,13 +208,14 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
         out[count] = cm_half ? (ds * 1.0E3 / 1.0E-2) * 0.5 : ds;
         ++count;
         const V3 rn = r + ds * s;
-        q = latlng(rn);
-        geo = ellipse(G, Rl[layer + 1], q);
+        double f;
+        shell_at(G, rn, f, n);
+        rNow = Rl[layer + 1] * f;
         r = rn;
-        n = geo.n;
         ++layer;
         c = -dot(s, n);
         if (layer + 1 >= nl) break;                  // IndexError on nr[layer + 1]
+        rNext = Rl[layer + 1] * f;
         nratio = 1.0;
         st = sqrt(1.0 - c * c);
         ct = sqrt(1.0 - st * st);
