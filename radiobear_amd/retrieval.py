@@ -66,6 +66,7 @@ class RetrievalBatch:
         self.planet = K.planet_desc(req, rpol, orientation)
         self.req, self.p_ref = float(req), float(p_ref)
         self.gm = None if gm is None else float(gm)
+        self._side = None   # side stream of the ray paths (created on first run)
         if device_props:
             self.update()
 
@@ -93,9 +94,19 @@ class RetrievalBatch:
 
     def run(self):
         """Tb (nray, nf) float64 and J (nray, 2, ldj, nf) float32 (rows past an atmosphere's
-        layers are 0)."""
+        layers are 0).  The ray paths (a few hundred latency-bound wavefronts) run on a side
+        stream beside the absorption pass; the Jacobian pass waits for both."""
+        import torch
+        main = torch.cuda.current_stream()
+        if self._side is None:
+            self._side = torch.cuda.Stream()
+        self._side.wait_stream(main)
+        with torch.cuda.stream(self._side):
+            hds, ns = self.paths()
         alpha, dalpha = self.absorption()
-        hds, ns = self.paths()
+        main.wait_stream(self._side)
+        hds.record_stream(main)   # allocated on the side stream, read on this one
+        ns.record_stream(main)
         J, Tb = K.jacobian_batch(alpha, dalpha, self.T, self.off_d, self.ray_atm_d, hds, ns, self.ldj,
                                  order=self.order_d, tau_cut=self.tau_cut, precision=self.precision)
         return Tb, J
