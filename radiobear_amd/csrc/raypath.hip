@@ -199,6 +199,9 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
     int count = 0;
     double rNow = geo.rmag;                           // shell layer at the current point
     double rNext = ellipse(G, Rl[1], q).rmag;         // shell layer + 1 at the same latitude (raypath.py:182)
+    // equatorial radii of shells layer + 1 and layer + 2, the latter loaded a step ahead (its
+    // load latency hides behind one step of the chain)
+    double R1 = Rl[1], R2 = nl > 2 ? Rl[2] : 0.0;
     while (true) {
         s = nratio * s + (nratio * c - ct) * n;      // raypath.py:179-180 (ingress)
         const double rdots = dot(r, s);
@@ -211,12 +214,14 @@ __global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
         const V3 rn = r + ds * s;
         double f;
         shell_at(G, rn, f, n);
-        rNow = Rl[layer + 1] * f;
+        rNow = R1 * f;
         r = rn;
         ++layer;
         c = -dot(s, n);
         if (layer + 1 >= nl) break;                  // IndexError on nr[layer + 1]
-        rNext = Rl[layer + 1] * f;
+        rNext = R2 * f;
+        R1 = R2;
+        R2 = layer + 2 < nl ? Rl[layer + 2] : 0.0;
         nratio = 1.0;
         st = sqrt(1.0 - c * c);
         ct = sqrt(1.0 - st * st);
