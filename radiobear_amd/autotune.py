@@ -11,11 +11,15 @@ Knobs:
                   narrower tiles waste fewer lanes and give more blocks for
                   the tail of the grid;
 * ``alpha_split`` parts each layer's line sums are split into (0 = the
-                  library's model of the grid tail, or 1, 2, 4, 8): more parts
+                  library's model of the grid tail, or 1, 2, 3, 4, 6, 8): more parts
                   give more, shorter blocks, so the last wave of blocks idles less;
 * ``int_unroll``  path steps per load batch of rbx_integrate (4, 8);
 * ``int_map``     block order of rbx_integrate: 1 frequency group major, 2 ray major
-                  (every frequency group of a ray block on one XCD).
+                  (every frequency group of a ray block on one XCD);
+* ``alpha_series`` line sums of rbx_alpha: 1 every line at every frequency, 2 per
+                  wavefront tile only the lines near it, the others as power series
+                  (the built-in choice takes 2 from 4 tiles up while the workspace
+                  stays under 4 GiB).
 """
 import json
 import os
