@@ -299,10 +299,12 @@ def raypaths(planet, R, N, b, units='half_cm'):
     return ds, ns
 
 
-# FP32-mode optical-depth cut: a ray's integration stops once tau > 30 at every frequency of a
-# wavefront (batch granularity); what is left weighs < e^-30 ~ 1e-13 of the sums, below FP32
-# resolution (on 800 layers x 1024 frequencies it moves Tb by 4e-14 K and saves 17 %)
-FP32_TAU_CUT = 30.0
+# FP32-mode optical-depth cut: a ray's integration stops once tau > 20 at every frequency of a
+# wavefront (batch granularity).  What is left weighs < e^-20 ~ 2e-9 of the weight sum, and of
+# the T-weighted sum at most T_max / Tb times that (~1e-8 with deep layers at 7 Tb): below the
+# FP32 sums' unit roundoff 6e-8.  On 800 layers x 1024 frequencies it moves Tb by 3e-13 K; the
+# integration takes 1.04 ms against 1.19 ms at a cut of 30 and 1.43 ms at 50.
+FP32_TAU_CUT = 20.0
 # FP64 mode: tau > 50 (what is left weighs < e^-50 ~ 2e-22 of the sums, below FP64 resolution)
 FP64_TAU_CUT = 50.0
 

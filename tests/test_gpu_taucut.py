@@ -25,7 +25,7 @@ def paths(nl, nray, bad_ray, bad_step):
     return (torch.as_tensor(hds, device='cuda'), torch.as_tensor(ns, device='cuda'))
 
 
-@pytest.mark.parametrize('precision,cut', [('fp64', 50.0), ('fp32', 30.0), ('fp32', 50.0)])
+@pytest.mark.parametrize('precision,cut', [('fp64', 50.0), ('fp32', 20.0), ('fp32', 30.0), ('fp32', 50.0)])
 def test_cut_keeps_nan_rays(K, precision, cut):
     import torch
     nl, nf, nray = 200, 130, 3
@@ -41,7 +41,9 @@ def test_cut_keeps_nan_rays(K, precision, cut):
     assert np.all(np.isnan(full[1])) and np.all(np.isnan(cutTb[1]))
     good = [0, 2]
     assert np.all(np.isfinite(cutTb[good]))
-    assert np.max(np.abs(cutTb[good] - full[good])) < 1e-9
+    # what the cut leaves out weighs < e^-cut of the weight sum and of the T-weighted sum
+    # < T_max e^-cut: Tb moves by at most ~2 T_max e^-cut (2.5e-6 K at a cut of 20)
+    assert np.max(np.abs(cutTb[good] - full[good])) < 1e-9 + 2.0 * T.max() * np.exp(-cut)
 
 
 def test_jacobian_cut_keeps_nan_rays(K):
