@@ -305,8 +305,9 @@ def raypaths(planet, R, N, b, units='half_cm'):
 # FP32 sums' unit roundoff 6e-8.  On 800 layers x 1024 frequencies it moves Tb by 3e-13 K; the
 # integration takes 1.04 ms against 1.19 ms at a cut of 30 and 1.43 ms at 50.
 FP32_TAU_CUT = 20.0
-# FP64 mode: tau > 50 (what is left weighs < e^-50 ~ 2e-22 of the sums, below FP64 resolution)
-FP64_TAU_CUT = 50.0
+# FP64 mode: tau > 40 (what is left weighs < e^-40 ~ 4e-18 of the weight sum and ~3e-17 of the
+# T-weighted sum, below the FP64 unit roundoff 1.1e-16)
+FP64_TAU_CUT = 40.0
 
 
 def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False):
