@@ -1,5 +1,5 @@
-// FP32-mode forward line sums with far-line series (included by alpha.hip, after the
-// line-sum kernel it replaces when the series are on).
+// Forward line sums with far-line series, FP32 and FP64 modes (included by alpha.hip, after
+// the line-sum kernel it replaces when the series are on).
 //
 // A line's term is a rational function of x = f^2 with two poles p (the roots of its
 // denominator x^2 - 2 Ds x + P0, P0 = Ds^2 + h):
@@ -35,7 +35,7 @@
 // per-slot partials the finish kernel reads (one part).
 
 constexpr int kFarTerms = 16;
-// pole ratio of a far line: 1/3 in FP32 mode, 1/8 in FP64 mode (at most 4e-13 of a far
+// pole ratio of a far line: 1/3 in FP32 mode, 1/8 in FP64 mode (at most ~5e-13 of a far
 // line's value there, against the 1e-9 FP64 parity bar)
 template <bool F32>
 __host__ __device__ constexpr double far_ratio() { return F32 ? 1.0 / 3.0 : 0.125; }
@@ -102,7 +102,8 @@ __device__ inline void pole_moduli(double Ds, double h, double &rs, double &rb) 
     }
 }
 
-// the segment's coefficient deriver(s) for a layer and their FP32 weight scales
+// the segment's coefficient deriver(s) for a layer and their FP32 weight scales (FP64 mode
+// uses none)
 template <class F>
 __device__ __forceinline__ void with_prep(const AlphaArgs &A, const LayerT<double> &L, int seg, F &&f) {
     const DevBlock *blk = A.cat.blk;
