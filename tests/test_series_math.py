@@ -13,8 +13,11 @@ random lines of every canonical form: Ben-Reuven (G = 0, complex or real poles),
 and Van Vleck-Weisskopf (G > 0)."""
 import numpy as np
 
-TERMS = 16        # the kernels' kFarTerms and kFarRatio
+import pytest
+
+TERMS = 16        # the kernels' kFarTerms and far_ratio<FP32>() / far_ratio<FP64>()
 RATIO = 1.0 / 3.0
+RATIO_FP64 = 0.125
 
 
 def below(a, c, Ds, P0):
@@ -87,27 +90,29 @@ def test_recurrences_match_partial_fractions():
     assert real_poles > 20   # both pole kinds exercised
 
 
-def test_series_sums_match_direct_evaluation():
-    """Sixteen terms at the kernels' ratio 1/3.  The tail is sum_i r_i p_i^16 / (x^16 (x - p_i));
+@pytest.mark.parametrize('ratio', [RATIO, RATIO_FP64])
+def test_series_sums_match_direct_evaluation(ratio):
+    """Sixteen terms at the kernels' ratios, 1/3 (FP32 mode) and 1/8 (FP64 mode, where the bound
+    below is 5.2e-13).  The tail is sum_i r_i p_i^16 / (x^16 (x - p_i));
     for two close poles (narrow lines, or real poles near a double root) the residues cancel and
     the combined coefficients grow like (k + 1) p^k, and a Gross line (a = 0, so m_0 = 0) keeps
     only 15 terms: the tail is then at most sum_{k>=15} (k + 1) (1/3)^k = 1.7e-6 of c / x^2, i.e.
     7.7e-7 of the line's own value at the ratio limit (4e-8 for two well-separated poles)."""
     rng = np.random.default_rng(11)
-    bound = RATIO ** (TERMS - 1) * (TERMS / (1 - RATIO) + RATIO / (1 - RATIO) ** 2)   # 1.7e-6
+    bound = ratio ** (TERMS - 1) * (TERMS / (1 - ratio) + ratio / (1 - ratio) ** 2)   # 1.7e-6 at 1/3
     for a, c, D, g, G in random_lines(rng, 300):
         Ds, h, P0 = shifted(D, g, G)
         p = np.abs(np.roots([1.0, -2.0 * Ds, P0]))
         m = below(a, c, Ds, P0)
         n = above(a, c, Ds, P0)
-        for x in (p.max() / RATIO, 3.0 * p.max() / RATIO):       # below: every |p| <= RATIO x
+        for x in (p.max() / ratio, 3.0 * p.max() / ratio):       # below: every |p| <= ratio x
             direct = (a * x + c) / (x * x - 2 * Ds * x + P0)
             series = np.sum(m * x ** -(np.arange(TERMS) + 1.0))
             size = (abs(a) * x + abs(c)) / abs(x * x - 2 * Ds * x + P0)   # the line's magnitude there
             assert abs(series - direct) <= bound * size + 1e-15 * abs(direct)
         if p.min() == 0.0:
             continue
-        for x in (RATIO * p.min(), RATIO * p.min() / 5.0):        # above: every |p| >= x / RATIO
+        for x in (ratio * p.min(), ratio * p.min() / 5.0):        # above: every |p| >= x / ratio
             direct = (a * x + c) / (x * x - 2 * Ds * x + P0)
             series = np.sum(n * x ** np.arange(TERMS))
             size = (abs(a) * x + abs(c)) / abs(x * x - 2 * Ds * x + P0)
