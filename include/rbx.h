@@ -164,8 +164,8 @@ int rbx_device_count(int *count);
                                     built in: ray major for tables up to 3.5 MiB */
 #define RBX_TUNE_ALPHA_SERIES 4  /* rbx_alpha: 1 sums every line at every frequency; 2 sums per
                                     wavefront tile only the lines whose poles lie within a factor
-                                    3 (FP32) or 8 (FP64) of the tile's x = f^2 range and adds the
-                                    others as 16-term power series (at most 8e-7 / 5e-13 of a far
+                                    3 (FP32) or 6.7 (FP64) of the tile's x = f^2 range and adds the
+                                    others as 16-term power series (at most 8e-7 / 8e-12 of a far
                                     line's value: FP32 tables within 3e-7 of FP64 as the direct
                                     sums, FP64 tables within 1e-11 of the direct FP64 sums); built
                                     in: 2 for requests of 4 or more wavefront tiles (64 or 128

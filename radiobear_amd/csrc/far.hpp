@@ -17,7 +17,7 @@
 // FP32 sums.  On 800 layers x 1024 frequencies (0.3-300 GHz, 128-column tiles) about
 // 30 % of the (line, tile) pairs are neither: only those are summed per column.  (12
 // terms at 1/4 leave more near pairs, 33 %, and were 2 % slower overall.)  FP64 mode uses
-// the same 16 terms at ratio 1/8 (at most ~4e-13 of a far line's value) and sums its near
+// the same 16 terms at ratio 0.15 (at most ~8e-12 of a far line's value) and sums its near
 // lines in FP64 from their (D, a, c, g, G), as the direct FP64 sums do.
 //
 // alpha_far_kernel (block = layer x group of kFarTiles tiles x part of the layer's
@@ -35,10 +35,10 @@
 // per-slot partials the finish kernel reads (one part).
 
 constexpr int kFarTerms = 16;
-// pole ratio of a far line: 1/3 in FP32 mode, 1/8 in FP64 mode (at most ~5e-13 of a far
+// pole ratio of a far line: 1/3 in FP32 mode, 0.15 in FP64 mode (at most ~8e-12 of a far
 // line's value there, against the 1e-9 FP64 parity bar)
 template <bool F32>
-__host__ __device__ constexpr double far_ratio() { return F32 ? 1.0 / 3.0 : 0.125; }
+__host__ __device__ constexpr double far_ratio() { return F32 ? 1.0 / 3.0 : 0.15; }
 constexpr int kFarTiles = 8;                          // tiles per alpha_far_kernel block
 constexpr int kFarSets = 6;                           // h2o, h2s, inversion set 0, set 1, rot + v2, ph3
 constexpr int kFarMom = kFarSets * 2 * kFarTerms;     // moments per (layer, part, tile)

@@ -1,7 +1,7 @@
 """Far-line series (RBX_TUNE_ALPHA_SERIES): the per-tile near sums plus the series of the far
 lines' moments against the direct sums over every line, in both precisions, and against the
 oracle.  Series against direct: 1e-6 in FP32 (16 terms at pole ratio 1/3, plus FP32 rounding),
-1e-11 in FP64 (16 terms at ratio 1/8 leave at most ~5e-13 of a far line's value); against the
+1e-11 in FP64 (16 terms at ratio 0.15 leave at most ~8e-12 of a far line's value); against the
 oracle the precision's own bar (1e-5 FP32, 1e-9 FP64)."""
 import numpy as np
 import pytest
