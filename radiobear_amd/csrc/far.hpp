@@ -527,8 +527,11 @@ template <int K, bool F32>
 __global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(AlphaArgs A) {
     __shared__ NearShared<K, F32> sh;
     // layer fastest: blocks are dealt to the XCDs in turn, so each XCD gets every tile (the
-    // tiles' near fractions differ several-fold) rather than one tile of every layer
-    const int layer = blockIdx.x % A.nl, tile = blockIdx.x / A.nl;
+    // tiles' near fractions differ several-fold) rather than one tile of every layer.  Last
+    // tile first: requests run low to high frequency and line lists grow denser with frequency,
+    // so the heavy tiles start first and the light ones fill the tail of the grid (800 layers
+    // x 1024 frequencies: step 3.92 -> 3.73 ms).
+    const int layer = blockIdx.x % A.nl, tile = A.tiles_total - 1 - static_cast<int>(blockIdx.x / A.nl);
     const int tid = threadIdx.x, lane = tid % 64, wave = __builtin_amdgcn_readfirstlane(tid / 64);
     const int nsplit = A.far_split;
     const LayerT<double> L = layer_of(A, layer);
