@@ -104,17 +104,21 @@ class Problem:
         self.nlines = {k: self.cat.lines(k) for k in range(7)}
         self.ws = None
         self.side = None
+        self.launch = {}    # precision -> launch shape ({field: value}) used by step()
 
     # the three launches of one step, on torch's current stream
-    def launch_alpha(self, prec):
+    def launch_alpha(self, prec, launch=None):
         import ctypes
         from radiobear_amd import _lib
         K = self.K
         out = self.alpha32 if prec == _lib.FP32 else self.alpha64
-        self.ws, nbytes = K.alpha_workspace(self.cat, self.ab, self.nl, self.nf, self.nf_low, prec, reuse=self.ws)
+        ls = _lib.launch_struct(self.launch.get(prec) if launch is None else launch)
+        self.ws, nbytes = K.alpha_workspace(self.cat, self.ab, self.nl, self.nf, self.nf_low, prec, reuse=self.ws,
+                                            launch=ls)
         _lib.check(_lib.lib().rbx_alpha(self.cat.handle, ctypes.byref(self.ab), K._ptr(self.P), K._ptr(self.T),
                                         K._ptr(self.mix), None, self.nl, K._ptr(self.f), None, self.nf, self.nf_low,
-                                        None, prec, K._ptr(out), None, K._ptr(self.ws), nbytes, K._stream()))
+                                        None, prec, K._ptr(out), None, _lib.launch_ref(ls), K._ptr(self.ws), nbytes,
+                                        K._stream()))
 
     def launch_paths(self):
         import ctypes
@@ -124,23 +128,22 @@ class Problem:
                                            K._ptr(self.b), self.nray, 1, K._ptr(self.hds), self.ld,
                                            K._ptr(self.ns), K._stream()))
 
-    def launch_integrate(self, prec):
+    def launch_integrate(self, prec, launch=None, tau_cut=None, Tb=None):
         from radiobear_amd import _lib
         K = self.K
         a = self.alpha32 if prec == _lib.FP32 else self.alpha64
-        Tb = self.Tb if prec == _lib.FP32 else self.Tb64
-        tau_cut = K.FP32_TAU_CUT if prec == _lib.FP32 else K.FP64_TAU_CUT
+        if Tb is None:
+            Tb = self.Tb if prec == _lib.FP32 else self.Tb64
+        if tau_cut is None:
+            tau_cut = K.FP32_TAU_CUT if prec == _lib.FP32 else K.FP64_TAU_CUT
+        ls = _lib.launch_struct(self.launch.get(prec) if launch is None else launch)
         _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
                                             self.ld, K._ptr(self.ns), self.nray, 0, prec, tau_cut, K._ptr(Tb), None,
-                                            K._stream()))
+                                            _lib.launch_ref(ls), K._stream()))
 
     def launch_integrate_nocut(self, prec):
-        from radiobear_amd import _lib
-        K = self.K
-        a = self.alpha32 if prec == _lib.FP32 else self.alpha64
-        _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
-                                            self.ld, K._ptr(self.ns), self.nray, 0, prec, 0.0, K._ptr(self.Tb), None,
-                                            K._stream()))
+        """The integration without the optical-depth cut, into the FP32 Tb buffer."""
+        self.launch_integrate(prec, tau_cut=0.0, Tb=self.Tb)
 
     def step(self, prec):
         """Absorption and ray paths are independent: the paths (a few hundred
@@ -165,27 +168,30 @@ class Problem:
 
 
 def autotune(prob, prec, args):
-    """Pick the launch shapes of rbx_alpha / rbx_integrate for this shape (outside the timed region)."""
+    """Pick the launch shapes of rbx_alpha / rbx_integrate for this shape (outside the timed
+    region) and keep them in prob.launch[prec] for step()."""
     import torch
-    from radiobear_amd import _lib, autotune as at
+    from radiobear_amd import autotune as at
     if args.no_tune:
-        at.reset()
+        prob.launch[prec] = {}
         return None
     key = '{}:{}x{}x{}:p{}'.format(args.workload, prob.nl, prob.nf, prob.nray, prec)
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         prob.launch_paths()
-        # far-line series (1 off, 2 on), then the tile width and split for that path
-        kf = at.tune('alpha_series', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
-        ka = at.tune('alpha_k', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
-        ks = at.tune('alpha_split', key, lambda: prob.launch_alpha(prec), path=args.tune_cache)
-        ki = at.tune('int_unroll', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
-        km = at.tune('int_map', key, lambda: prob.launch_integrate(prec), path=args.tune_cache)
+        # far-line series (1 off, 2 on), then the tile width and the splits for that path
+        la, rec = at.tune_all(('alpha_series', 'alpha_k'), key, lambda L: prob.launch_alpha(prec, L),
+                              path=args.tune_cache)
+        split = 'far_split' if la['alpha_series'] == 2 else 'alpha_split'
+        la, rec_s = at.tune(split, key, lambda L: prob.launch_alpha(prec, L), la, path=args.tune_cache)
+        li, rec_i = at.tune_all(('int_unroll', 'int_map'), key, lambda L: prob.launch_integrate(prec, L),
+                                path=args.tune_cache)
     torch.cuda.synchronize()
-    return {'alpha_path': 'far-line series' if kf and kf['best'] == 2 else 'direct line sums',
-            'alpha_series_ms': kf['ms'] if kf else None, 'alpha_k': ka['best'], 'alpha_split': ks['best'],
-            'int_unroll': ki['best'], 'int_map': km['best'], 'alpha_k_ms': ka['ms'], 'alpha_split_ms': ks['ms'],
-            'integrate_ms': ki['ms'], 'int_map_ms': km['ms']}
+    prob.launch[prec] = dict(la, **li)
+    return {'alpha_path': 'far-line series' if la['alpha_series'] == 2 else 'direct line sums',
+            'launch': prob.launch[prec], 'alpha_series_ms': rec['alpha_series']['ms'],
+            'alpha_k_ms': rec['alpha_k']['ms'], split + '_ms': rec_s['ms'],
+            'int_unroll_ms': rec_i['int_unroll']['ms'], 'int_map_ms': rec_i['int_map']['ms']}
 
 
 def other_workloads(args):
@@ -193,8 +199,7 @@ def other_workloads(args):
     bound, so its absolute time is the figure) and the image case (workload 2: 400 layers,
     10000 lines, 512 freqs, 64x64 pixels; FP64 and FP32), one graph-replayed step each, with the
     library's built-in launch choices (the knobs tuned for workload 3 are cleared first)."""
-    from radiobear_amd import _lib, autotune as at
-    at.reset()
+    from radiobear_amd import _lib
     out = {}
     for name, precs in (('c1_plain_fp64', ('fp64',)), ('c2_image', ('fp64', 'fp32'))):
         prob = Problem(name)

@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 4
+#define RBX_ABI_VERSION 5
 
 /* status codes */
 #define RBX_OK          0
@@ -151,28 +151,38 @@ int rbx_last_error(char *buf, size_t len);
 /* Number of HIP devices visible to the library (host call, no GPU work). */
 int rbx_device_count(int *count);
 
-/* Launch-parameter overrides (no reference counterpart; used by the autotuner):
- * applies to every later call in the process; value 0 restores the built-in choice. */
-#define RBX_TUNE_ALPHA_K     0   /* frequencies per lane of rbx_alpha: 1, 2 or 4 (built in: 2 from
-                                    256 frequencies up, else 1) */
-#define RBX_TUNE_INT_UNROLL  1   /* path steps per load batch of rbx_integrate: 4 or 8 */
-#define RBX_TUNE_ALPHA_SPLIT 2   /* parts each layer's line sums are split into: 1..16 */
-#define RBX_TUNE_INT_MAP     3   /* rbx_integrate block order: 1 frequency group major (each XCD
-                                    keeps a slice of the table for every ray), 2 ray major (each
-                                    XCD runs every frequency group of its rays: their ds rows are
-                                    read once per ray, the whole table lives in each XCD's L2);
-                                    built in: ray major for tables up to 3.5 MiB */
-#define RBX_TUNE_ALPHA_SERIES 4  /* rbx_alpha: 1 sums every line at every frequency; 2 sums per
-                                    wavefront tile only the lines whose poles lie within a factor
-                                    3 (FP32) or 6.7 (FP64) of the tile's x = f^2 range and adds the
-                                    others as 16-term power series (at most 8e-7 / 8e-12 of a far
-                                    line's value: FP32 tables within 3e-7 of FP64 as the direct
-                                    sums, FP64 tables within 1e-11 of the direct FP64 sums); built
-                                    in: 2 for requests of 4 or more wavefront tiles (64 or 128
-                                    frequencies each) while the workspace (24 B FP32, 40 B FP64
-                                    per line and layer more) stays under 4 GiB, else 1 */
-#define RBX_NUM_TUNE_KEYS    5
-int rbx_tune(int key, int value);
+/* Launch shape of one call (no reference counterpart; radiobear_amd.autotune fills it).
+ * Every entry point that makes a launch choice takes a `const rbx_launch *launch`:
+ * NULL, or a field left 0, means the library's built-in choice for that call's shape.
+ * Nothing is kept between calls: the same arguments give the same plan, so a
+ * workspace sized by rbx_alpha_workspace_size(..., launch, ...) fits rbx_alpha(...,
+ * launch, ...) on any thread. */
+typedef struct rbx_launch {
+    int alpha_k;        /* frequencies per lane of the forward line sums: 1, 2 or 4 (built in:
+                           2 from 256 frequencies up, else 1) */
+    int alpha_series;   /* rbx_alpha: 1 sums every line at every frequency; 2 sums per wavefront
+                           tile only the lines whose poles lie within a factor 3 (FP32) or 6.7
+                           (FP64) of the tile's x = f^2 range and adds the others as 16-term power
+                           series (at most 8e-7 / 8e-12 of a far line's value: FP32 tables within
+                           3e-7 of FP64 as the direct sums, FP64 tables within 1e-11 of the direct
+                           FP64 sums); built in: 2 for requests of 4 or more wavefront tiles while
+                           the workspace (24 B FP32, 40 B FP64 per line and layer more) stays
+                           under 4 GiB, else 1 */
+    int alpha_split;    /* parts each layer's direct line sums are split into: 1..16 (built in: a
+                           model of the grid's last dispatch wave) */
+    int far_split;      /* parts each layer's lines are split into in the series' classification
+                           pass: 1..16 (built in: about 4096 blocks) */
+    int int_unroll;     /* path steps per load batch of the integration: 4 or 8 (built in: 8) */
+    int int_map;        /* integration block order: 1 frequency group major (each XCD keeps a
+                           slice of the table for every ray), 2 ray major (each XCD runs every
+                           frequency group of its rays: their ds rows are read once per ray and
+                           the whole table lives in each XCD's L2); built in: ray major for
+                           tables up to 3.5 MiB */
+    int tan_split;      /* parts each layer's lines are split into by rbx_alpha_tangent: 1..16
+                           (built in: 1 when layers x 64-column tiles fill the device, else
+                           enough parts for about 4 wavefronts per SIMD) */
+    int reserved[9];    /* must be 0 */
+} rbx_launch;
 
 /* ---- line catalogue (device-resident, owned by the library) ---- */
 int rbx_catalog_create(rbx_catalog **out);
@@ -208,12 +218,11 @@ int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
               const double *P, const double *T, const double *mix, const double *cloud, int nl,
               const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
               const double *scale, int precision, void *alpha, double *by_constituent,
-              void *workspace, size_t workspace_bytes, void *stream);
+              const rbx_launch *launch, void *workspace, size_t workspace_bytes, void *stream);
 
-/* Workspace bytes rbx_alpha needs for this shape (host call; depends on the
- * rbx_tune settings in force at the rbx_alpha call). */
+/* Workspace bytes rbx_alpha needs for this shape and launch (host call). */
 int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
-                             int nf_nh3_low, int precision, size_t *bytes);
+                             int nf_nh3_low, int precision, const rbx_launch *launch, size_t *bytes);
 
 /* ---- reuse of a saved absorption table (Alpha get_alpha='memory'/'file',
  * alpha.py:110-149 and total_layer_alpha alpha.py:151-192) ----
@@ -265,7 +274,7 @@ int rbx_raypaths(const rbx_planet *planet, const double *R, const double *N, int
 int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
                   const double *hds, int ld, const int *nsteps, int nray,
                   int mode, int precision, double tau_cut, double *Tb, double *intW,
-                  void *stream);
+                  const rbx_launch *launch, void *stream);
 
 /* ---- per-point profiles (Brightness.single's .tau, .W, .Tb_lyr, brightness.py:53-120) ----
  * tau, W, Tb_lyr  [nray][nl][nf] float64 at path points 0..nsteps-1 (0 beyond);
@@ -302,7 +311,8 @@ int rbx_raypaths_batch(const rbx_planet *planet, const double *R, const double *
 int rbx_integrate_batch(const void *alpha, int alpha_precision, const double *T, int nl_total, int nf,
                         const int *layer_off, int natm, const double *hds, int ld, const int *nsteps,
                         const int *ray_atm, const int *order, int nray, int mode, int precision,
-                        double tau_cut, double *Tb, double *intW, void *stream);
+                        double tau_cut, double *Tb, double *intW, const rbx_launch *launch,
+                        void *stream);
 /* J [nray][2][ldj][nf] float32, ldj >= the layer count of every atmosphere (rows
  * past a ray's atmosphere are 0); precision as rbx_integrate's (e^-tau evaluation). */
 int rbx_jacobian_batch(const void *alpha, int alpha_precision, const double *dalpha, const double *T,

@@ -7,7 +7,7 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'librbx.so')
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 FP64 = 0
 FP32 = 1
@@ -33,13 +33,9 @@ CLOUDS = ('H2O', 'SOLN', 'NH4SH', 'NH3', 'H2S', 'CH4')               # RBX_CL_* 
 CLOUD_FLAGS = ('ice_p', 'water_p', 'nh4sh_p', 'nh3ice_p', 'h2sice_p', 'ch4')   # clouds_idp other_dict keys
 CO_SHAPES = {'voigt': 0, 'vvw': 1, 'diff': 2}                         # any other coshape: 3
 
-TUNE_ALPHA_K = 0
-TUNE_INT_UNROLL = 1
-TUNE_ALPHA_SPLIT = 2
-TUNE_INT_MAP = 3
-TUNE_ALPHA_SERIES = 4
+LAUNCH_FIELDS = ('alpha_k', 'alpha_series', 'alpha_split', 'far_split', 'int_unroll', 'int_map', 'tan_split')
 
-EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_tune', 'rbx_catalog_create',
+EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
            'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles',
            'rbx_jacobian', 'rbx_raypaths_batch', 'rbx_integrate_batch', 'rbx_jacobian_batch',
@@ -64,6 +60,27 @@ class Absorbers(ctypes.Structure):
                 ('clouds', ctypes.c_int), ('cloud_types', ctypes.c_int)]
 
 
+class Launch(ctypes.Structure):
+    """rbx_launch: the launch shape of one call (0 = the library's built-in choice)."""
+    _fields_ = [(n, ctypes.c_int) for n in LAUNCH_FIELDS] + [('reserved', ctypes.c_int * 9)]
+
+
+def launch_struct(launch):
+    """Launch from None / a {field: value} dict / a Launch (None stays None: all built in)."""
+    if launch is None or isinstance(launch, Launch):
+        return launch
+    bad = set(launch) - set(LAUNCH_FIELDS)
+    if bad:
+        raise ValueError('unknown launch fields {}'.format(sorted(bad)))
+    return Launch(**{k: int(v) for k, v in launch.items()})
+
+
+def launch_ref(launch):
+    """ctypes argument for a `const rbx_launch *` (NULL for None)."""
+    ls = launch_struct(launch)
+    return None if ls is None else ctypes.byref(ls)
+
+
 class PlanetDesc(ctypes.Structure):
     _fields_ = [('req', ctypes.c_double), ('rpol', ctypes.c_double),
                 ('orientation', ctypes.c_double * 2), ('gtype', ctypes.c_int),
@@ -73,23 +90,23 @@ class PlanetDesc(ctypes.Structure):
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
+_LP = ctypes.POINTER(Launch)
 _SIGS = {
     'rbx_abi_version': (_I, []),
     'rbx_last_error': (_I, [ctypes.c_char_p, ctypes.c_size_t]),
     'rbx_device_count': (_I, [ctypes.POINTER(_I)]),
-    'rbx_tune': (_I, [_I, _I]),
     'rbx_catalog_create': (_I, [ctypes.POINTER(_VP)]),
     'rbx_catalog_set_lines': (_I, [_VP, _I, _I, ctypes.POINTER(ctypes.POINTER(_D))]),
     'rbx_catalog_lines': (_I, [_VP, _I, ctypes.POINTER(_I)]),
     'rbx_catalog_destroy': (_I, [_VP]),
     'rbx_alpha': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
-                       _VP, _I, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
-    'rbx_alpha_workspace_size': (_I, [_VP, ctypes.POINTER(Absorbers), _I, _I, _I, _I,
+                       _VP, _I, _VP, _VP, _LP, _VP, ctypes.c_size_t, _VP]),
+    'rbx_alpha_workspace_size': (_I, [_VP, ctypes.POINTER(Absorbers), _I, _I, _I, _I, _LP,
                                       ctypes.POINTER(ctypes.c_size_t)]),
     'rbx_alpha_tangent': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
                                _VP, _I, _VP, _VP, _VP]),
     'rbx_raypaths': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _I, _VP, _I, _VP, _VP]),
-    'rbx_integrate': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _VP]),
+    'rbx_integrate': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _LP, _VP]),
     'rbx_integrate_profiles': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
     'rbx_jacobian': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP]),
     'rbx_alpha_rescale': (_I, [_VP, _I, _I, _I, _VP, _I, _VP, _VP]),
@@ -97,7 +114,7 @@ _SIGS = {
     'rbx_raypaths_batch': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _VP, _I, _VP, _I, _VP, _I,
                                 _VP, _VP]),
     'rbx_integrate_batch': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _I, _I, _I, _D, _VP,
-                                 _VP, _VP]),
+                                 _VP, _LP, _VP]),
     'rbx_jacobian_batch': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _I, _I, _D, _VP,
                                 _I, _VP, _VP]),
 }

@@ -1,25 +1,28 @@
-"""Launch-shape autotuner for the absorption and integration kernels.
+"""Launch-shape autotuner for the absorption, tangent and integration kernels.
 
-For a problem shape it times each candidate launch configuration on the
-device (HIP events on the launch stream, after a warm-up), keeps the fastest
-and pins it with ``rbx_tune``.  Results are cached per shape in memory and,
-optionally, in a JSON file so a production run tunes once.
+For a problem shape it times each candidate value of one launch knob on the
+device (HIP events on the launch stream, after a warm-up) and returns the
+launch shape with the fastest one.  Nothing is set in the library: a launch
+shape is a plain {field: value} dict (the fields of ``rbx_launch``) that the
+caller passes to every call it is meant for, so calls with different shapes on
+other threads or streams are not affected.  Results are cached per shape in
+memory and, optionally, in a JSON file so a production run tunes once.
 
-Knobs:
-* ``alpha_k``     frequencies per lane of rbx_alpha (1, 2, 4): wider tiles
-                  amortise the LDS reads of each line over more frequencies,
-                  narrower tiles waste fewer lanes and give more blocks for
-                  the tail of the grid;
-* ``alpha_split`` parts each layer's line sums are split into (0 = the
-                  library's model of the grid tail, or 1, 2, 3, 4, 6, 8): more parts
-                  give more, shorter blocks, so the last wave of blocks idles less;
-* ``int_unroll``  path steps per load batch of rbx_integrate (4, 8);
-* ``int_map``     block order of rbx_integrate: 1 frequency group major, 2 ray major
-                  (every frequency group of a ray block on one XCD);
+Knobs (0 = the library's built-in choice):
+* ``alpha_k``      frequencies per lane of rbx_alpha (1, 2, 4): wider tiles
+                   amortise the LDS reads of each line over more frequencies,
+                   narrower tiles waste fewer lanes and give more blocks for
+                   the tail of the grid;
 * ``alpha_series`` line sums of rbx_alpha: 1 every line at every frequency, 2 per
-                  wavefront tile only the lines near it, the others as power series
-                  (the built-in choice takes 2 from 4 tiles up while the workspace
-                  stays under 4 GiB).
+                   wavefront tile only the lines near it, the others as power series;
+* ``alpha_split``  parts each layer's direct line sums are split into: more parts
+                   give more, shorter blocks, so the last wave of blocks idles less;
+* ``far_split``    parts of each layer's lines in the series' classification pass;
+* ``tan_split``    parts of each layer's lines in rbx_alpha_tangent (small layer
+                   counts need several parts to fill the device);
+* ``int_unroll``   path steps per load batch of the integration (4, 8);
+* ``int_map``      block order of the integration: 1 frequency group major, 2 ray
+                   major (every frequency group of a ray block on one XCD).
 """
 import json
 import os
@@ -28,20 +31,10 @@ import torch
 
 from . import _lib
 
-CANDIDATES = {'alpha_k': (1, 2, 4), 'alpha_split': (0, 1, 2, 3, 4, 6, 8), 'int_unroll': (4, 8), 'int_map': (1, 2),
-              'alpha_series': (1, 2)}
-_KEYS = {'alpha_k': _lib.TUNE_ALPHA_K, 'alpha_split': _lib.TUNE_ALPHA_SPLIT, 'int_unroll': _lib.TUNE_INT_UNROLL,
-         'int_map': _lib.TUNE_INT_MAP, 'alpha_series': _lib.TUNE_ALPHA_SERIES}
+CANDIDATES = {'alpha_k': (1, 2, 4), 'alpha_series': (1, 2), 'alpha_split': (0, 1, 2, 3, 4, 6, 8),
+              'far_split': (0, 1, 2, 4, 8), 'tan_split': (0, 1, 2, 4, 8, 16), 'int_unroll': (4, 8),
+              'int_map': (1, 2)}
 _cache = {}
-
-
-def set_knob(name, value):
-    _lib.check(_lib.lib().rbx_tune(_KEYS[name], int(value)))
-
-
-def reset():
-    for name in _KEYS:
-        set_knob(name, 0)
 
 
 def time_call(fn, reps=3, warmup=1):
@@ -58,33 +51,49 @@ def time_call(fn, reps=3, warmup=1):
     return start.elapsed_time(stop) / reps
 
 
-def tune(name, shape_key, fn, reps=3, path=None):
-    """Pick the fastest candidate of knob `name` for fn() (which launches the kernel)."""
-    key = '{}:{}'.format(name, shape_key)
-    if key in _cache:
-        set_knob(name, _cache[key]['best'])
-        return _cache[key]
+def _load(path):
     if path and os.path.exists(path):
         with open(path) as fp:
-            disk = json.load(fp)
-        if key in disk:
-            _cache[key] = disk[key]
-            set_knob(name, disk[key]['best'])
-            return disk[key]
-    times = {}
-    for v in CANDIDATES[name]:
-        set_knob(name, v)
-        times[v] = time_call(fn, reps=reps)
-    best = min(times, key=times.get)
-    set_knob(name, best)
-    rec = {'best': best, 'ms': {str(k): t for k, t in times.items()}}
+            return json.load(fp)
+    return {}
+
+
+def tune(name, shape_key, fn, launch=None, reps=3, path=None, candidates=None):
+    """Time fn(launch) for every candidate value of knob `name` (the other fields as in
+    `launch`) and return (launch with the fastest value of `name`, record).
+
+    fn(launch) must launch the kernel with the given {field: value} dict.  The record
+    holds 'best' and the time of each candidate in ms ('ms')."""
+    if name not in CANDIDATES:
+        raise ValueError('unknown launch knob {}'.format(name))
+    base = dict(launch or {})
+    key = '{}:{}:{}'.format(name, shape_key, json.dumps(base, sort_keys=True))
+    rec = _cache.get(key) or _load(path).get(key)
+    if rec is None:
+        times = {}
+        for v in (candidates or CANDIDATES[name]):
+            trial = dict(base, **{name: v})
+            times[v] = time_call(lambda: fn(trial), reps=reps)
+        best = min(times, key=times.get)
+        rec = {'best': best, 'ms': {str(k): t for k, t in times.items()}}
+        if path:
+            disk = _load(path)
+            disk[key] = rec
+            with open(path, 'w') as fp:
+                json.dump(disk, fp, indent=1, sort_keys=True)
     _cache[key] = rec
-    if path:
-        disk = {}
-        if os.path.exists(path):
-            with open(path) as fp:
-                disk = json.load(fp)
-        disk[key] = rec
-        with open(path, 'w') as fp:
-            json.dump(disk, fp, indent=1, sort_keys=True)
-    return rec
+    return dict(base, **{name: rec['best']}), rec
+
+
+def tune_all(names, shape_key, fn, launch=None, reps=3, path=None):
+    """tune() each knob of `names` in turn, each starting from the shape the previous ones
+    chose: (launch, {name: record})."""
+    recs = {}
+    for n in names:
+        launch, recs[n] = tune(n, shape_key, fn, launch, reps=reps, path=path)
+    return launch, recs
+
+
+def launch_struct(launch):
+    """The ctypes rbx_launch of a {field: value} dict (None: built in)."""
+    return _lib.launch_struct(launch)

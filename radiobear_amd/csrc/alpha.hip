@@ -1328,11 +1328,11 @@ static int fill_args(AlphaArgs &A, const rbx_catalog *cat, const rbx_absorbers *
 }
 
 
-// frequencies per lane of the forward kernel: rbx_tune(RBX_TUNE_ALPHA_K), else 2 for wide
-// requests (each staged line is read once for two columns; on 800 layers x 1024 frequencies
-// 4.2 ms against 4.8 ms with one) and 1 below 256 frequencies (fewer idle lanes)
-static int alpha_tile_k(int nf) {
-    const int k = tune_get(RBX_TUNE_ALPHA_K);
+// frequencies per lane of the forward kernel: launch.alpha_k, else 2 for wide requests (each
+// staged line is read once for two columns; on 800 layers x 1024 frequencies 4.2 ms against
+// 4.8 ms with one) and 1 below 256 frequencies (fewer idle lanes)
+static int alpha_tile_k(int nf, const rbx_launch &L) {
+    const int k = L.alpha_k;
     if (k == 1 || k == 2 || k == 4) return k;
     return nf >= 256 ? 2 : 1;
 }
@@ -1369,10 +1369,10 @@ static int resident_blocks(int K, int precision, int &out) {
     return RBX_OK;
 }
 
-// Parts per layer of the line sums: rbx_tune(RBX_TUNE_ALPHA_SPLIT), else the split
-// that minimises (dispatch rounds of equal blocks) x (staging rounds per block + 0.2,
-// the block's own set-up), i.e. the grid's tail.
-static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
+// Parts per layer of the line sums: launch.alpha_split, else the split that minimises
+// (dispatch rounds of equal blocks) x (staging rounds per block + 0.2, the block's own
+// set-up), i.e. the grid's tail.
+static int alpha_split(const AlphaArgs &A, const rbx_launch &L, int K, int precision, int &nsplit) {
     const DevBlock *blk = A.cat.blk;
     const bool hs = nh3_has_hs(A.abs.nh3);
     const int R = (A.abs.h2o ? rounds_of(blk[RBX_LINES_H2O].n) : 0) +
@@ -1381,9 +1381,8 @@ static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
                             rounds_of(blk[RBX_LINES_NH3_V2].n)
                       : 0) +
                   (A.abs.ph3 ? rounds_of(blk[RBX_LINES_PH3].n) : 0);
-    const int forced = tune_get(RBX_TUNE_ALPHA_SPLIT);
-    if (forced > 0) {
-        nsplit = std::min(forced, kMaxSplit);
+    if (L.alpha_split > 0) {
+        nsplit = std::min(L.alpha_split, kMaxSplit);
         return RBX_OK;
     }
     nsplit = 1;
@@ -1404,15 +1403,15 @@ static int alpha_split(const AlphaArgs &A, int K, int precision, int &nsplit) {
     return RBX_OK;
 }
 
-// far-line series (far.hpp): rbx_tune(RBX_TUNE_ALPHA_SERIES) 1 off, 2 on; built in on for
+// far-line series (far.hpp): launch.alpha_series 1 off, 2 on; built in on for
 // requests of at least kSeriesMinTiles wavefront tiles while the workspace stays under
 // kSeriesMaxWorkspace.  (With fewer tiles few (line, tile) pairs are far and the extra pass
 // costs more than it saves: one 20-frequency tile of 150 layers took 0.37 ms per step against
 // 0.21 ms with the direct sums.)
 constexpr size_t kSeriesMaxWorkspace = size_t(4) << 30;
 constexpr int kSeriesMinTiles = 4;
-static bool series_on(const AlphaArgs &A) {
-    const int t = tune_get(RBX_TUNE_ALPHA_SERIES);
+static bool series_on(const AlphaArgs &A, const rbx_launch &L) {
+    const int t = L.alpha_series;
     return t == 2 || (t != 1 && A.tiles_total >= kSeriesMinTiles);
 }
 
@@ -1437,9 +1436,9 @@ static void far_layout(AlphaArgs &A) {
     A.far_words = at;
 }
 
-// parts per layer of the classification pass: rbx_tune(RBX_TUNE_ALPHA_SPLIT), else enough
-// blocks (layers x tile groups x parts) to fill the device a few times over
-static int far_split(const AlphaArgs &A) {
+// parts per layer of the classification pass: launch.far_split, else enough blocks
+// (layers x tile groups x parts) to fill the device a few times over
+static int far_split(const AlphaArgs &A, const rbx_launch &L) {
     const DevBlock *blk = A.cat.blk;
     const bool hs = nh3_has_hs(A.abs.nh3);
     const int R = (A.abs.h2o ? rounds_of(blk[RBX_LINES_H2O].n) : 0) +
@@ -1448,8 +1447,7 @@ static int far_split(const AlphaArgs &A) {
                             rounds_of(blk[RBX_LINES_NH3_V2].n)
                       : 0) +
                   (A.abs.ph3 ? rounds_of(blk[RBX_LINES_PH3].n) : 0);
-    const int forced = tune_get(RBX_TUNE_ALPHA_SPLIT);
-    if (forced > 0) return std::min(forced, kMaxSplit);
+    if (L.far_split > 0) return std::min(L.far_split, kMaxSplit);
     const long long items = static_cast<long long>(A.nl) * ((A.tiles_total + kFarTiles - 1) / kFarTiles);
     const long long want = (4096 + items - 1) / items;
     return static_cast<int>(std::max(1LL, std::min<long long>(want, std::min(kMaxSplit, std::max(R, 1)))));
@@ -1519,83 +1517,43 @@ static int launch_series(AlphaArgs &A, int precision, hipStream_t st) {
 }
 
 // workspace of one rbx_alpha call: the tile width, split and path it will use
-static int plan_alpha(AlphaArgs &A, int precision, int &K, int &nsplit, bool &series, size_t &bytes) {
-    K = alpha_tile_k(A.nf);
+static int plan_alpha(AlphaArgs &A, const rbx_launch &L, int precision, int &K, int &nsplit, bool &series,
+                      size_t &bytes) {
+    K = alpha_tile_k(A.nf, L);
     set_wave_tiles(A, K);
-    series = series_on(A);
+    series = series_on(A, L);
     if (series) {
         far_layout(A);
         series = A.far_words > 0;   // no line lists selected: the plain path (one part)
     }
     if (series) {
-        A.far_split = far_split(A);
+        A.far_split = far_split(A, L);
         nsplit = 1;
         // the staged lines take 24 B (FP32) or 40 B (FP64) per line and layer: past
-        // kSeriesMaxWorkspace the built-in choice is the direct sums (rbx_tune(RBX_TUNE_ALPHA_SERIES,
-        // 2) forces the series)
-        if (tune_get(RBX_TUNE_ALPHA_SERIES) != 2 && alpha_ws_bytes(A, precision, 1, true) > kSeriesMaxWorkspace)
+        // kSeriesMaxWorkspace the built-in choice is the direct sums (launch.alpha_series = 2
+        // forces the series)
+        if (L.alpha_series != 2 && alpha_ws_bytes(A, precision, 1, true) > kSeriesMaxWorkspace)
             series = false;
     }
     if (!series) {
-        int rc = alpha_split(A, K, precision, nsplit);
+        int rc = alpha_split(A, L, K, precision, nsplit);
         if (rc) return rc;
     }
     bytes = alpha_ws_bytes(A, precision, nsplit, series);
     return RBX_OK;
 }
 
-}  // namespace rbx
-
-extern "C" int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
-                                        int nf_nh3_low, int precision, size_t *bytes) {
-    RBX_CHECK_ARG(bytes, "rbx_alpha_workspace_size: null output");
-    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_workspace_size: bad precision %d",
-                  precision);
-    RBX_CHECK_ARG(cat && absorbers, "rbx_alpha_workspace_size: null catalogue or absorber selection");
-    RBX_CHECK_ARG(nl > 0 && nf > 0 && static_cast<long long>(nl) * ((nf + 63) / 64) < (1LL << 31),
-                  "rbx_alpha_workspace_size: nl=%d nf=%d", nl, nf);
-    RBX_CHECK_ARG(nf_nh3_low >= 0 && nf_nh3_low <= nf, "rbx_alpha_workspace_size: nf_nh3_low=%d", nf_nh3_low);
-    int rc = rbx::check_catalog(cat, *absorbers, "rbx_alpha_workspace_size");
-    if (rc) return rc;
-    rbx::AlphaArgs A{};
-    A.cat = cat->dev;
-    A.abs = *absorbers;
-    A.nl = nl;
-    A.nf = nf;
-    A.nf_low = nf_nh3_low;
-    int K = 1, nsplit = 1;
-    bool series = false;
-    return rbx::plan_alpha(A, precision, K, nsplit, series, *bytes);
-}
-
-extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P, const double *T,
-                         const double *mix, const double *cloud, int nl, const double *freq, const double *freq_nh3,
-                         int nf,
-                         int nf_nh3_low, const double *scale, int precision, void *alpha, double *by_constituent,
-                         void *workspace, size_t workspace_bytes, void *stream) {
-    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha: bad precision %d", precision);
-    rbx::AlphaArgs A{};
-    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, cloud, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
-                            by_constituent, "rbx_alpha");
-    if (rc) return rc;
-    int K = 1, nsplit = 1;
-    bool series = false;
-    size_t need = 0;
-    rc = rbx::plan_alpha(A, precision, K, nsplit, series, need);
-    if (rc) return rc;
-    RBX_CHECK_ARG(workspace, "rbx_alpha: null workspace (rbx_alpha_workspace_size gives its size)");
-    RBX_CHECK_ARG(workspace_bytes >= need, "rbx_alpha: workspace of %zu bytes, this call needs %zu", workspace_bytes,
-                  need);
-    RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha: workspace not 8-byte aligned");
+// Lay the planned workspace out and launch (rbx_alpha and rbx_alpha_noalloc).
+static int run_planned(AlphaArgs &A, int precision, int K, int nsplit, bool series, void *workspace, hipStream_t st) {
+    const int nl = A.nl, nf = A.nf;
     A.ws = static_cast<double *>(workspace);
-    A.ws_base = A.ws + static_cast<size_t>(rbx::kSlots) * nsplit * nl * nf;
+    A.ws_base = A.ws + static_cast<size_t>(kSlots) * nsplit * nl * nf;
     A.ws_extra = A.ws_base + static_cast<size_t>(nsplit) * nl;
-    hipStream_t st = static_cast<hipStream_t>(stream);
     if (series) {
         const size_t plane = static_cast<size_t>(nl) * nf;
         const size_t lines = static_cast<size_t>(nl) * A.far_words * 32;
         const uintptr_t at = reinterpret_cast<uintptr_t>(
-            A.ws_extra + (rbx::has_extra(A.abs) ? static_cast<size_t>(rbx::kNumExtra) * plane : 0));
+            A.ws_extra + (has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0));
         const uintptr_t lines_at = (at + 15) / 16 * 16;
         double *after;
         if (precision == RBX_FP32) {
@@ -1607,20 +1565,72 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
             after = A.near_c64 + 5 * lines;
         }
         A.far_mom = after;
-        A.far_base = A.far_mom + static_cast<size_t>(nl) * A.far_split * A.tiles_total * rbx::kFarMom;
+        A.far_base = A.far_mom + static_cast<size_t>(nl) * A.far_split * A.tiles_total * kFarMom;
         A.far_scale = A.far_base + static_cast<size_t>(A.far_split) * nl;
-        A.near = reinterpret_cast<unsigned int *>(A.far_scale + static_cast<size_t>(nl) * rbx::kSegs * 2);
+        A.near = reinterpret_cast<unsigned int *>(A.far_scale + static_cast<size_t>(nl) * kSegs * 2);
         switch (K) {
-            case 4: return rbx::launch_series<4>(A, precision, st);
-            case 2: return rbx::launch_series<2>(A, precision, st);
-            default: return rbx::launch_series<1>(A, precision, st);
+            case 4: return launch_series<4>(A, precision, st);
+            case 2: return launch_series<2>(A, precision, st);
+            default: return launch_series<1>(A, precision, st);
         }
     }
     switch (K) {
-        case 4: return rbx::launch_alpha<4>(A, precision, nsplit, st);
-        case 2: return rbx::launch_alpha<2>(A, precision, nsplit, st);
-        default: return rbx::launch_alpha<1>(A, precision, nsplit, st);
+        case 4: return launch_alpha<4>(A, precision, nsplit, st);
+        case 2: return launch_alpha<2>(A, precision, nsplit, st);
+        default: return launch_alpha<1>(A, precision, nsplit, st);
     }
+}
+
+}  // namespace rbx
+
+extern "C" int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
+                                        int nf_nh3_low, int precision, const rbx_launch *launch, size_t *bytes) {
+    RBX_CHECK_ARG(bytes, "rbx_alpha_workspace_size: null output");
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_workspace_size: bad precision %d",
+                  precision);
+    RBX_CHECK_ARG(cat && absorbers, "rbx_alpha_workspace_size: null catalogue or absorber selection");
+    RBX_CHECK_ARG(nl > 0 && nf > 0 && static_cast<long long>(nl) * ((nf + 63) / 64) < (1LL << 31),
+                  "rbx_alpha_workspace_size: nl=%d nf=%d", nl, nf);
+    RBX_CHECK_ARG(nf_nh3_low >= 0 && nf_nh3_low <= nf, "rbx_alpha_workspace_size: nf_nh3_low=%d", nf_nh3_low);
+    int rc = rbx::check_catalog(cat, *absorbers, "rbx_alpha_workspace_size");
+    if (rc) return rc;
+    rbx_launch L;
+    rc = rbx::read_launch(launch, L, "rbx_alpha_workspace_size");
+    if (rc) return rc;
+    rbx::AlphaArgs A{};
+    A.cat = cat->dev;
+    A.abs = *absorbers;
+    A.nl = nl;
+    A.nf = nf;
+    A.nf_low = nf_nh3_low;
+    int K = 1, nsplit = 1;
+    bool series = false;
+    return rbx::plan_alpha(A, L, precision, K, nsplit, series, *bytes);
+}
+
+extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P, const double *T,
+                         const double *mix, const double *cloud, int nl, const double *freq, const double *freq_nh3,
+                         int nf,
+                         int nf_nh3_low, const double *scale, int precision, void *alpha, double *by_constituent,
+                         const rbx_launch *launch, void *workspace, size_t workspace_bytes, void *stream) {
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha: bad precision %d", precision);
+    rbx::AlphaArgs A{};
+    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, cloud, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
+                            by_constituent, "rbx_alpha");
+    if (rc) return rc;
+    rbx_launch L;
+    rc = rbx::read_launch(launch, L, "rbx_alpha");
+    if (rc) return rc;
+    int K = 1, nsplit = 1;
+    bool series = false;
+    size_t need = 0;
+    rc = rbx::plan_alpha(A, L, precision, K, nsplit, series, need);
+    if (rc) return rc;
+    RBX_CHECK_ARG(workspace, "rbx_alpha: null workspace (rbx_alpha_workspace_size gives its size)");
+    RBX_CHECK_ARG(workspace_bytes >= need, "rbx_alpha: workspace of %zu bytes, this call needs %zu", workspace_bytes,
+                  need);
+    RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha: workspace not 8-byte aligned");
+    return rbx::run_planned(A, precision, K, nsplit, series, workspace, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,

@@ -1,6 +1,5 @@
 // Error handling, version query and the line catalogue of librbx.
 #include <algorithm>
-#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -22,18 +21,28 @@ void set_error(const char *fmt, ...) {
 
 const std::string &last_error() { return g_error; }
 
-static std::atomic<int> g_tune[RBX_NUM_TUNE_KEYS];
-
-int tune_get(int key) { return (key >= 0 && key < RBX_NUM_TUNE_KEYS) ? g_tune[key].load() : 0; }
-
-}  // namespace rbx
-
-extern "C" int rbx_tune(int key, int value) {
-    RBX_CHECK_ARG(key >= 0 && key < RBX_NUM_TUNE_KEYS, "rbx_tune: bad key %d", key);
-    RBX_CHECK_ARG(value >= 0, "rbx_tune: negative value %d", value);
-    rbx::g_tune[key].store(value);
+int read_launch(const rbx_launch *in, rbx_launch &out, const char *who) {
+    out = rbx_launch{};
+    if (in == nullptr) return RBX_OK;
+    out = *in;
+    RBX_CHECK_ARG(out.alpha_k == 0 || out.alpha_k == 1 || out.alpha_k == 2 || out.alpha_k == 4,
+                  "%s: launch.alpha_k=%d (0, 1, 2 or 4)", who, out.alpha_k);
+    RBX_CHECK_ARG(out.alpha_series >= 0 && out.alpha_series <= 2, "%s: launch.alpha_series=%d (0, 1 or 2)", who,
+                  out.alpha_series);
+    RBX_CHECK_ARG(out.alpha_split >= 0 && out.alpha_split <= kMaxLaunchSplit, "%s: launch.alpha_split=%d (0..%d)",
+                  who, out.alpha_split, kMaxLaunchSplit);
+    RBX_CHECK_ARG(out.far_split >= 0 && out.far_split <= kMaxLaunchSplit, "%s: launch.far_split=%d (0..%d)", who,
+                  out.far_split, kMaxLaunchSplit);
+    RBX_CHECK_ARG(out.int_unroll == 0 || out.int_unroll == 4 || out.int_unroll == 8,
+                  "%s: launch.int_unroll=%d (0, 4 or 8)", who, out.int_unroll);
+    RBX_CHECK_ARG(out.int_map >= 0 && out.int_map <= 2, "%s: launch.int_map=%d (0, 1 or 2)", who, out.int_map);
+    RBX_CHECK_ARG(out.tan_split >= 0 && out.tan_split <= kMaxLaunchSplit, "%s: launch.tan_split=%d (0..%d)", who,
+                  out.tan_split, kMaxLaunchSplit);
+    for (int r : out.reserved) RBX_CHECK_ARG(r == 0, "%s: launch.reserved must be 0", who);
     return RBX_OK;
 }
+
+}  // namespace rbx
 
 using rbx::kNumCols;
 

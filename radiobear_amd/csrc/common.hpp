@@ -32,8 +32,9 @@ const std::string &last_error();
         }                                                                            \
     } while (0)
 
-// launch-parameter overrides set through rbx_tune (0 = built-in choice)
-int tune_get(int key);
+// Launch shape of one call: `in` (NULL = all built in) checked and copied to `out`.
+constexpr int kMaxLaunchSplit = 16;
+int read_launch(const rbx_launch *in, rbx_launch &out, const char *who);
 
 inline int launch_status(const char *what) {
     hipError_t e = hipGetLastError();

@@ -11,7 +11,7 @@
 // for a step is one coalesced row read alpha[k][f0..f0+63]; the ray's ds and
 // T[k] are wave-uniform scalar loads.  Blocks map frequency groups to XCDs
 // (block b -> group b % ngroups) so each XCD's L2 holds a slice of the table, or,
-// ray-major (rbx_tune RBX_TUNE_INT_MAP 2), every group of a ray block to one XCD so the
+// ray-major (launch.int_map 2), every group of a ray block to one XCD so the
 // ray's ds row is fetched once and the whole table stays in that XCD's L2.
 #include "common.hpp"
 
@@ -475,13 +475,13 @@ static void fill_int_args(IntArgs &A, const void *alpha, int alpha_precision, co
 
 }  // namespace rbx
 
-static int launch_integrate(rbx::IntArgs A, void *stream) {
+static int launch_integrate(rbx::IntArgs A, const rbx_launch &L, void *stream) {
     const long long rblocks = (A.nray + rbx::kWaves - 1) / rbx::kWaves;
     // ray major pads the ray blocks to a multiple of the XCD count (padding blocks find no ray).
     // Built-in choice: ray major while the whole table fits one XCD's L2 (4 MiB) beside the
     // streamed ds rows (measured on the 800 x 1024 table: 2.7 ms ray major in FP32, 3.3 ms
     // group major; the 6.5 MB FP64 table is faster group major).
-    const int forced = rbx::tune_get(RBX_TUNE_INT_MAP);
+    const int forced = L.int_map;
     const double table = static_cast<double>(A.nl) * A.nf * (A.f32 ? 4.0 : 8.0);
     A.ray_major = forced ? forced == 2 : table <= 3.5 * 1024 * 1024;
     const long long blocks = static_cast<long long>(A.ngroups) *
@@ -489,7 +489,7 @@ static int launch_integrate(rbx::IntArgs A, void *stream) {
     RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate: too many blocks");
     const dim3 grid(static_cast<unsigned>(blocks)), block(rbx::kWaves * rbx::kLanes);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const bool u4 = rbx::tune_get(RBX_TUNE_INT_UNROLL) == 4;
+    const bool u4 = L.int_unroll == 4;
 #define RBX_INT_LAUNCH(AT, MODE, FAST, U) hipLaunchKernelGGL((rbx::integrate_kernel<AT, MODE, FAST, U>), grid, block, 0, st, A)
     if (A.mode == 1) {
         if (A.f32) RBX_INT_LAUNCH(float, 1, false, 4); else RBX_INT_LAUNCH(double, 1, false, 4);
@@ -512,8 +512,11 @@ static int launch_integrate(rbx::IntArgs A, void *stream) {
 
 extern "C" int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
                              const double *hds, int ld, const int *nsteps, int nray, int mode, int precision,
-                             double tau_cut, double *Tb, double *intW, void *stream) {
+                             double tau_cut, double *Tb, double *intW, const rbx_launch *launch, void *stream) {
     int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    rbx_launch L;
+    rc = rbx::read_launch(launch, L, "rbx_integrate");
     if (rc) return rc;
     RBX_CHECK_ARG(ld >= nl - 1, "rbx_integrate: ld=%d < nl-1=%d", ld, nl - 1);
     RBX_CHECK_ARG(Tb != nullptr, "rbx_integrate: null Tb");
@@ -524,14 +527,18 @@ extern "C" int rbx_integrate(const void *alpha, int alpha_precision, const doubl
     A.mode = mode;
     A.Tb = Tb;
     A.intW = intW;
-    return launch_integrate(A, stream);
+    return launch_integrate(A, L, stream);
 }
 
 extern "C" int rbx_integrate_batch(const void *alpha, int alpha_precision, const double *T, int nl_total, int nf,
                                    const int *layer_off, int natm, const double *hds, int ld, const int *nsteps,
                                    const int *ray_atm, const int *order, int nray, int mode, int precision,
-                                   double tau_cut, double *Tb, double *intW, void *stream) {
+                                   double tau_cut, double *Tb, double *intW, const rbx_launch *launch,
+                                   void *stream) {
     int rc = rbx::check_int_args(alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    rbx_launch L;
+    rc = rbx::read_launch(launch, L, "rbx_integrate_batch");
     if (rc) return rc;
     RBX_CHECK_ARG(layer_off && ray_atm && natm > 0, "rbx_integrate_batch: null offsets / ray map or natm=%d", natm);
     RBX_CHECK_ARG(ld >= 1, "rbx_integrate_batch: ld=%d", ld);
@@ -547,7 +554,7 @@ extern "C" int rbx_integrate_batch(const void *alpha, int alpha_precision, const
     A.layer_off = layer_off;
     A.natm = natm;
     A.order = order;
-    return launch_integrate(A, stream);
+    return launch_integrate(A, L, stream);
 }
 
 static int launch_jacobian(const rbx::JacArgs &Aj, void *stream) {

@@ -197,9 +197,11 @@ def _alpha_inputs(P, T, mix, freqs, scale):
     return P, T, mix, fd, fn, nf_low, sc, nl
 
 
-def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by_constituent=False, cloud=None):
+def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by_constituent=False, cloud=None,
+          launch=None):
     """alpha[nl][nf] (cm^-1) on the device; optionally also [7][nl][nf] per constituent
-    (RBX_C_* rows).  cloud: (6, nl) densities (cloud_table), needed by clouds_idp."""
+    (RBX_C_* rows).  cloud: (6, nl) densities (cloud_table), needed by clouds_idp.
+    launch: launch shape ({field: value} of _lib.LAUNCH_FIELDS or None for the built-in one)."""
     prec = _lib.PRECISIONS[precision]
     P, T, mix, fd, fn, nf_low, sc, nl = _alpha_inputs(P, T, mix, freqs, scale)
     nf = fd.shape[0]
@@ -215,9 +217,11 @@ def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by
     out = torch.empty((nl, nf), dtype=torch.float32 if prec == _lib.FP32 else torch.float64, device=device())
     byc = torch.zeros((len(_lib.CONSTITUENTS), nl, nf), dtype=torch.float64, device=device()) \
         if by_constituent else None
-    ws, nbytes = alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec)
+    ls = _lib.launch_struct(launch)
+    ws, nbytes = alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec, launch=ls)
     check(lib().rbx_alpha(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), _ptr(cl), nl, _ptr(fd),
-                          _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(out), _ptr(byc), _ptr(ws), nbytes, _stream()))
+                          _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(out), _ptr(byc), _lib.launch_ref(ls), _ptr(ws),
+                          nbytes, _stream()))
     return (out, byc) if by_constituent else out
 
 
@@ -236,12 +240,12 @@ def alpha_rescale(absorb, scale=None, precision='fp64'):
     return out
 
 
-def alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec, reuse=None):
-    """Device scratch for rbx_alpha (torch's stream-ordered allocator): (tensor, bytes).
-    `reuse` is returned instead of a new tensor when it is large enough."""
+def alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec, reuse=None, launch=None):
+    """Device scratch for rbx_alpha with this launch shape (torch's stream-ordered allocator):
+    (tensor, bytes).  `reuse` is returned instead of a new tensor when it is large enough."""
     nb = ctypes.c_size_t(0)
     check(lib().rbx_alpha_workspace_size(catalog.handle, ctypes.byref(absorbers), nl, nf, nf_low, prec,
-                                         ctypes.byref(nb)))
+                                         _lib.launch_ref(launch), ctypes.byref(nb)))
     if reuse is not None and reuse.numel() * reuse.element_size() >= nb.value:
         return reuse, nb.value
     return torch.empty((max(1, (nb.value + 7) // 8),), dtype=torch.float64, device=device()), nb.value
@@ -310,7 +314,7 @@ FP32_TAU_CUT = 20.0
 FP64_TAU_CUT = 40.0
 
 
-def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False):
+def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False, launch=None):
     """Tb (nray, nf) float64 from an alpha table (nl, nf) and ray paths."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
     prec = aprec if precision is None else _lib.PRECISIONS[precision]
@@ -323,7 +327,7 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
     iw = torch.empty((nray, nf), dtype=torch.float64, device=device()) if with_intW else None
     check(lib().rbx_integrate(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds), ld,
                               _ptr(nsteps), nray, 1 if disc else 0, prec, float(tau_cut),
-                              _ptr(Tb), _ptr(iw), _stream()))
+                              _ptr(Tb), _ptr(iw), _lib.launch_ref(launch), _stream()))
     return (Tb, iw) if with_intW else Tb
 
 
@@ -458,7 +462,7 @@ def jacobian_batch(alpha_table, dalpha, T, layer_off, ray_atm, hds, nsteps, ldj,
 
 
 def integrate_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, order=None, disc=False, precision=None,
-                    tau_cut=0.0):
+                    tau_cut=0.0, launch=None):
     """Tb (nray, nf) float64 for rays through several atmospheres laid end to end."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
     prec = aprec if precision is None else _lib.PRECISIONS[precision]
@@ -473,7 +477,8 @@ def integrate_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, order=None,
     Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
     check(lib().rbx_integrate_batch(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(off), off.shape[0] - 1,
                                     _ptr(hds), ld, _ptr(nsteps), _ptr(ra), _ptr(od), nray,
-                                    1 if disc else 0, prec, float(tau_cut), _ptr(Tb), None, _stream()))
+                                    1 if disc else 0, prec, float(tau_cut), _ptr(Tb), None, _lib.launch_ref(launch),
+                                    _stream()))
     return Tb
 
 

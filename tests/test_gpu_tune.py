@@ -19,11 +19,10 @@ def K():
     return kernels
 
 
-@pytest.fixture
+@pytest.fixture(scope='module')
 def at(K):
     from radiobear_amd import autotune
-    yield autotune
-    autotune.reset()
+    return autotune
 
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
@@ -43,12 +42,10 @@ def test_alpha_every_tile_width(K, at, raw_lines, jupiter_lines, precision, tol)
     f = np.logspace(np.log10(1.0), np.log10(200.0), 330)
     ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(lines))
     for series in (1, 2):
-        at.set_knob('alpha_series', series)
         out = {}
         for k in at.CANDIDATES['alpha_k']:
-            at.set_knob('alpha_k', k)
             out[k] = K.alpha(cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), f,
-                             precision).double().cpu().numpy()
+                             precision, launch={'alpha_series': series, 'alpha_k': k}).double().cpu().numpy()
             got = out[k].T
             assert np.max(np.abs(got - ref) / np.abs(ref)) < tol, (series, k)
         for k in out:
@@ -74,9 +71,8 @@ def test_alpha_every_split(K, at, raw_lines, jupiter_lines, precision, tol):
     ref = layers.get_layers(list(f), atm.gas, C, layers.jupiter_absorbers(lines))
     out = {}
     for n in (1, 2, 3, 7, 16):
-        at.set_knob('alpha_split', n)
         a, byc = K.alpha(cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), f,
-                         precision, by_constituent=True)
+                         precision, by_constituent=True, launch={'alpha_series': 1, 'alpha_split': n})
         out[n] = a.double().cpu().numpy()
         assert np.max(np.abs(out[n].T - ref) / np.abs(ref)) < tol, n
         assert np.allclose(byc.sum(axis=0).cpu().numpy(), out[n], rtol=1e-6 if precision == 'fp32' else 1e-12)
@@ -102,9 +98,9 @@ def test_integrate_every_unroll(K, at, precision):
     hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
     res = {}
     for u in at.CANDIDATES['int_unroll']:
-        at.set_knob('int_unroll', u)
-        a = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=0.0).cpu().numpy()
-        b = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=50.0).cpu().numpy()
+        L = {'int_unroll': u}
+        a = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=0.0, launch=L).cpu().numpy()
+        b = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=50.0, launch=L).cpu().numpy()
         res[u] = (a, b)
     base = res[at.CANDIDATES['int_unroll'][0]][0]
     ok = ~np.isnan(base)
@@ -134,12 +130,9 @@ def test_integrate_every_block_order(K, at, precision):
         rays = synth.random_disc_rays(nray, 12 + nray)
         hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
         out = {}
-        try:
-            for m in at.CANDIDATES['int_map']:
-                at.set_knob('int_map', m)
-                out[m] = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=50.0).cpu().numpy()
-        finally:
-            at.set_knob('int_map', 0)
+        for m in at.CANDIDATES['int_map']:
+            out[m] = K.integrate(tab, atm.T, hds, ns, precision=precision, tau_cut=50.0,
+                                 launch={'int_map': m}).cpu().numpy()
         a, b = out[1], out[2]
         assert np.array_equal(np.isnan(a), np.isnan(b))
         assert np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)]), nray
@@ -147,8 +140,78 @@ def test_integrate_every_block_order(K, at, precision):
 
 def test_tune_picks_a_candidate_and_caches(K, at, tmp_path):
     calls = []
-    rec = at.tune('int_unroll', 'test-shape', lambda: calls.append(1), reps=1, path=str(tmp_path / 't.json'))
+    L, rec = at.tune('int_unroll', 'test-shape', lambda L: calls.append(dict(L)), {'int_map': 2}, reps=1,
+                     path=str(tmp_path / 't.json'))
     assert rec['best'] in at.CANDIDATES['int_unroll']
+    assert L == {'int_map': 2, 'int_unroll': rec['best']}
+    assert {c['int_unroll'] for c in calls} == set(at.CANDIDATES['int_unroll'])
+    assert all(c['int_map'] == 2 for c in calls)
     n = len(calls)
-    again = at.tune('int_unroll', 'test-shape', lambda: calls.append(1), reps=1, path=str(tmp_path / 't.json'))
-    assert again == rec and len(calls) == n
+    L2, again = at.tune('int_unroll', 'test-shape', lambda L: calls.append(dict(L)), {'int_map': 2}, reps=1,
+                        path=str(tmp_path / 't.json'))
+    assert again == rec and L2 == L and len(calls) == n
+
+
+def test_far_split_parts(K, at, raw_lines):
+    """The series' classification pass split into 1..16 parts per layer: the same table up to
+    the order the parts' moments are added in (FP64 tables to 1e-12, FP32 to 1e-6)."""
+    atm = synth.jupiter_like(29)
+    cat = K.jupiter_catalog(raw_lines, h2o=synth.h2o_lines(500, 4))
+    f = np.geomspace(0.5, 300.0, 300)
+    ab = K.absorbers_struct(JUPITER_ALPHA)
+    for prec, tol in (('fp64', 1e-12), ('fp32', 1e-6)):
+        out = {n: K.alpha(cat, ab, atm.P, atm.T, K.mix_table(atm.gas, C), f, prec,
+                          launch={'alpha_series': 2, 'far_split': n}).double().cpu().numpy() for n in (1, 3, 16)}
+        for n in out:
+            assert np.max(np.abs(out[n] - out[1]) / np.abs(out[1])) < tol, (prec, n)
+
+
+def test_two_threads_with_different_launch_shapes(K, raw_lines):
+    """Launch shapes are per call: two threads, each on its own stream, computing tables with
+    different shapes (direct sums split 3 ways at 1 frequency per lane against the far-line
+    series at 4 per lane, and different integration block orders) get the results each shape
+    gives when run alone, call after call."""
+    import threading
+
+    import torch
+    atm = synth.jupiter_like(33)
+    cat = K.jupiter_catalog(raw_lines, h2o=synth.h2o_lines(400, 6))
+    ab = K.absorbers_struct(JUPITER_ALPHA)
+    f = np.geomspace(1.0, 200.0, 520)
+    mix = K.mix_table(atm.gas, C)
+    pl = K.planet_desc(atm.Req, atm.Rpol)
+    hds, ns = K.raypaths(pl, atm.R, atm.N, synth.random_disc_rays(90, 21))
+    shapes = [{'alpha_series': 1, 'alpha_split': 3, 'alpha_k': 1, 'int_map': 1, 'int_unroll': 4},
+              {'alpha_series': 2, 'alpha_k': 4, 'far_split': 5, 'int_map': 2, 'int_unroll': 8}]
+
+    def run(L):
+        a = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp64', launch=L)
+        return a, K.integrate(a, atm.T, hds, ns, precision='fp64', tau_cut=40.0, launch=L)
+
+    alone = []
+    for L in shapes:
+        a, tb = run(L)
+        alone.append((a.cpu().numpy(), tb.cpu().numpy()))
+    torch.cuda.synchronize()
+    errors = []
+
+    def worker(i):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for _ in range(6):
+                    a, tb = run(shapes[i])
+                    s.synchronize()
+                    if not (np.array_equal(a.cpu().numpy(), alone[i][0]) and
+                            np.array_equal(tb.cpu().numpy(), alone[i][1], equal_nan=True)):
+                        errors.append(i)
+        except Exception as e:   # reported below: a thread's exception would be lost
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert errors == []
+    assert np.max(np.abs(alone[0][0] - alone[1][0]) / np.abs(alone[1][0])) < 1e-11

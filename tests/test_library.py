@@ -34,7 +34,7 @@ def test_argument_errors_are_reported_without_gpu_work():
     assert rc == -1
     assert 'null' in _lib.last_error() or 'layers' in _lib.last_error()
     with pytest.raises(_lib.RbxError) as e:
-        _lib.check(lib.rbx_integrate(None, 0, None, 10, 5, None, 9, None, 1, 0, 0, 0.0, None, None, None))
+        _lib.check(lib.rbx_integrate(None, 0, None, 10, 5, None, 9, None, 1, 0, 0, 0.0, None, None, None, None))
     assert e.value.status == -1
     h = ctypes.c_void_p()
     assert lib.rbx_catalog_create(ctypes.byref(h)) == 0
@@ -46,15 +46,15 @@ def test_argument_errors_are_reported_without_gpu_work():
     ab = kernels.absorbers_struct({'nh3': 'nh3_hs'})
     dummy = ctypes.c_void_p(8)
     rc = lib.rbx_alpha(h, ctypes.byref(ab), dummy, dummy, dummy, None, 3, dummy, None, 2, 0, None, 0, dummy, None,
-                       dummy, 0, None)
+                       None, dummy, 0, None)
     assert rc == -4 and 'nh3_hs' in _lib.last_error()             # empty inversion block
     for sel, what in (({'co': 'co_ddb'}, 'co_ddb'), ({'h2': 'h2_orton'}, 'h2_orton')):
         rc = lib.rbx_alpha(h, ctypes.byref(kernels.absorbers_struct(sel)), dummy, dummy, dummy, None, 3, dummy,
-                           None, 2, 0, None, 0, dummy, None, dummy, 0, None)
+                           None, 2, 0, None, 0, dummy, None, None, dummy, 0, None)
         assert rc == -4 and what in _lib.last_error()             # its block is empty
     cl = kernels.absorbers_struct({'clouds': 'clouds_idp'})
     rc = lib.rbx_alpha(h, ctypes.byref(cl), dummy, dummy, dummy, None, 3, dummy, None, 2, 0, None, 0, dummy, None,
-                       dummy, 0, None)
+                       None, dummy, 0, None)
     assert rc == -1 and 'cloud table' in _lib.last_error()        # clouds need the cloud densities
     rc = lib.rbx_alpha_tangent(h, ctypes.byref(cl), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy,
                                dummy, None)
@@ -62,15 +62,23 @@ def test_argument_errors_are_reported_without_gpu_work():
     # workspace: partial sums [4 slots][parts][nl][nf] + h2o base [parts][nl] doubles
     nb = ctypes.c_size_t(0)
     h2 = kernels.absorbers_struct({'h2': 'h2_jj_ddb'})
-    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, ctypes.byref(nb)) == 0
+    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, None, ctypes.byref(nb)) == 0
     assert nb.value == (4 * 10 * 7 + 10) * 8                      # no lines: one part
-    try:
-        assert lib.rbx_tune(_lib.TUNE_ALPHA_SPLIT, 3) == 0
-        assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, ctypes.byref(nb)) == 0
-        assert nb.value == (4 * 3 * 10 * 7 + 3 * 10) * 8
-    finally:
-        lib.rbx_tune(_lib.TUNE_ALPHA_SPLIT, 0)
-    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 0, 7, 0, 1, ctypes.byref(nb)) == -1
+    # the launch shape is an argument of each call: nothing persists between calls
+    three = _lib.launch_ref({'alpha_split': 3})
+    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, three, ctypes.byref(nb)) == 0
+    assert nb.value == (4 * 3 * 10 * 7 + 3 * 10) * 8
+    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, None, ctypes.byref(nb)) == 0
+    assert nb.value == (4 * 10 * 7 + 10) * 8
+    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 0, 7, 0, 1, None, ctypes.byref(nb)) == -1
+    for bad in ({'alpha_k': 3}, {'alpha_split': 17}, {'int_unroll': 5}, {'alpha_series': 3}, {'tan_split': -1}):
+        rc = lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, _lib.launch_ref(bad), ctypes.byref(nb))
+        assert rc == -1 and 'launch.' in _lib.last_error(), bad
+    res = _lib.Launch()
+    res.reserved[2] = 1
+    assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, ctypes.byref(res), ctypes.byref(nb)) == -1
+    with pytest.raises(ValueError):
+        _lib.launch_struct({'no_such_knob': 1})
     assert lib.rbx_catalog_destroy(h) == 0
 
 
