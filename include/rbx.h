@@ -220,6 +220,23 @@ int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
               const double *scale, int precision, void *alpha, double *by_constituent,
               const rbx_launch *launch, void *workspace, size_t workspace_bytes, void *stream);
 
+/* rbx_alpha for callers that cannot allocate (no counterpart in the reference: for code
+ * under HIP graph capture, tight serving loops and harnesses that forbid hipMalloc).  Same
+ * arguments and results as rbx_alpha without the workspace: the partial sums go to a static
+ * device array of RBX_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends.  The call allocates nothing.  It uses the plan rbx_alpha would use with
+ * `launch`; when that needs more scratch than the maximum and launch->alpha_series is not 2,
+ * it takes the direct line sums with the most parts (up to that plan's) that fit.  A call
+ * that still does not fit returns RBX_ERR_ARG (the direct sums need 32 B per layer and
+ * frequency plus 8 B per layer, per part).  Calls on one GPU share its array: calls that
+ * overlap in time on one GPU (other streams, other threads) race and must not be made. */
+#define RBX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int rbx_alpha_noalloc(const rbx_catalog *cat, const rbx_absorbers *absorbers,
+                      const double *P, const double *T, const double *mix, const double *cloud, int nl,
+                      const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
+                      const double *scale, int precision, void *alpha, double *by_constituent,
+                      const rbx_launch *launch, void *stream);
+
 /* Workspace bytes rbx_alpha needs for this shape and launch (host call). */
 int rbx_alpha_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
                              int nf_nh3_low, int precision, const rbx_launch *launch, size_t *bytes);

@@ -39,7 +39,8 @@ EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_catalog
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
            'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles',
            'rbx_jacobian', 'rbx_raypaths_batch', 'rbx_integrate_batch', 'rbx_jacobian_batch',
-           'rbx_atmosphere_batch', 'rbx_alpha_rescale')
+           'rbx_atmosphere_batch', 'rbx_alpha_rescale', 'rbx_alpha_noalloc')
+NOALLOC_SCRATCH_BYTES = 64 << 20     # RBX_NOALLOC_SCRATCH_BYTES
 
 
 class RbxLibraryError(RuntimeError):
@@ -101,6 +102,8 @@ _SIGS = {
     'rbx_catalog_destroy': (_I, [_VP]),
     'rbx_alpha': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
                        _VP, _I, _VP, _VP, _LP, _VP, ctypes.c_size_t, _VP]),
+    'rbx_alpha_noalloc': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _VP, _I, _VP, _VP, _I, _I,
+                               _VP, _I, _VP, _VP, _LP, _VP]),
     'rbx_alpha_workspace_size': (_I, [_VP, ctypes.POINTER(Absorbers), _I, _I, _I, _I, _LP,
                                       ctypes.POINTER(ctypes.c_size_t)]),
     'rbx_alpha_tangent': (_I, [_VP, ctypes.POINTER(Absorbers), _VP, _VP, _VP, _I, _VP, _VP, _I, _I,

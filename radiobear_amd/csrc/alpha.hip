@@ -44,6 +44,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "catalog.hpp"
@@ -1631,6 +1632,83 @@ extern "C" int rbx_alpha(const rbx_catalog *cat, const rbx_absorbers *absorbers,
                   need);
     RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha: workspace not 8-byte aligned");
     return rbx::run_planned(A, precision, K, nsplit, series, workspace, static_cast<hipStream_t>(stream));
+}
+
+// ---- rbx_alpha without a workspace argument: the partial sums go to a static device array ----
+// For callers that may not allocate (code under HIP graph capture, serving loops).  Every
+// device that runs the library holds its own copy of `scratch` (RBX_NOALLOC_SCRATCH_BYTES)
+// until the process ends; its device address is looked up once per device with
+// hipGetSymbolAddress and used for every launch (the array's name is a host-side stand-in
+// in host code).  Calls on one device share the array: two that overlap race.  No kernel
+// of rbx_alpha reads a workspace entry before writing it, so nothing is cleared per call.
+namespace rbx {
+
+static __device__ __align__(256) unsigned char scratch[RBX_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_at[kMaxScratchDevices];
+
+static int scratch_address(void **out) {
+    int dev = 0;
+    RBX_CHECK_HIP(hipGetDevice(&dev));
+    RBX_CHECK_ARG(dev >= 0 && dev < kMaxScratchDevices, "rbx_alpha_noalloc: device %d beyond %d", dev,
+                  kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_at[dev] == nullptr) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("rbx_alpha_noalloc: hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                      hipGetErrorString(e));
+            return RBX_ERR_HIP;
+        }
+        g_scratch_at[dev] = p;
+    }
+    *out = g_scratch_at[dev];
+    return RBX_OK;
+}
+
+}  // namespace rbx
+
+extern "C" int rbx_alpha_noalloc(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,
+                                 const double *T, const double *mix, const double *cloud, int nl, const double *freq,
+                                 const double *freq_nh3, int nf, int nf_nh3_low, const double *scale, int precision,
+                                 void *alpha, double *by_constituent, const rbx_launch *launch, void *stream) {
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_noalloc: bad precision %d", precision);
+    rbx::AlphaArgs A{};
+    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, cloud, nl, freq, freq_nh3, nf, nf_nh3_low, scale, alpha,
+                            by_constituent, "rbx_alpha_noalloc");
+    if (rc) return rc;
+    rbx_launch L;
+    rc = rbx::read_launch(launch, L, "rbx_alpha_noalloc");
+    if (rc) return rc;
+    const size_t cap = RBX_NOALLOC_SCRATCH_BYTES;
+    int K = 1, nsplit = 1;
+    bool series = false;
+    size_t need = 0;
+    rc = rbx::plan_alpha(A, L, precision, K, nsplit, series, need);
+    if (rc) return rc;
+    if (need > cap && L.alpha_series != 2) {
+        // the direct sums need far less: take them, with the most parts (up to the plan's) that fit
+        rbx_launch D = L;
+        D.alpha_series = 1;
+        rc = rbx::plan_alpha(A, D, precision, K, nsplit, series, need);
+        if (rc) return rc;
+        for (int n = nsplit - 1; n >= 1 && need > cap && L.alpha_split == 0; --n) {
+            D.alpha_split = n;
+            rc = rbx::plan_alpha(A, D, precision, K, nsplit, series, need);
+            if (rc) return rc;
+        }
+    }
+    RBX_CHECK_ARG(need <= cap,
+                  "rbx_alpha_noalloc: this call needs %zu bytes of scratch, more than its %zu-byte maximum "
+                  "(rbx_alpha with a workspace has no such limit)",
+                  need, cap);
+    void *ws = nullptr;
+    rc = rbx::scratch_address(&ws);
+    if (rc) return rc;
+    return rbx::run_planned(A, precision, K, nsplit, series, ws, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,

@@ -225,6 +225,32 @@ def alpha(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by
     return (out, byc) if by_constituent else out
 
 
+def alpha_noalloc(catalog, absorbers, P, T, mix, freqs, precision='fp64', scale=None, by_constituent=False,
+                  cloud=None, launch=None, out=None, by_c=None):
+    """alpha() through rbx_alpha_noalloc: the library's static scratch instead of a workspace.
+    out / by_c: preallocated outputs ((nl, nf) of the precision's dtype / (7, nl, nf) float64),
+    so that the call itself allocates nothing on the device."""
+    prec = _lib.PRECISIONS[precision]
+    P, T, mix, fd, fn, nf_low, sc, nl = _alpha_inputs(P, T, mix, freqs, scale)
+    nf = fd.shape[0]
+    cl = None
+    if absorbers.clouds:
+        if cloud is None:
+            raise ValueError('clouds_idp needs the cloud densities (cloud=)')
+        cl = as_device(cloud)
+    dt = torch.float32 if prec == _lib.FP32 else torch.float64
+    if out is None:
+        out = torch.empty((nl, nf), dtype=dt, device=device())
+    if out.shape != (nl, nf) or out.dtype != dt or not out.is_contiguous():
+        raise ValueError('out must be a contiguous ({}, {}) {} tensor'.format(nl, nf, dt))
+    if by_constituent and by_c is None:
+        by_c = torch.zeros((len(_lib.CONSTITUENTS), nl, nf), dtype=torch.float64, device=device())
+    check(lib().rbx_alpha_noalloc(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), _ptr(cl), nl,
+                                  _ptr(fd), _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(out),
+                                  _ptr(by_c if by_constituent else None), _lib.launch_ref(launch), _stream()))
+    return (out, by_c) if by_constituent else out
+
+
 def alpha_rescale(absorb, scale=None, precision='fp64'):
     """alpha[nl][nf] = sum_c scale[c][l] absorb[l][f][c] (rbx_alpha_rescale) on the device."""
     prec = _lib.PRECISIONS[precision]
