@@ -37,6 +37,10 @@
  *   rbx_integrate_profiles  the .tau / .W / .Tb_lyr attributes Brightness.single keeps
  *                       (radiobear/brightness.py:53-120)
  *   rbx_jacobian        (no reference counterpart: dTb/dT and dTb/dq per layer)
+ *   rbx_integrate_adjoint, rbx_alpha_vjp  (no reference counterpart: the vector-Jacobian
+ *                       product of Brightness.single's Tb, brightness.py:93-117, with respect
+ *                       to the layer alpha and T, and its chain through alpha.py:151-192 to
+ *                       d/d{T, q_NH3, q_H2O} per layer)
  */
 #ifndef RBX_H
 #define RBX_H
@@ -47,7 +51,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 5
+#define RBX_ABI_VERSION 6
 
 /* status codes */
 #define RBX_OK          0
@@ -181,7 +185,9 @@ typedef struct rbx_launch {
     int tan_split;      /* parts each layer's lines are split into by rbx_alpha_tangent: 1..16
                            (built in: 1 when layers x 64-column tiles fill the device, else
                            enough parts for about 4 wavefronts per SIMD) */
-    int reserved[9];    /* must be 0 */
+    int adj_rays;       /* rays per wavefront of rbx_integrate_adjoint: 1, 2, 4 or 8 (built in: 8);
+                           their terms are summed in registers before each atomic add */
+    int reserved[8];    /* must be 0 */
 } rbx_launch;
 
 /* ---- line catalogue (device-resident, owned by the library) ---- */
@@ -265,7 +271,14 @@ int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers,
                       const double *P, const double *T, const double *mix, int nl,
                       const double *freq, const double *freq_nh3, int nf, int nf_nh3_low,
                       const double *scale, int precision, double *alpha, double *dalpha,
+                      const rbx_launch *launch, void *workspace, size_t workspace_bytes,
                       void *stream);
+
+/* Workspace bytes rbx_alpha_tangent needs for this shape and launch (host call; 0 when
+ * each layer's lines are summed in one part, the built-in choice once layers x 64-column
+ * tiles fill the device; with P parts 96 P bytes per layer and frequency). */
+int rbx_alpha_tangent_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
+                                     int nf_nh3_low, int precision, const rbx_launch *launch, size_t *bytes);
 
 /* ---- ray geometry (FP64) ----
  * R, N            [nl] layer radii (km) and refractive indices, top layer first
@@ -303,8 +316,8 @@ int rbx_integrate_profiles(const void *alpha, int alpha_precision, const double 
 
 /* ---- Jacobian rows ----
  * dalpha          [3][nl][nf] from rbx_alpha_tangent
- * J               [nray][2][nl][nf] float32: dTb/dT[l] and dTb/dq_NH3[l] of row (ray, f);
- *                 layers a ray does not reach (or past tau_cut) are 0
+ * J               [nray][3][nl][nf] float32: dTb/dT[l], dTb/dq_NH3[l] and dTb/dq_H2O[l] of
+ *                 row (ray, f); layers a ray does not reach (or past tau_cut) are 0
  * Tb              [nray][nf] float64 or NULL
  */
 int rbx_jacobian(const void *alpha, int alpha_precision, const double *dalpha, const double *T,
@@ -330,12 +343,41 @@ int rbx_integrate_batch(const void *alpha, int alpha_precision, const double *T,
                         const int *ray_atm, const int *order, int nray, int mode, int precision,
                         double tau_cut, double *Tb, double *intW, const rbx_launch *launch,
                         void *stream);
-/* J [nray][2][ldj][nf] float32, ldj >= the layer count of every atmosphere (rows
+/* J [nray][3][ldj][nf] float32, ldj >= the layer count of every atmosphere (rows
  * past a ray's atmosphere are 0); precision as rbx_integrate's (e^-tau evaluation). */
 int rbx_jacobian_batch(const void *alpha, int alpha_precision, const double *dalpha, const double *T,
                        int nl_total, int nf, const int *layer_off, int natm, const double *hds, int ld,
                        const int *nsteps, const int *ray_atm, const int *order, int nray, int precision,
                        double tau_cut, float *J, int ldj, double *Tb, void *stream);
+
+/* ---- adjoint of the integration: the vector-Jacobian product ----
+ * For a loss L with upstream gradient gTb [nray][nf] = dL/dTb (float64, device):
+ *   galpha [nl][nf]  dL/dalpha[l][f] = sum_ray gTb dTb/dalpha[l][f]
+ *   gT     [nl]      the direct part of dL/dT[l]: sum_ray,f gTb dTb/dT_l through T_l W_l
+ * (both overwritten; float64).  Same arithmetic and tau_cut as rbx_jacobian: a forward pass
+ * per ray, then the derivatives along the path with e^-tau recomputed.  order [nray]
+ * (device, or NULL for 0..nray-1) groups rays: launch.adj_rays consecutive entries of it
+ * share a wavefront, whose rays should be of one atmosphere and of similar path length
+ * (longest first) for speed; any order gives the same result up to the order of FP64
+ * additions.  An entry gTb = 0 contributes nothing (also where Tb is NaN); a NaN Tb under
+ * a non-zero gTb makes the gradient NaN.  Paths are held fixed (the ray geometry is not
+ * differentiated).  The batch form takes several atmospheres as rbx_integrate_batch. */
+int rbx_integrate_adjoint(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                          const double *hds, int ld, const int *nsteps, int nray, const int *order,
+                          int precision, double tau_cut, const double *gTb, double *galpha, double *gT,
+                          const rbx_launch *launch, void *stream);
+int rbx_integrate_adjoint_batch(const void *alpha, int alpha_precision, const double *T, int nl_total, int nf,
+                                const int *layer_off, int natm, const double *hds, int ld, const int *nsteps,
+                                const int *ray_atm, const int *order, int nray, int precision, double tau_cut,
+                                const double *gTb, double *galpha, double *gT, const rbx_launch *launch,
+                                void *stream);
+
+/* Chain through the absorption: grad [3][nl] (float64, overwritten) =
+ *   dL/dT[l]     = gT[l] + sum_f galpha[l][f] dalpha[0][l][f]
+ *   dL/dq_NH3[l] = sum_f galpha[l][f] dalpha[1][l][f],  dL/dq_H2O[l] = sum_f galpha dalpha[2]
+ * dalpha [3][nl][nf] from rbx_alpha_tangent; gT may be NULL (no direct part). */
+int rbx_alpha_vjp(const double *dalpha, const double *galpha, const double *gT, int nl, int nf,
+                  double *grad, void *stream);
 
 /* ---- atmosphere construction (computeProp on the device, atm_base.py:120-181) ----
  * For natm atmospheres laid end to end (layer_off as above), each ordered top

@@ -208,11 +208,11 @@ def brightness(alpha, T, ds, layer4ds):
     return torch.where(Tb_lyr < 2.725, torch.full_like(Tb_lyr, 2.725), Tb_lyr / intW)
 
 
-def jacobian(freqs, T, P, mix, lines, paths):
-    """dTb/dT[l] and dTb/dq_NH3[l] by reverse-mode autograd.
+def jacobian(freqs, T, P, mix, lines, paths, planes=2):
+    """dTb/dT[l], dTb/dq_NH3[l] (and, planes=3, dTb/dq_H2O[l]) by reverse-mode autograd.
 
     paths: list of (ds, layer4ds) per ray.  Returns (Tb [nray][nf],
-    J [nray][2][nl][nf]) as numpy float64.
+    J [nray][planes][nl][nf]) as numpy float64.
     """
     T = _t(T).requires_grad_(True)
     mix = _t(mix).requires_grad_(True)
@@ -220,12 +220,91 @@ def jacobian(freqs, T, P, mix, lines, paths):
     alpha = alpha_layers(freqs, T, P, mix, lines)
     nl, nf = alpha.shape
     Tb = torch.stack([brightness(alpha, T, ds, l4) for ds, l4 in paths])
-    J = np.zeros((len(paths), 2, nl, nf))
+    J = np.zeros((len(paths), planes, nl, nf))
+    rows = (ROWS['NH3'], ROWS['H2O'])[:planes - 1]
     for r in range(len(paths)):
         for j in range(nf):
             gT, gm = torch.autograd.grad(Tb[r, j], (T, mix), retain_graph=True, allow_unused=True)
             if gT is not None:
                 J[r, 0, :, j] = gT.numpy()
             if gm is not None:
-                J[r, 1, :, j] = gm[ROWS['NH3']].numpy()
+                for p, row in enumerate(rows):
+                    J[r, 1 + p, :, j] = gm[row].numpy()
     return Tb.detach().numpy(), J
+
+
+def vjp(freqs, T, P, mix, lines, paths, gTb):
+    """One reverse pass of L = sum_{ray,f} gTb[ray][f] Tb[ray][f] (rays whose gTb row is all
+    zero are left out of L, so a NaN Tb there does not reach the gradient).  Returns
+    (Tb [nray][nf], grad [3][nl]): dL/dT, dL/dq_NH3, dL/dq_H2O per layer, numpy float64."""
+    T = _t(T).requires_grad_(True)
+    mix = _t(mix).requires_grad_(True)
+    P = _t(P)
+    alpha = alpha_layers(freqs, T, P, mix, lines)
+    g = np.asarray(gTb, dtype=np.float64)
+    Tb = torch.stack([brightness(alpha, T, ds, l4) for ds, l4 in paths])
+    keep = [r for r in range(len(paths)) if np.any(g[r] != 0.0)]
+    L = sum((Tb[r] * _t(g[r])).sum() for r in keep) if keep else (T * 0.0).sum()
+    gT, gm = torch.autograd.grad(L, (T, mix), allow_unused=True)
+    nl = T.shape[0]
+    grad = np.zeros((3, nl))
+    if gT is not None:
+        grad[0] = gT.numpy()
+    if gm is not None:
+        grad[1] = gm[ROWS['NH3']].numpy()
+        grad[2] = gm[ROWS['H2O']].numpy()
+    return Tb.detach().numpy(), grad
+
+
+def alpha_tangents(freqs, T, P, mix, lines):
+    """alpha (nl, nf) and d alpha / d(T, q_NH3, q_H2O) (3, nl, nf), numpy float64, by
+    forward-mode automatic differentiation of alpha_layers: a layer's absorption depends
+    on that layer's inputs only, so one Jacobian-vector product with a tangent of ones in
+    one variable gives that variable's derivative for every layer at once (no graph is
+    kept, so this scales to full line lists)."""
+    from torch.func import jvp
+    T = _t(T)
+    P = _t(P)
+    mix = _t(mix)
+    nl = T.shape[0]
+
+    def of_T(t):
+        return alpha_layers(freqs, t, P, mix, lines)
+
+    def of_row(row):
+        def f(q):
+            m = mix.clone()
+            m[row] = q
+            return alpha_layers(freqs, T, P, m, lines)
+        return f
+
+    ones = torch.ones(nl, dtype=DT)
+    alpha, dT = jvp(of_T, (T,), (ones,))
+    _, dq_nh3 = jvp(of_row(ROWS['NH3']), (mix[ROWS['NH3']],), (ones,))
+    _, dq_h2o = jvp(of_row(ROWS['H2O']), (mix[ROWS['H2O']],), (ones,))
+    return alpha.numpy(), torch.stack([dT, dq_nh3, dq_h2o]).numpy()
+
+
+def rt_jacobian(alpha, T, paths, freq_index=None):
+    """The integration's own derivatives by reverse-mode autograd of brightness():
+    for each ray and frequency column, dTb/dalpha[l] and the direct dTb/dT[l] (T entering
+    through T_l W_l only).  alpha (nl, nf), T (nl,); paths: list of (ds, layer4ds).
+    Returns (Tb [nray][nc], Ja [nray][nl][nc], JT [nray][nl][nc]) over the columns
+    freq_index (default all); chained with alpha_tangents these give the full Jacobian."""
+    a = _t(alpha)
+    nl, nf = a.shape
+    cols = list(range(nf)) if freq_index is None else list(freq_index)
+    a = a[:, cols].clone().requires_grad_(True)
+    Tt = _t(T).requires_grad_(True)
+    nc = len(cols)
+    Tb = torch.stack([brightness(a, Tt, ds, l4) for ds, l4 in paths])
+    Ja = np.zeros((len(paths), nl, nc))
+    JT = np.zeros((len(paths), nl, nc))
+    for r in range(len(paths)):
+        for j in range(nc):
+            ga, gt = torch.autograd.grad(Tb[r, j], (a, Tt), retain_graph=True, allow_unused=True)
+            if ga is not None:
+                Ja[r, :, j] = ga[:, j].numpy()
+            if gt is not None:
+                JT[r, :, j] = gt.numpy()
+    return Tb.detach().numpy(), Ja, JT

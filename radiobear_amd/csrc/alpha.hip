@@ -129,6 +129,10 @@ struct AlphaArgs {
     double *far_scale;  // FP32 line-weight scale per layer, segment and coefficient set [nl][kSegs][2]
     int far_words, far_split;
     int far_off[kFarStreams];   // first mask word of each stream
+    // tangent pass split into tan_parts parts of a layer's tan_rounds 64-line rounds;
+    // tan_seg0[s]: the first round of segment s (h2o, h2s, nh3 inversion, rot, v2, ph3)
+    int tan_parts, tan_rounds;
+    int tan_seg0[6];
 };
 
 // ---- per-layer coefficient sets: each derives (D, a, c, g, G) of one line ----
@@ -1042,6 +1046,9 @@ union WaveTan {
 template <bool F32>
 struct SharedTan {
     WaveTan<F32> w[kThreads / 64];
+    // each wavefront's coefficient sets of the block being staged (uniform across the
+    // wavefront: kept in LDS, not registers, while the staged lines are summed)
+    alignas(16) double prep[kThreads / 64][64];
 };
 
 // value + (T, q_own) tangent sums of one line block for this lane's column.
@@ -1050,17 +1057,18 @@ struct SharedTan {
 // scale[s] = FP32 power-of-two weight scale of set s (1 in FP64 mode).
 template <bool F32, class Prep>
 __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int mine, const double *scale,
-                             int qslot, double x, double &acc, double (&dacc)[2], TanLines<F32> &st, Dual &base) {
+                             int qslot, double x, double &acc, double (&dacc)[2], TanLines<F32> &st, Dual &base,
+                             int lb, int le) {
     const int lane = threadIdx.x % 64;
     const float xf = static_cast<float>(x);
-    for (int l0 = 0; l0 < B.n; l0 += kWaveLines) {
+    for (int l0 = lb; l0 < le; l0 += kWaveLines) {
         const int i = l0 + lane;
-        const int cnt = min(kWaveLines, B.n - l0);
+        const int cnt = min(kWaveLines, le - l0);
         const int cntp = (cnt + kSubT - 1) / kSubT * kSubT;
         if (lane < cntp) {
             CoefT<Dual> qs[2] = {{Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)},
                                  {Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)}};   // padding: t = 0
-            if (i < B.n) line_coefs(prep, need, B, i, base, qs);
+            if (i < le) line_coefs(prep, need, B, i, base, qs);
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 if (!(need & (1 << s))) continue;
@@ -1154,19 +1162,38 @@ __device__ inline Dual wave_sum(const Dual &v) {
     return r;
 }
 
-template <bool F32>
+// The tangent pass in one launch (TAN_ALL) or split (TAN_PART, then TAN_FINISH): with
+// several parts per layer, part p of a (layer, tile) sums the staging rounds
+// [p R / parts, (p + 1) R / parts) of the layer's sequence of 64-line rounds over the
+// segments h2o, h2s, nh3 inversion, rotational, v2, ph3 and writes its per-constituent
+// partial sums (value, d/dT, d/dq of the constituent's own species) to the workspace; the
+// finish adds the parts in order and applies what needs whole sums (the H2O base term, the
+// nh3_hs clamp, nh3_sjs, the closed forms, the constituent scales).
+constexpr int kTanAll = 0, kTanPart = 1, kTanFinish = 2;
+constexpr int kTanSegs = 6;
+constexpr int kTanSlots = 4;   // h2o, h2s, nh3_hs, ph3 (the forward kSlot* order)
+constexpr int kTanComps = 3;   // value, d/dT, d/dq_own
+
+__device__ __host__ inline int tan_rounds(int n) { return (n + kWaveLines - 1) / kWaveLines; }
+
+template <bool F32, int MODE>
 __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan<F32> &sh) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
     const long long item = static_cast<long long>(blockIdx.x) * (kThreads / 64) + wave;
     const int tiles = (A.nf + 63) / 64;
     if (item >= static_cast<long long>(A.nl) * tiles) return;   // the whole wavefront: no block barriers follow
     const int layer = static_cast<int>(item / tiles);
-    const int col = static_cast<int>(item % tiles) * 64 + lane;
+    const int tile = static_cast<int>(item % tiles);
+    const int col = tile * 64 + lane;
     const bool ok = col < A.nf;
     const int cc = ok ? col : A.nf - 1;
     const int mine = cc < A.nf_low ? 0 : 1;                      // nh3_hs set of this lane's column
     const int need = (__any(mine == 0) ? 1 : 0) | (__any(mine == 1) ? 2 : 0);
     TanLines<F32> &st = sh.w[wave].lines;
+    const int parts = A.tan_parts;
+    const int part = MODE == kTanPart ? static_cast<int>(blockIdx.y) : 0;
+    const int r0 = MODE == kTanPart ? part * A.tan_rounds / parts : 0;
+    const int r1 = MODE == kTanPart ? (part + 1) * A.tan_rounds / parts : 0;
 
     const LayerT<double> L0 = layer_of(A, layer);
     const int nl = A.nl;
@@ -1184,70 +1211,149 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
     const double fn = A.freq_nh3 ? A.freq_nh3[cc] : f[0];
     const double x = f[0] * f[0], xn = fn * fn;
 
+    // workspace of the split pass: slot sums [parts][kTanSlots][kTanComps][nl][nf], then
+    // the H2O base terms [parts][kTanComps][nl]
+    const size_t plane = static_cast<size_t>(nl) * A.nf;
+    const size_t o = static_cast<size_t>(layer) * A.nf + col;
+    double *ws_base = A.ws + static_cast<size_t>(parts) * kTanSlots * kTanComps * plane;
+    auto slot_out = [&](int slot, int own, const Dual &v) {
+        if (!ok) return;
+        double *w = A.ws + (static_cast<size_t>(part) * kTanSlots + slot) * kTanComps * plane + o;
+        w[0] = v.v;
+        w[plane] = v.d[0];
+        w[2 * plane] = v.d[own];
+    };
+    auto slot_in = [&](int slot, int own) {
+        Dual r;
+        for (int p = 0; p < parts; ++p) {
+            const double *w = A.ws + (static_cast<size_t>(p) * kTanSlots + slot) * kTanComps * plane + o;
+            r.v += w[0];
+            r.d[0] += w[plane];
+            r.d[own] += w[2 * plane];
+        }
+        return r;
+    };
+    // the lines of one segment this pass sums (all of them unless split)
+    auto range = [&](int seg, int n, int &lb, int &le) {
+        lb = 0;
+        le = n;
+        if (MODE != kTanPart) return;
+        const int g0 = A.tan_seg0[seg];
+        const int ra = max(r0, g0), rb = min(r1, g0 + tan_rounds(n));
+        lb = min(n, max(0, ra - g0) * kWaveLines);
+        le = rb > ra ? min(n, (rb - g0) * kWaveLines) : lb;
+    };
+
     // total_layer_alpha (alpha.py:151-192): constituents scale-summed as each completes
     Dual tot, unused;
     auto emit = [&](int c, const Dual &v) { tot = tot + v * (A.scale ? A.scale[c * nl + layer] : 1.0); };
     // one line block: x (sum / scale) with the value, d/dT and d/dq_own parts, added to out
-    auto lines = [&](const DevBlock &B, const auto *prep, int nset, int qslot, double xx, Dual &out, Dual &base) {
+    auto lines = [&](int seg, const DevBlock &B, const auto *prep, int nset, int qslot, double xx, Dual &out,
+                     Dual &base) {
         double s[2] = {1.0, 1.0};
         if (F32)
             for (int k = 0; k < nset; ++k) s[k] = pow2_scale(prep[k].pref.v * B.wscale);
         double acc = 0.0, dacc[2] = {0.0, 0.0};
         const int nd = nset == 2 ? need : 1, me = nset == 2 ? mine : 0;
-        line_sum_tan<F32>(B, prep, nd, me, s, qslot, xx, acc, dacc, st, base);
+        int lb, le;
+        range(seg, B.n, lb, le);
+        using Prep = std::remove_const_t<std::remove_reference_t<decltype(prep[0])>>;
+        static_assert(2 * sizeof(Prep) <= sizeof(sh.prep[0]), "coefficient sets exceed their LDS slot");
+        Prep *lp = reinterpret_cast<Prep *>(sh.prep[wave]);
+        if (lane == 0)
+            for (int k = 0; k < nset; ++k) lp[k] = prep[k];
+        wave_sync();
+        line_sum_tan<F32>(B, lp, nd, me, s, qslot, xx, acc, dacc, st, base, lb, le);
+        wave_sync();   // the next block's sets overwrite these
         Dual r(xx * acc / s[me]);
         r.d[0] = xx * dacc[0] / s[me];
         r.d[qslot] = xx * dacc[1] / s[me];
         out = out + r;
     };
 
-    if (A.abs.h2) emit(RBX_C_H2, h2_cia(L, A.abs.h2) * x);
+    if (MODE != kTanPart && A.abs.h2) emit(RBX_C_H2, h2_cia(L, A.abs.h2) * x);
     if (A.abs.h2o) {
         Dual v, base;
-        {
+        if (MODE == kTanFinish) {
+            v = slot_in(kSlotH2o, 2);
+            for (int p = 0; p < parts; ++p) {
+                base.v += ws_base[(static_cast<size_t>(p) * kTanComps) * nl + layer];
+                base.d[0] += ws_base[(static_cast<size_t>(p) * kTanComps + 1) * nl + layer];
+                base.d[2] += ws_base[(static_cast<size_t>(p) * kTanComps + 2) * nl + layer];
+            }
+        } else {
             const H2oPrep<Dual> prep[1] = {H2oPrep<Dual>(L)};
-            lines(A.cat.blk[RBX_LINES_H2O], prep, 1, 2, x, v, base);
+            lines(0, A.cat.blk[RBX_LINES_H2O], prep, 1, 2, x, v, base);
+            base = wave_sum(base);
         }
-        emit(RBX_C_H2O, v + (wave_sum(base) + h2o_continuum(L)) * x);
+        if (MODE == kTanPart) {
+            slot_out(kSlotH2o, 2, v);
+            if (tile == 0 && lane == 0) {
+                ws_base[(static_cast<size_t>(part) * kTanComps) * nl + layer] = base.v;
+                ws_base[(static_cast<size_t>(part) * kTanComps + 1) * nl + layer] = base.d[0];
+                ws_base[(static_cast<size_t>(part) * kTanComps + 2) * nl + layer] = base.d[2];
+            }
+        } else {
+            emit(RBX_C_H2O, v + (base + h2o_continuum(L)) * x);
+        }
     }
     if (A.abs.h2s) {
         Dual v;
-        const H2sPrep<Dual> prep[1] = {H2sPrep<Dual>(L)};
-        lines(A.cat.blk[RBX_LINES_H2S], prep, 1, 1, x, v, unused);
-        emit(RBX_C_H2S, v);
+        if (MODE == kTanFinish) {
+            v = slot_in(kSlotH2s, 1);
+        } else {
+            const H2sPrep<Dual> prep[1] = {H2sPrep<Dual>(L)};
+            lines(1, A.cat.blk[RBX_LINES_H2S], prep, 1, 1, x, v, unused);
+        }
+        if (MODE == kTanPart)
+            slot_out(kSlotH2s, 1, v);
+        else
+            emit(RBX_C_H2S, v);
     }
     if (A.abs.nh3) {
         const int sel = A.abs.nh3;
         Dual hs, sj[1];
         if (nh3_uses_hs(sel, L0.P)) {
-            {
-                const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, nh3_low_set(sel)), InvPrep<Dual>(L, 1)};
-                lines(A.cat.blk[RBX_LINES_NH3_INV], prep, 2, 1, xn, hs, unused);
+            if (MODE == kTanFinish) {
+                hs = slot_in(kSlotNh3, 1);
+            } else {
+                {
+                    const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, nh3_low_set(sel)), InvPrep<Dual>(L, 1)};
+                    lines(2, A.cat.blk[RBX_LINES_NH3_INV], prep, 2, 1, xn, hs, unused);
+                }
+                {
+                    const RotPrep<Dual> prep[1] = {RotPrep<Dual>(L)};
+                    lines(3, A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, 1, xn, hs, unused);
+                }
+                {
+                    const V2Prep<Dual> prep[1] = {V2Prep<Dual>(L)};
+                    lines(4, A.cat.blk[RBX_LINES_NH3_V2], prep, 1, 1, xn, hs, unused);
+                }
             }
-            {
-                const RotPrep<Dual> prep[1] = {RotPrep<Dual>(L)};
-                lines(A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, 1, xn, hs, unused);
-            }
-            {
-                const V2Prep<Dual> prep[1] = {V2Prep<Dual>(L)};
-                lines(A.cat.blk[RBX_LINES_NH3_V2], prep, 1, 1, xn, hs, unused);
-            }
+            if (MODE == kTanPart) slot_out(kSlotNh3, 1, hs);
             if (hs.v < 0.0) hs = Dual(1.0E-8);   // nh3_hs.py:337-338
         }
-        if (nh3_uses_sjs(sel, L0.P)) nh3_sjs<1, true>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh.w[wave].sjs);
-        emit(RBX_C_NH3, nh3_select(sel, L0.P, hs, sj[0]));
+        if (MODE != kTanPart) {
+            if (nh3_uses_sjs(sel, L0.P)) nh3_sjs<1, true>(A.cat.blk[RBX_LINES_NH3_SJS], L, f, sj, sh.w[wave].sjs);
+            emit(RBX_C_NH3, nh3_select(sel, L0.P, hs, sj[0]));
+        }
     }
     if (A.abs.ph3) {
         // ph3 does not depend on q_NH3: its slot-1 tangent is identically zero
         Dual v;
-        const Ph3Prep<Dual> prep[1] = {Ph3Prep<Dual>(L)};
-        lines(A.cat.blk[RBX_LINES_PH3], prep, 1, 1, x, v, unused);
-        emit(RBX_C_PH3, v);
+        if (MODE == kTanFinish) {
+            v = slot_in(kSlotPh3, 1);
+        } else {
+            const Ph3Prep<Dual> prep[1] = {Ph3Prep<Dual>(L)};
+            lines(5, A.cat.blk[RBX_LINES_PH3], prep, 1, 1, x, v, unused);
+        }
+        if (MODE == kTanPart)
+            slot_out(kSlotPh3, 1, v);
+        else
+            emit(RBX_C_PH3, v);
     }
 
-    if (!ok) return;
-    const size_t plane = static_cast<size_t>(nl) * A.nf;
-    const size_t o = static_cast<size_t>(layer) * A.nf + col;
+    if (MODE == kTanPart || !ok) return;
     double *out = static_cast<double *>(A.alpha);
 #pragma unroll
     for (int m = 0; m < kNd; ++m) out[m * plane + o] = tot.d[m];
@@ -1256,14 +1362,16 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
 
 // FP32: ask for 3 wavefronts per SIMD (the staged-line buffers take 35 KB per
 // block); FP64's 65 KB of staging allows 2 blocks per CU whatever the registers.
+template <int MODE>
 __global__ __launch_bounds__(kThreads, 3) void alpha_tangent_kernel_f32(AlphaArgs A) {
     __shared__ SharedTan<true> sh;
-    alpha_tangent_body<true>(A, sh);
+    alpha_tangent_body<true, MODE>(A, sh);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void alpha_tangent_kernel_f64(AlphaArgs A) {
     __shared__ SharedTan<false> sh;
-    alpha_tangent_body<false>(A, sh);
+    alpha_tangent_body<false, MODE>(A, sh);
 }
 
 // check that the catalogue holds what the selected absorbers read
@@ -1711,29 +1819,126 @@ extern "C" int rbx_alpha_noalloc(const rbx_catalog *cat, const rbx_absorbers *ab
     return rbx::run_planned(A, precision, K, nsplit, series, ws, static_cast<hipStream_t>(stream));
 }
 
+namespace rbx {
+
+// Parts per layer of the tangent pass: launch.tan_split, else 1 when the (layer, tile)
+// wavefronts alone give the device about 4 per SIMD, else enough parts for that (a
+// 500-layer, 64-frequency request is 500 wavefronts on 1024 SIMDs: one part leaves half of
+// them idle and the rest with no latency hiding).
+static void plan_tangent(AlphaArgs &A, const rbx_launch &L) {
+    const DevBlock *blk = A.cat.blk;
+    const bool hs = nh3_has_hs(A.abs.nh3);
+    const int n[kTanSegs] = {A.abs.h2o ? blk[RBX_LINES_H2O].n : 0, A.abs.h2s ? blk[RBX_LINES_H2S].n : 0,
+                             hs ? blk[RBX_LINES_NH3_INV].n : 0,    hs ? blk[RBX_LINES_NH3_ROT].n : 0,
+                             hs ? blk[RBX_LINES_NH3_V2].n : 0,     A.abs.ph3 ? blk[RBX_LINES_PH3].n : 0};
+    int r = 0;
+    for (int s = 0; s < kTanSegs; ++s) {
+        A.tan_seg0[s] = r;
+        r += tan_rounds(n[s]);
+    }
+    A.tan_rounds = r;
+    int parts = L.tan_split;
+    if (parts == 0) {
+        const long long items = static_cast<long long>(A.nl) * ((A.nf + 63) / 64);
+        const long long want = 4 * 1024;   // wavefronts: 4 per SIMD of a 256-CU device
+        parts = static_cast<int>(std::min<long long>(kMaxLaunchSplit, (want + items - 1) / items));
+    }
+    A.tan_parts = std::max(1, std::min(parts, std::max(r, 1)));
+}
+
+static size_t tangent_ws_bytes(const AlphaArgs &A) {
+    if (A.tan_parts <= 1) return 0;
+    const size_t plane = static_cast<size_t>(A.nl) * A.nf;
+    return (static_cast<size_t>(A.tan_parts) * kTanSlots * kTanComps * plane +
+            static_cast<size_t>(A.tan_parts) * kTanComps * A.nl) *
+           sizeof(double);
+}
+
+static int tangent_args(AlphaArgs &A, rbx_launch &L, const rbx_catalog *cat, const rbx_absorbers *absorbers,
+                        int nl, int nf, int nf_nh3_low, int precision, const rbx_launch *launch, const char *who) {
+    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "%s: bad precision %d", who, precision);
+    RBX_CHECK_ARG(cat && absorbers, "%s: null catalogue or absorber selection", who);
+    RBX_CHECK_ARG(!has_extra(*absorbers), "%s: co_ddb, clouds_idp and h2_orton have no tangent; select them off",
+                  who);
+    RBX_CHECK_ARG(nl > 0 && nf > 0 && static_cast<long long>(nl) * ((nf + 63) / 64) < (1LL << 31),
+                  "%s: nl=%d nf=%d", who, nl, nf);
+    RBX_CHECK_ARG(nf_nh3_low >= 0 && nf_nh3_low <= nf, "%s: nf_nh3_low=%d", who, nf_nh3_low);
+    int rc = check_catalog(cat, *absorbers, who);
+    if (rc) return rc;
+    rc = read_launch(launch, L, who);
+    if (rc) return rc;
+    A.cat = cat->dev;
+    A.abs = *absorbers;
+    A.nl = nl;
+    A.nf = nf;
+    A.nf_low = nf_nh3_low;
+    plan_tangent(A, L);
+    return RBX_OK;
+}
+
+}  // namespace rbx
+
+extern "C" int rbx_alpha_tangent_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl,
+                                                int nf, int nf_nh3_low, int precision, const rbx_launch *launch,
+                                                size_t *bytes) {
+    RBX_CHECK_ARG(bytes, "rbx_alpha_tangent_workspace_size: null output");
+    rbx::AlphaArgs A{};
+    rbx_launch L;
+    int rc = rbx::tangent_args(A, L, cat, absorbers, nl, nf, nf_nh3_low, precision, launch,
+                               "rbx_alpha_tangent_workspace_size");
+    if (rc) return rc;
+    *bytes = rbx::tangent_ws_bytes(A);
+    return RBX_OK;
+}
+
 extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers, const double *P,
                                  const double *T, const double *mix, int nl, const double *freq,
                                  const double *freq_nh3, int nf, int nf_nh3_low, const double *scale, int precision,
-                                 double *alpha, double *dalpha, void *stream) {
-    RBX_CHECK_ARG(precision == RBX_FP64 || precision == RBX_FP32, "rbx_alpha_tangent: bad precision %d", precision);
+                                 double *alpha, double *dalpha, const rbx_launch *launch, void *workspace,
+                                 size_t workspace_bytes, void *stream) {
     rbx::AlphaArgs A{};
-    RBX_CHECK_ARG(absorbers && !rbx::has_extra(*absorbers),
-                  "rbx_alpha_tangent: co_ddb, clouds_idp and h2_orton have no tangent; select them off");
-    int rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nullptr, nl, freq, freq_nh3, nf, nf_nh3_low, scale, dalpha,
-                            nullptr, "rbx_alpha_tangent");
+    rbx_launch L;
+    int rc = rbx::tangent_args(A, L, cat, absorbers, nl, nf, nf_nh3_low, precision, launch, "rbx_alpha_tangent");
     if (rc) return rc;
+    // (fill_args sets the call's arrays and leaves the tangent plan as it is)
+    rc = rbx::fill_args(A, cat, absorbers, P, T, mix, nullptr, nl, freq, freq_nh3, nf, nf_nh3_low, scale, dalpha,
+                        nullptr, "rbx_alpha_tangent");
+    if (rc) return rc;
+    const int parts = A.tan_parts;
     A.value = alpha;
+    const size_t need = rbx::tangent_ws_bytes(A);
+    if (need > 0) {
+        RBX_CHECK_ARG(workspace && workspace_bytes >= need,
+                      "rbx_alpha_tangent: workspace of %zu bytes, this call needs %zu "
+                      "(rbx_alpha_tangent_workspace_size)",
+                      workspace_bytes, need);
+        RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha_tangent: workspace not 8-byte aligned");
+        A.ws = static_cast<double *>(workspace);
+    }
     const long long items = static_cast<long long>(nl) * ((nf + 63) / 64);
     const long long blocks = (items + rbx::kThreads / 64 - 1) / (rbx::kThreads / 64);
     RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_alpha_tangent: too many layers x frequencies");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (precision == RBX_FP32)
-        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kThreads),
-                           0, st, A);
+    const dim3 g1(static_cast<unsigned>(blocks)), gp(static_cast<unsigned>(blocks), static_cast<unsigned>(parts));
+    const bool f32 = precision == RBX_FP32;
+    if (parts <= 1) {
+        if (f32)
+            hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanAll>, g1, dim3(rbx::kThreads), 0, st, A);
+        else
+            hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanAll>, g1, dim3(rbx::kThreads), 0, st, A);
+        return rbx::launch_status("alpha_tangent_kernel");
+    }
+    if (f32)
+        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanPart>, gp, dim3(rbx::kThreads), 0, st, A);
     else
-        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kThreads),
-                           0, st, A);
-    return rbx::launch_status("alpha_tangent_kernel");
+        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanPart>, gp, dim3(rbx::kThreads), 0, st, A);
+    rc = rbx::launch_status("alpha_tangent_kernel (parts)");
+    if (rc) return rc;
+    if (f32)
+        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanFinish>, g1, dim3(rbx::kThreads), 0, st, A);
+    else
+        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanFinish>, g1, dim3(rbx::kThreads), 0, st, A);
+    return rbx::launch_status("alpha_tangent_kernel (finish)");
 }
 
 // ---- reuse path: a saved per-constituent absorption table, rescaled ----

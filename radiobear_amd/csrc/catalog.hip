@@ -38,6 +38,9 @@ int read_launch(const rbx_launch *in, rbx_launch &out, const char *who) {
     RBX_CHECK_ARG(out.int_map >= 0 && out.int_map <= 2, "%s: launch.int_map=%d (0, 1 or 2)", who, out.int_map);
     RBX_CHECK_ARG(out.tan_split >= 0 && out.tan_split <= kMaxLaunchSplit, "%s: launch.tan_split=%d (0..%d)", who,
                   out.tan_split, kMaxLaunchSplit);
+    RBX_CHECK_ARG(out.adj_rays == 0 || out.adj_rays == 1 || out.adj_rays == 2 || out.adj_rays == 4 ||
+                      out.adj_rays == 8,
+                  "%s: launch.adj_rays=%d (0, 1, 2, 4 or 8)", who, out.adj_rays);
     for (int r : out.reserved) RBX_CHECK_ARG(r == 0, "%s: launch.reserved must be 0", who);
     return RBX_OK;
 }

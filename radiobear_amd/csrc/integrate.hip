@@ -13,6 +13,8 @@
 // (block b -> group b % ngroups) so each XCD's L2 holds a slice of the table, or,
 // ray-major (launch.int_map 2), every group of a ray block to one XCD so the
 // ray's ds row is fetched once and the whole table stays in that XCD's L2.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace rbx {
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
 struct JacArgs {
     IntArgs I;
     const double *dalpha;  // [3][nl][nf]
-    float *J;              // [nray][2][ldj][nf]
+    float *J;              // [nray][3][ldj][nf]
     int ldj;               // layer rows per ray in J (>= the layers of any ray's atmosphere)
 };
 
@@ -306,8 +308,9 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
     const double *h = A.hds + static_cast<size_t>(ray) * A.ld;
     const size_t nlnf = static_cast<size_t>(A.nl) * A.nf;        // dalpha plane
     const size_t jrow = static_cast<size_t>(Aj.ldj) * A.nf;      // J plane of one ray
-    float *JT = Aj.J + static_cast<size_t>(ray) * 2 * jrow;
+    float *JT = Aj.J + static_cast<size_t>(ray) * 3 * jrow;
     float *Jq = JT + jrow;
+    float *Jw = Jq + jrow;
     const size_t row0 = static_cast<size_t>(base) * A.nf;
     const void *alpha = A.f32 ? static_cast<const void *>(static_cast<const float *>(A.alpha) + row0)
                               : static_cast<const void *>(static_cast<const double *>(A.alpha) + row0);
@@ -363,9 +366,11 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
             const size_t o = static_cast<size_t>(k) * A.nf + jj;
             const double dT = ck * W * invD + ga * dal[o];
             const double dq = ga * dal[nlnf + o];
+            const double dw = ga * dal[2 * nlnf + o];
             if (ok) {
                 JT[static_cast<size_t>(k) * A.nf + j] = static_cast<float>(dT);
                 Jq[static_cast<size_t>(k) * A.nf + j] = static_cast<float>(dq);
+                Jw[static_cast<size_t>(k) * A.nf + j] = static_cast<float>(dw);
             }
             if (k < last) {
                 const double a_next = load_alpha(alpha, A.f32, static_cast<size_t>(k + 1) * A.nf + jj);
@@ -380,7 +385,226 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
         for (; k < Aj.ldj; ++k) {
             JT[static_cast<size_t>(k) * A.nf + j] = 0.0f;
             Jq[static_cast<size_t>(k) * A.nf + j] = 0.0f;
+            Jw[static_cast<size_t>(k) * A.nf + j] = 0.0f;
         }
+    }
+}
+
+// ---- adjoint of the integration (vector-Jacobian product) ----
+// For upstream gradients gTb[ray][f] this accumulates, over every ray,
+//   galpha[l][f] = sum_ray gTb dTb/dalpha_l   and   gT[l] = sum_ray,f gTb dTb/dT_l (direct part)
+// where the direct part is the dependence through T_l W_l; the dependence of alpha on T is
+// galpha's, which rbx_alpha_vjp contracts with d alpha / d(T, q).  Per ray and frequency the
+// derivative is the one jacobian_kernel writes (a forward pass for Tb and the integrated
+// weight D, then a second pass along the path with e^-tau recomputed, never stored).  Here the
+// second pass of a wavefront's NR rays runs interleaved, layer by layer, so the rays' terms
+// add up in registers and reach HBM as one FP64 atomic add per (wavefront, layer, frequency)
+// instead of one per ray; the direct T part is summed over the wavefront's 64 frequencies
+// first (one atomic add per wavefront and layer).  Lanes whose gTb is 0 contribute nothing
+// (not 0 x NaN), so rays and frequencies a loss masks out are skipped even where Tb is NaN; a
+// NaN Tb under a non-zero gTb propagates.
+struct AdjArgs {
+    IntArgs I;
+    const double *gTb;   // [nray][nf]
+    double *galpha;      // [nl][nf], accumulated (zeroed by adjoint_clear_kernel first)
+    double *gT;          // [nl], accumulated
+};
+
+constexpr int kAdjChunk = 8;   // layers summed in registers between atomic flushes
+
+template <bool FAST>
+__device__ __forceinline__ double exp_neg(double tau) {
+    if constexpr (FAST)
+        return exp_neg_mixed(tau);
+    else
+        return exp(-tau);
+}
+
+// One ray of the wavefront: its path, atmosphere rows and pass-1 results.
+struct AdjRay {
+    const double *h;     // half steps (cm)
+    const double *Tl;    // layer temperatures of its atmosphere
+    size_t row0;         // offset of its atmosphere's first alpha row (base layer x nf)
+    int base;            // its atmosphere's first layer
+    int last;            // last path point reached (pass 1; -1: no gradient from this ray)
+    double Tb, u;        // Tb and gTb / D (0 where the lane contributes nothing)
+};
+
+// Pass 1 of one ray: Tb and the integrated weight D, and the last path point reached (the
+// cut is checked every 8 steps, as in jacobian_kernel).
+template <bool FAST>
+__device__ inline void adj_forward(const IntArgs &A, AdjRay &R, int S, int jj, bool ok, double g) {
+    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
+    int last = 0;
+    if (S > 1) {
+        const double cut = (A.tau_cut > 0.0 && !path_has_nan(R.h, R.Tl, S - 1)) ? A.tau_cut : 0.0;
+        double a0 = load_alpha(A.alpha, A.f32, R.row0 + jj);
+        for (int i = 0; i < S - 1; ++i) {
+            const double hh = R.h[i];
+            const double a1 = load_alpha(A.alpha, A.f32, R.row0 + static_cast<size_t>(i + 1) * A.nf + jj);
+            tau = tau + (a0 + a1) * hh;
+            const double W1 = a1 * exp_neg<FAST>(tau);
+            intW = intW + (W1 + W0) * hh;
+            const double TW1 = R.Tl[i + 1] * W1;
+            num = num + (TW1 + TW0) * hh;
+            a0 = a1;
+            W0 = W1;
+            TW0 = TW1;
+            last = i + 1;
+            if (cut > 0.0 && (i & 7) == 7) {
+                if (__all(!ok || tau > cut)) break;
+            }
+        }
+    }
+    // no gradient: off the planet or a single point (S <= 1), or Tb at the T_cmb floor
+    const bool live = S > 1 && !(num < kTcmb);
+    R.last = __any(live && ok && g != 0.0) ? last : -1;
+    R.Tb = num / intW;
+    R.u = (live && ok && g != 0.0) ? g / intW : 0.0;
+}
+
+// Pass 2 of NR rays of one atmosphere (row0 shared), layers [0, kend]: per layer and ray
+//   c_k = h_{k-1} + h_k, gl = gTb lambda_k = c_k (T_k - Tb) u,  W_k = alpha_k e^-tau_k,
+//   P_k = P_{k-1} + gl W_k (gTb times the prefix of lambda W),
+//   gTb dTb/dalpha_k = gl e^-tau_k + h_{k-1} P_{k-1} + h_k P_k,   gTb dTb/dT_k = c_k W_k u,
+// with h_{last} = 0 closing each ray's sums (the path end or the cut).
+template <int NR, bool FAST>
+__device__ inline void adj_backward(const AdjArgs &Aa, const AdjRay (&R)[NR], int kend, int j, int jj, bool ok) {
+    const IntArgs &A = Aa.I;
+    const size_t nf = A.nf;
+    const size_t row0 = R[0].row0;
+    const int base = R[0].base;
+    double tau[NR], P[NR], hp[NR];   // per ray: tau_k, P_{k-1}, h_{k-1}
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        tau[r] = 0.0;
+        P[r] = 0.0;
+        hp[r] = 0.0;
+    }
+    double a_prev = 0.0;
+    for (int k0 = 0; k0 <= kend; k0 += kAdjChunk) {
+        double accA[kAdjChunk], accT[kAdjChunk];
+#pragma unroll
+        for (int u = 0; u < kAdjChunk; ++u) {
+            accA[u] = 0.0;
+            accT[u] = 0.0;
+            const int k = k0 + u;
+            if (k > kend) continue;   // wavefront-uniform
+            const double a_k = load_alpha(A.alpha, A.f32, row0 + static_cast<size_t>(k) * nf + jj);
+            const double T_k = R[0].Tl[k];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (k > R[r].last) continue;   // wavefront-uniform: this ray has ended
+                const double hprev = hp[r];
+                const double hk = k < R[r].last ? R[r].h[k] : 0.0;
+                double e = 1.0, W = 0.0;
+                if (k > 0) {
+                    tau[r] = tau[r] + (a_prev + a_k) * hprev;
+                    e = exp_neg<FAST>(tau[r]);
+                    W = a_k * e;
+                }
+                const double ck = hprev + hk;
+                const double gl = ck * (T_k - R[r].Tb) * R[r].u;
+                const double Pn = P[r] + gl * W;
+                const double ga = (k > 0 ? gl * e : 0.0) + hprev * P[r] + hk * Pn;
+                const bool on = R[r].u != 0.0;
+                accA[u] += on ? ga : 0.0;
+                accT[u] += on ? ck * W * R[r].u : 0.0;
+                P[r] = Pn;
+                hp[r] = hk;
+            }
+            a_prev = a_k;
+        }
+        // flush the chunk: galpha rows by frequency, gT rows summed over the wavefront
+#pragma unroll
+        for (int u = 0; u < kAdjChunk; ++u) {
+            const int k = k0 + u;
+            if (k > kend) continue;
+            if (ok && accA[u] != 0.0) unsafeAtomicAdd(&Aa.galpha[row0 + static_cast<size_t>(k) * nf + j], accA[u]);
+            const double t = wave_sum(ok ? accT[u] : 0.0);
+            if (t != 0.0 && threadIdx.x % kLanes == 0) unsafeAtomicAdd(&Aa.gT[base + k], t);
+        }
+    }
+}
+
+// ---- kernel ----
+template <int NR, bool FAST>
+__global__ __launch_bounds__(kWaves * kLanes) void adjoint_kernel(AdjArgs Aa) {
+    const IntArgs &A = Aa.I;
+    const int wave = threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
+    const int group = blockIdx.x % A.ngroups;
+    const int slot0 = ((blockIdx.x / A.ngroups) * kWaves + wave) * NR;   // first schedule slot
+    const int j = group * kLanes + lane;
+    const bool ok = j < A.nf;
+    const int jj = ok ? j : A.nf - 1;
+    AdjRay R[NR];
+    int kend = -1, base0 = -1;
+    bool same = true;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        R[r].h = A.hds;
+        R[r].Tl = A.T;
+        R[r].row0 = 0;
+        R[r].base = 0;
+        R[r].last = -1;
+        R[r].Tb = 0.0;
+        R[r].u = 0.0;
+        const int ray = ray_of(A, slot0 + r);
+        if (ray < 0) continue;
+        int base, nlr;
+        if (!ray_span(A, ray, base, nlr)) continue;    // bad atmosphere index: Tb NaN, no gradient
+        const int S = min(A.nsteps[ray], nlr - 1);
+        R[r].h = A.hds + static_cast<size_t>(ray) * A.ld;
+        R[r].Tl = A.T + base;
+        R[r].row0 = static_cast<size_t>(base) * A.nf;
+        R[r].base = base;
+        const double g = ok ? Aa.gTb[static_cast<size_t>(ray) * A.nf + j] : 0.0;
+        adj_forward<FAST>(A, R[r], S, jj, ok, g);
+        if (R[r].last < 0) continue;
+        if (base0 < 0)
+            base0 = base;
+        else if (base != base0)
+            same = false;
+        kend = max(kend, R[r].last);
+    }
+    if (kend < 0) return;
+    if (same) {
+        adj_backward<NR, FAST>(Aa, R, kend, j, jj, ok);
+    } else {   // rays of several atmospheres in one wavefront: one at a time
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (R[r].last < 0) continue;
+            const AdjRay one[1] = {R[r]};
+            adj_backward<1, FAST>(Aa, one, R[r].last, j, jj, ok);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void adjoint_clear_kernel(double *galpha, size_t n, double *gT, int nl) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) galpha[i] = 0.0;
+    if (i < static_cast<size_t>(nl)) gT[i] = 0.0;
+}
+
+// grad[m][l] = sum_f galpha[l][f] dalpha[m][l][f] (+ gT[l] for m = 0): one wavefront per layer
+__global__ __launch_bounds__(256) void alpha_vjp_kernel(const double *dalpha, const double *galpha, const double *gT,
+                                                        int nl, int nf, double *grad) {
+    const int l = blockIdx.x * (blockDim.x / kLanes) + threadIdx.x / kLanes;
+    if (l >= nl) return;
+    const int lane = threadIdx.x % kLanes;
+    const size_t plane = static_cast<size_t>(nl) * nf;
+    const size_t row = static_cast<size_t>(l) * nf;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int f = lane; f < nf; f += kLanes) {
+        const double g = galpha[row + f];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) s[m] = fma(g, dalpha[m * plane + row + f], s[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const double t = wave_sum(s[m]);
+        if (lane == 0) grad[static_cast<size_t>(m) * nl + l] = t + (m == 0 && gT ? gT[l] : 0.0);
     }
 }
 
@@ -625,4 +849,91 @@ extern "C" int rbx_integrate_profiles(const void *alpha, int alpha_precision, co
     hipLaunchKernelGGL(rbx::profile_kernel, dim3(static_cast<unsigned>(blocks)), dim3(rbx::kWaves * rbx::kLanes), 0,
                        static_cast<hipStream_t>(stream), Ap);
     return rbx::launch_status("profile_kernel");
+}
+
+static int launch_adjoint(rbx::AdjArgs &Aa, const rbx_launch &L, void *stream) {
+    const int nr = L.adj_rays ? L.adj_rays : 8;
+    const long long slots = (static_cast<long long>(Aa.I.nray) + static_cast<long long>(rbx::kWaves) * nr - 1) /
+                            (static_cast<long long>(rbx::kWaves) * nr);
+    const long long blocks = static_cast<long long>(Aa.I.ngroups) * slots;
+    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate_adjoint: too many blocks");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t n = static_cast<size_t>(Aa.I.nl) * Aa.I.nf;
+    const size_t nclear = std::max(n, static_cast<size_t>(Aa.I.nl));
+    hipLaunchKernelGGL(rbx::adjoint_clear_kernel, dim3(static_cast<unsigned>((nclear + 255) / 256)), dim3(256), 0, st,
+                       Aa.galpha, n, Aa.gT, Aa.I.nl);
+    int rc = rbx::launch_status("adjoint_clear_kernel");
+    if (rc || blocks == 0) return rc;
+    const dim3 grid(static_cast<unsigned>(blocks)), block(rbx::kWaves * rbx::kLanes);
+#define RBX_ADJ_LAUNCH(NR)                                                                   \
+    do {                                                                                     \
+        if (Aa.I.fast_exp)                                                                   \
+            hipLaunchKernelGGL((rbx::adjoint_kernel<NR, true>), grid, block, 0, st, Aa);     \
+        else                                                                                 \
+            hipLaunchKernelGGL((rbx::adjoint_kernel<NR, false>), grid, block, 0, st, Aa);    \
+    } while (0)
+    switch (nr) {
+        case 1: RBX_ADJ_LAUNCH(1); break;
+        case 2: RBX_ADJ_LAUNCH(2); break;
+        case 4: RBX_ADJ_LAUNCH(4); break;
+        default: RBX_ADJ_LAUNCH(8); break;
+    }
+#undef RBX_ADJ_LAUNCH
+    return rbx::launch_status("adjoint_kernel");
+}
+
+extern "C" int rbx_integrate_adjoint(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
+                                     const double *hds, int ld, const int *nsteps, int nray, const int *order,
+                                     int precision, double tau_cut, const double *gTb, double *galpha, double *gT,
+                                     const rbx_launch *launch, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    rbx_launch L;
+    rc = rbx::read_launch(launch, L, "rbx_integrate_adjoint");
+    if (rc) return rc;
+    RBX_CHECK_ARG(ld >= nl - 1, "rbx_integrate_adjoint: ld=%d < nl-1=%d", ld, nl - 1);
+    RBX_CHECK_ARG(gTb && galpha && gT, "rbx_integrate_adjoint: null gTb, galpha or gT");
+    rbx::AdjArgs Aa;
+    rbx::fill_int_args(Aa.I, alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision, tau_cut);
+    Aa.I.order = order;
+    Aa.gTb = gTb;
+    Aa.galpha = galpha;
+    Aa.gT = gT;
+    return launch_adjoint(Aa, L, stream);
+}
+
+extern "C" int rbx_integrate_adjoint_batch(const void *alpha, int alpha_precision, const double *T, int nl_total,
+                                           int nf, const int *layer_off, int natm, const double *hds, int ld,
+                                           const int *nsteps, const int *ray_atm, const int *order, int nray,
+                                           int precision, double tau_cut, const double *gTb, double *galpha,
+                                           double *gT, const rbx_launch *launch, void *stream) {
+    int rc = rbx::check_int_args(alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision);
+    if (rc) return rc;
+    rbx_launch L;
+    rc = rbx::read_launch(launch, L, "rbx_integrate_adjoint_batch");
+    if (rc) return rc;
+    RBX_CHECK_ARG(layer_off && ray_atm && natm > 0, "rbx_integrate_adjoint_batch: null offsets / ray map or natm=%d",
+                  natm);
+    RBX_CHECK_ARG(ld >= 1, "rbx_integrate_adjoint_batch: ld=%d", ld);
+    RBX_CHECK_ARG(gTb && galpha && gT, "rbx_integrate_adjoint_batch: null gTb, galpha or gT");
+    rbx::AdjArgs Aa;
+    rbx::fill_int_args(Aa.I, alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision, tau_cut);
+    Aa.I.ray_atm = ray_atm;
+    Aa.I.layer_off = layer_off;
+    Aa.I.natm = natm;
+    Aa.I.order = order;
+    Aa.gTb = gTb;
+    Aa.galpha = galpha;
+    Aa.gT = gT;
+    return launch_adjoint(Aa, L, stream);
+}
+
+extern "C" int rbx_alpha_vjp(const double *dalpha, const double *galpha, const double *gT, int nl, int nf,
+                             double *grad, void *stream) {
+    RBX_CHECK_ARG(dalpha && galpha && grad, "rbx_alpha_vjp: null dalpha, galpha or grad");
+    RBX_CHECK_ARG(nl > 0 && nf > 0, "rbx_alpha_vjp: nl=%d nf=%d", nl, nf);
+    const int per = 256 / rbx::kLanes;
+    hipLaunchKernelGGL(rbx::alpha_vjp_kernel, dim3(static_cast<unsigned>((nl + per - 1) / per)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), dalpha, galpha, gT, nl, nf, grad);
+    return rbx::launch_status("alpha_vjp_kernel");
 }

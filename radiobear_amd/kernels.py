@@ -277,16 +277,24 @@ def alpha_workspace(catalog, absorbers, nl, nf, nf_low, prec, reuse=None, launch
     return torch.empty((max(1, (nb.value + 7) // 8),), dtype=torch.float64, device=device()), nb.value
 
 
-def alpha_tangent(catalog, absorbers, P, T, mix, freqs, scale=None, precision='fp64', with_alpha=False):
+def alpha_tangent(catalog, absorbers, P, T, mix, freqs, scale=None, precision='fp64', with_alpha=False,
+                  launch=None):
     """d alpha / d(T, q_NH3, q_H2O): (3, nl, nf) float64 on the device; with_alpha also
-    returns alpha (nl, nf) float64 from the same pass: (alpha, dalpha)."""
+    returns alpha (nl, nf) float64 from the same pass: (alpha, dalpha).  launch.tan_split
+    splits each layer's lines into parts (a workspace from torch's allocator)."""
     prec = _lib.PRECISIONS[precision]
     P, T, mix, fd, fn, nf_low, sc, nl = _alpha_inputs(P, T, mix, freqs, scale)
     nf = fd.shape[0]
+    ls = _lib.launch_struct(launch)
+    nb = ctypes.c_size_t(0)
+    check(lib().rbx_alpha_tangent_workspace_size(catalog.handle, ctypes.byref(absorbers), nl, nf, nf_low, prec,
+                                                 _lib.launch_ref(ls), ctypes.byref(nb)))
+    ws = torch.empty((max(1, (nb.value + 7) // 8),), dtype=torch.float64, device=device()) if nb.value else None
     out = torch.empty((3, nl, nf), dtype=torch.float64, device=device())
     val = torch.empty((nl, nf), dtype=torch.float64, device=device()) if with_alpha else None
     check(lib().rbx_alpha_tangent(catalog.handle, ctypes.byref(absorbers), _ptr(P), _ptr(T), _ptr(mix), nl,
-                                  _ptr(fd), _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(val), _ptr(out), _stream()))
+                                  _ptr(fd), _ptr(fn), nf, nf_low, _ptr(sc), prec, _ptr(val), _ptr(out),
+                                  _lib.launch_ref(ls), _ptr(ws), nb.value, _stream()))
     return (val, out) if with_alpha else out
 
 
@@ -374,7 +382,7 @@ def integrate_profiles(alpha_table, T, hds, nsteps, disc=False, precision=None):
 
 
 def jacobian(alpha_table, dalpha, T, hds, nsteps, tau_cut=0.0):
-    """J (nray, 2, nl, nf) float32: dTb/dT and dTb/dq_NH3 per layer; and Tb (nray, nf)."""
+    """J (nray, 3, nl, nf) float32: dTb/dT, dTb/dq_NH3 and dTb/dq_H2O per layer; and Tb (nray, nf)."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
     a = alpha_table.contiguous()
     T = as_device(T)
@@ -382,12 +390,88 @@ def jacobian(alpha_table, dalpha, T, hds, nsteps, tau_cut=0.0):
     nray, ld = hds.shape
     hds, nsteps = hds.contiguous(), nsteps.contiguous()   # held here: the C call reads their memory
     dalpha = dalpha.contiguous()
-    J = torch.empty((nray, 2, nl, nf), dtype=torch.float32, device=device())
+    J = torch.empty((nray, 3, nl, nf), dtype=torch.float32, device=device())
     Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
     check(lib().rbx_jacobian(_ptr(a), aprec, _ptr(dalpha), _ptr(T), nl, nf,
                              _ptr(hds), ld, _ptr(nsteps), nray, float(tau_cut),
                              _ptr(J), _ptr(Tb), _stream()))
     return J, Tb
+
+
+def _grad_input(gTb, nray, nf):
+    g = gTb if isinstance(gTb, torch.Tensor) else torch.as_tensor(np.asarray(gTb))
+    g = g.to(device=device(), dtype=torch.float64).contiguous()
+    if g.shape != (nray, nf):
+        raise ValueError('gTb must be ({}, {}), got {}'.format(nray, nf, tuple(g.shape)))
+    return g
+
+
+def path_order(nsteps):
+    """Rays by path length, longest first (int32 on the device): the schedule that lets the
+    adjoint's wavefronts hold rays of similar length."""
+    return torch.argsort(nsteps.to(torch.int64), descending=True, stable=True).to(torch.int32)
+
+
+def integrate_adjoint(alpha_table, T, hds, nsteps, gTb, order=None, precision=None, tau_cut=0.0, launch=None):
+    """Vector-Jacobian product of integrate(): (galpha (nl, nf), gT (nl,)) float64, the gradient
+    of L = sum gTb * Tb with respect to the alpha table and (directly) the layer temperatures."""
+    aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
+    prec = aprec if precision is None else _lib.PRECISIONS[precision]
+    a = alpha_table.contiguous()
+    T = as_device(T)
+    nl, nf = a.shape
+    nray, ld = hds.shape
+    hds, nsteps = hds.contiguous(), nsteps.contiguous()
+    g = _grad_input(gTb, nray, nf)
+    od = path_order(nsteps) if order is None else (order if isinstance(order, torch.Tensor) else _i32(order))
+    galpha = torch.empty((nl, nf), dtype=torch.float64, device=device())
+    gT = torch.empty((nl,), dtype=torch.float64, device=device())
+    check(lib().rbx_integrate_adjoint(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds), ld, _ptr(nsteps), nray, _ptr(od),
+                                      prec, float(tau_cut), _ptr(g), _ptr(galpha), _ptr(gT),
+                                      _lib.launch_ref(launch), _stream()))
+    return galpha, gT
+
+
+def integrate_adjoint_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, gTb, order=None, precision=None,
+                            tau_cut=0.0, launch=None):
+    """integrate_adjoint() for rays through several atmospheres laid end to end: galpha
+    (nl_total, nf) and gT (nl_total,).  order: the ray schedule (device int32), whose
+    consecutive launch.adj_rays entries should share an atmosphere (RetrievalBatch's does)."""
+    aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
+    prec = aprec if precision is None else _lib.PRECISIONS[precision]
+    a = alpha_table.contiguous()
+    T = as_device(T)
+    nl, nf = a.shape
+    nray, ld = hds.shape
+    hds, nsteps = hds.contiguous(), nsteps.contiguous()
+    g = _grad_input(gTb, nray, nf)
+    off = layer_off if isinstance(layer_off, torch.Tensor) else _i32(layer_off)
+    ra = ray_atm if isinstance(ray_atm, torch.Tensor) else _i32(ray_atm)
+    od = None if order is None else (order if isinstance(order, torch.Tensor) else _i32(order))
+    galpha = torch.empty((nl, nf), dtype=torch.float64, device=device())
+    gT = torch.empty((nl,), dtype=torch.float64, device=device())
+    check(lib().rbx_integrate_adjoint_batch(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(off), off.shape[0] - 1,
+                                            _ptr(hds), ld, _ptr(nsteps), _ptr(ra), _ptr(od), nray, prec,
+                                            float(tau_cut), _ptr(g), _ptr(galpha), _ptr(gT),
+                                            _lib.launch_ref(launch), _stream()))
+    return galpha, gT
+
+
+def alpha_vjp(dalpha, galpha, gT=None):
+    """grad (3, nl) float64: dL/dT, dL/dq_NH3, dL/dq_H2O per layer from the adjoint's galpha
+    (nl, nf) and gT (nl,) and the tangents dalpha (3, nl, nf) (rbx_alpha_vjp)."""
+    dalpha = dalpha.contiguous()
+    galpha = galpha.contiguous()
+    nl, nf = galpha.shape
+    if dalpha.shape != (3, nl, nf) or dalpha.dtype != torch.float64 or galpha.dtype != torch.float64:
+        raise ValueError('dalpha (3, nl, nf) and galpha (nl, nf) float64 expected')
+    if gT is not None:
+        gT = gT.contiguous()
+        if gT.shape != (nl,) or gT.dtype != torch.float64:
+            raise ValueError('gT must be ({},) float64'.format(nl))
+    grad = torch.empty((3, nl), dtype=torch.float64, device=device())
+    check(lib().rbx_alpha_vjp(_ptr(dalpha), _ptr(galpha), _ptr(gT), nl, nf, _ptr(grad), _stream()))
+    return grad
 
 
 # ------------------------------------------------------------- Jupiter lines
@@ -464,7 +548,7 @@ def _check_batch(off, ray_atm, nl_total, ld):
 
 def jacobian_batch(alpha_table, dalpha, T, layer_off, ray_atm, hds, nsteps, ldj, order=None, tau_cut=0.0,
                    precision=None):
-    """Tb (nray, nf) and J (nray, 2, ldj, nf) float32 for rays through several atmospheres.
+    """Tb (nray, nf) and J (nray, 3, ldj, nf) float32 for rays through several atmospheres.
 
     alpha_table (nl_total, nf), dalpha (3, nl_total, nf) and T (nl_total,) concatenated;
     layer_off / ray_atm / order as device or host int arrays (order None: 0..nray-1)."""
@@ -479,7 +563,7 @@ def jacobian_batch(alpha_table, dalpha, T, layer_off, ray_atm, hds, nsteps, ldj,
     ra = ray_atm if isinstance(ray_atm, torch.Tensor) else _i32(ray_atm)
     od = None if order is None else (order if isinstance(order, torch.Tensor) else _i32(order))
     dalpha = dalpha.contiguous()
-    J = torch.empty((nray, 2, ldj, nf), dtype=torch.float32, device=device())
+    J = torch.empty((nray, 3, ldj, nf), dtype=torch.float32, device=device())
     Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
     check(lib().rbx_jacobian_batch(_ptr(a), aprec, _ptr(dalpha), _ptr(T), nl, nf, _ptr(off),
                                    off.shape[0] - 1, _ptr(hds), ld, _ptr(nsteps), _ptr(ra),

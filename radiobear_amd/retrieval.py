@@ -10,8 +10,10 @@ batch:
    of every atmosphere (layers are independent);
 2. ``rbx_raypaths_batch`` the ray paths of each ray through its atmosphere's
    shells (T perturbations move the radii, so geometry is per atmosphere);
-3. ``rbx_jacobian_batch`` Tb and the Jacobian rows dTb/dT[l], dTb/dq_NH3[l] of
-   every (ray, frequency), forward and adjoint sweeps in one kernel.
+3. ``rbx_jacobian_batch`` Tb and the Jacobian rows dTb/dT[l], dTb/dq_NH3[l],
+   dTb/dq_H2O[l] of every (ray, frequency), forward and adjoint sweeps in one
+   kernel; or, for a scalar loss, ``model()`` (a differentiable torch function
+   whose backward pass is rbx_integrate_adjoint_batch + rbx_alpha_vjp).
 
 Atmospheres have uneven layer counts, so ray wavefronts are scheduled longest
 path first (``order``), which keeps the last wave of the grid short.
@@ -30,7 +32,7 @@ class RetrievalBatch:
     """
 
     def __init__(self, atmospheres, catalog, absorbers, freqs, rays, req, rpol, C, precision='fp32',
-                 tau_cut=None, orientation=(0.0, 0.0), gm=None, p_ref=1.0, device_props=False):
+                 tau_cut=None, orientation=(0.0, 0.0), gm=None, p_ref=1.0, device_props=False, launch=None):
         if precision not in _lib.PRECISIONS:
             raise ValueError('precision must be fp32 or fp64')
         self.precision = precision
@@ -67,6 +69,7 @@ class RetrievalBatch:
         self.req, self.p_ref = float(req), float(p_ref)
         self.gm = None if gm is None else float(gm)
         self._side = None   # side stream of the ray paths (created on first run)
+        self.launch = launch   # launch shape of every call ({field: value}; None: built in)
         if device_props:
             self.update()
 
@@ -86,14 +89,14 @@ class RetrievalBatch:
     def absorption(self):
         """alpha (nl_total, nf) and dalpha (3, nl_total, nf), float64, one fused pass."""
         return K.alpha_tangent(self.catalog, self.absorbers, self.P, self.T, self.mix, self.freqs,
-                               precision=self.precision, with_alpha=True)
+                               precision=self.precision, with_alpha=True, launch=self.launch)
 
     def paths(self):
         return K.raypaths_batch(self.planet, self.R, self.N, self.layer_off, self.ray_atm, self.b, self.ld,
                                 off_dev=self.off_d, ray_atm_dev=self.ray_atm_d)
 
     def run(self):
-        """Tb (nray, nf) float64 and J (nray, 2, ldj, nf) float32 (rows past an atmosphere's
+        """Tb (nray, nf) float64 and J (nray, 3, ldj, nf) float32 (rows past an atmosphere's
         layers are 0).  The ray paths (a few hundred latency-bound wavefronts) run on a side
         stream beside the absorption pass; the Jacobian pass waits for both."""
         import torch
@@ -110,6 +113,17 @@ class RetrievalBatch:
         J, Tb = K.jacobian_batch(alpha, dalpha, self.T, self.off_d, self.ray_atm_d, hds, ns, self.ldj,
                                  order=self.order_d, tau_cut=self.tau_cut, precision=self.precision)
         return Tb, J
+
+    def model(self, hds=None, nsteps=None):
+        """A differentiable forward model of the batch (adjoint.ForwardModel): Tb (nray, nf)
+        as a function of the concatenated T, q_NH3 and q_H2O, its backward pass the adjoint.
+        The ray paths (computed now unless given) are held fixed."""
+        from .adjoint import ForwardModel
+        if hds is None:
+            hds, nsteps = self.paths()
+        return ForwardModel(self.catalog, self.absorbers, self.P, self.mix, self.freqs, hds, nsteps,
+                            precision=self.precision, tau_cut=self.tau_cut, layer_off=self.off_d,
+                            ray_atm=self.ray_atm_d, order=self.order_d, launch=self.launch)
 
     def rows(self):
         return self.nray * len(self.freqs)

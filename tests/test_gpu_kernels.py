@@ -276,7 +276,7 @@ def test_jacobian_vs_torch_autograd(K, raw_lines, jupiter_lines, alpha_precision
     for b in rays:
         r = geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol)
         paths.append((r.ds, r.layer4ds))
-    Tb_ref, J_ref = OJ.jacobian(f, atm.T, atm.P, mix, jupiter_lines, paths)
+    Tb_ref, J_ref = OJ.jacobian(f, atm.T, atm.P, mix, jupiter_lines, paths, planes=3)
     pl = K.planet_desc(atm.Req, atm.Rpol)
     hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
     tab = K.alpha(cat, ab, atm.P, atm.T, mix, f, alpha_precision)
@@ -284,7 +284,7 @@ def test_jacobian_vs_torch_autograd(K, raw_lines, jupiter_lines, alpha_precision
     J, Tb = K.jacobian(tab, dal, atm.T, hds, ns)
     J = J.double().cpu().numpy()
     assert np.max(np.abs(Tb.cpu().numpy() - Tb_ref)) < (1e-8 if alpha_precision == 'fp64' else 1e-3)
-    for slot in (0, 1):
+    for slot in (0, 1, 2):
         scale = np.max(np.abs(J_ref[:, slot]), axis=1, keepdims=True) + 1e-30    # (nray, 1, nf)
         err = np.max(np.abs(J[:, slot] - J_ref[:, slot]) / scale)
         assert err < 1e-3, (slot, err)

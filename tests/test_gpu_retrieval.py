@@ -33,7 +33,9 @@ def small_batch(K, raw_lines):
     a0 = atms[0]
     out = {}
     for prec in ('fp64', 'fp32'):
-        out[prec] = RetrievalBatch(atms, cat, ab, f, rays, a0.Req, a0.Rpol, C, precision=prec)
+        # one launch shape for the batch and the single-atmosphere calls it is compared with
+        out[prec] = RetrievalBatch(atms, cat, ab, f, rays, a0.Req, a0.Rpol, C, precision=prec,
+                                   launch={'tan_split': 3})
     return atms, cat, ab, f, rays, out
 
 
@@ -47,7 +49,8 @@ def test_batch_equals_one_atmosphere_at_a_time(K, small_batch, prec):
     for a, atm in enumerate(atms):
         rows, _ = rb.atmosphere_slice(a)
         mix = K.mix_table(atm.gas, C)
-        alpha, dal = K.alpha_tangent(cat, ab, atm.P, atm.T, mix, f, precision=prec, with_alpha=True)
+        alpha, dal = K.alpha_tangent(cat, ab, atm.P, atm.T, mix, f, precision=prec, with_alpha=True,
+                                     launch=rb.launch)
         hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
         J1, Tb1 = K.jacobian(alpha.float() if prec == 'fp32' else alpha, dal, atm.T, hds, ns,
                              tau_cut=rb.tau_cut)
@@ -126,9 +129,9 @@ def test_retrieval_rows_vs_torch_autograd(K, raw_lines, jupiter_lines):
             for b in rays:
                 r = geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol)
                 paths.append((r.ds, r.layer4ds))
-            Tr, Jr = OJ.jacobian(f, atm.T, atm.P, K.mix_table(atm.gas, C), jupiter_lines, paths)
+            Tr, Jr = OJ.jacobian(f, atm.T, atm.P, K.mix_table(atm.gas, C), jupiter_lines, paths, planes=3)
             assert np.max(np.abs(Tb[rows] - Tr)) < (1e-8 if prec == 'fp64' else 1e-3)
-            for slot in (0, 1):
+            for slot in (0, 1, 2):
                 scale = np.max(np.abs(Jr[:, slot]), axis=1, keepdims=True) + 1e-30
                 err = np.max(np.abs(J[rows, slot, :atm.nl] - Jr[:, slot]) / scale)
                 assert err < 1e-3, (prec, a, slot, err)
@@ -173,8 +176,28 @@ def test_retrieval_with_device_atmosphere(K, small_batch):
     atms, cat, ab, f, rays, batches = small_batch
     a0 = atms[0]
     rb = RetrievalBatch(atms, cat, ab, f, rays, a0.Req, a0.Rpol, C, precision='fp64', gm=synth.JUPITER_GM,
-                        device_props=True)
+                        device_props=True, launch=batches['fp64'].launch)
     Tb, J = rb.run()
     Tb0, J0 = batches['fp64'].run()
     assert torch.max(torch.abs(Tb - Tb0)).item() < 1e-9
     assert torch.max(torch.abs(J - J0)).item() <= 1e-6 * torch.max(torch.abs(J0)).item()
+
+
+@pytest.mark.parametrize('prec,tol', [('fp64', 1e-12), ('fp32', 1e-6)])
+def test_tangent_every_split(K, small_batch, prec, tol):
+    """The tangent pass with each layer's lines in 1, 2, 5 and 16 parts (partial sums in a
+    workspace, then a finish pass): the same alpha and tangents up to the order of the
+    parts' additions (and, in FP32, the FP32 blocks of lines each part sums)."""
+    atms, cat, ab, f, rays, batches = small_batch
+    rb = batches[prec]
+    out = {}
+    for n in (1, 2, 5, 16):
+        L = {'tan_split': n}
+        a, d = K.alpha_tangent(cat, ab, rb.P, rb.T, rb.mix, f, precision=prec, with_alpha=True, launch=L)
+        out[n] = (a.cpu().numpy(), d.cpu().numpy())
+    a1, d1 = out[1]
+    for n, (a, d) in out.items():
+        assert np.max(np.abs(a - a1) / np.abs(a1)) < tol, n
+        for m in range(3):
+            scale = np.max(np.abs(d1[m]), axis=1, keepdims=True) + 1e-300
+            assert np.max(np.abs(d[m] - d1[m]) / scale) < tol, (n, m)

@@ -57,7 +57,7 @@ def test_argument_errors_are_reported_without_gpu_work():
                        None, dummy, 0, None)
     assert rc == -1 and 'cloud table' in _lib.last_error()        # clouds need the cloud densities
     rc = lib.rbx_alpha_tangent(h, ctypes.byref(cl), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy,
-                               dummy, None)
+                               dummy, None, None, 0, None)
     assert rc == -1 and 'no tangent' in _lib.last_error()
     # workspace: partial sums [4 slots][parts][nl][nf] + h2o base [parts][nl] doubles
     nb = ctypes.c_size_t(0)

@@ -289,3 +289,59 @@ def test_autograd_jacobian_vs_central_differences_of_oracle(jupiter_lines):
             fd = (tb(gp) - tb(gm)) / (2 * h)
             scale = np.max(np.abs(Jm[0, slot]), axis=0) + 1e-30
             assert np.max(np.abs(Jm[0, slot, l] - fd) / scale) < 1e-6, (l, slot)
+
+
+def test_autograd_vjp_vs_jacobian_and_central_differences(jupiter_lines):
+    """oracle.jacobian.vjp (one reverse pass of sum gTb Tb) equals the contraction of the
+    per-row Jacobian with gTb, and its dq_H2O row equals central FD of the numpy oracle (the
+    H2O row has no other check).  A ray whose gTb row is all zero is left out: its NaN or
+    finite Tb does not change the gradient."""
+    J, atm, mix, f = _jac_case(jupiter_lines)
+    C = synth.GAS_ROWS
+    aset = layers.jupiter_absorbers(jupiter_lines)
+    rays = [geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol) for b in ([0.0, 0.0], [0.45, 0.3])]
+    paths = [(r.ds, r.layer4ds) for r in rays]
+    rng = np.random.default_rng(8)
+    g = rng.normal(size=(2, len(f)))
+    Tb, grad = J.vjp(f, atm.T, atm.P, mix, jupiter_lines, paths, g)
+    _, Jm = J.jacobian(f, atm.T, atm.P, mix, jupiter_lines, paths, planes=3)
+    contracted = np.einsum('rf,rplf->pl', g, Jm)
+    assert np.max(np.abs(grad - contracted)) < 1e-9 * np.max(np.abs(contracted))
+
+    def loss(gas):
+        tab = layers.get_layers(list(f), gas, C, aset)
+        return sum(float(np.dot(g[i], rt.single(tab, gas[C['T']], r.ds, r.layer4ds))) for i, r in enumerate(rays))
+
+    wet = [int(i) for i in np.argsort(atm.gas[C['H2O']])[-2:]]    # q_H2O is tiny above the cloud
+    for l, slot, row in [(3, 0, C['T']), (9, 0, C['T'])] + [(i, 2, C['H2O']) for i in wet]:
+        h = 1e-5 * atm.gas[row, l]
+        gp = atm.gas.copy()
+        gm = atm.gas.copy()
+        gp[row, l] += h
+        gm[row, l] -= h
+        fd = (loss(gp) - loss(gm)) / (2 * h)
+        assert abs(grad[slot, l] - fd) < 1e-6 * np.max(np.abs(grad[slot])), (l, slot)
+    g0 = g.copy()
+    g0[1] = 0.0
+    _, one = J.vjp(f, atm.T, atm.P, mix, jupiter_lines, paths[:1], g0[:1])
+    _, both = J.vjp(f, atm.T, atm.P, mix, jupiter_lines, paths, g0)
+    assert np.array_equal(one, both)
+
+
+def test_forward_mode_tangents_and_chained_rt_jacobian(jupiter_lines):
+    """alpha_tangents (forward mode, one JVP per variable) chained with rt_jacobian (the
+    integration's autograd derivatives) reproduce the reverse-mode jacobian() rows, for the
+    T, q_NH3 and q_H2O planes."""
+    J, atm, mix, f = _jac_case(jupiter_lines)
+    rays = [geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol) for b in ([0.2, 0.0], [0.7, 0.1])]
+    paths = [(r.ds, r.layer4ds) for r in rays]
+    _, Jm = J.jacobian(f, atm.T, atm.P, mix, jupiter_lines, paths, planes=3)
+    alpha, dal = J.alpha_tangents(f, atm.T, atm.P, mix, jupiter_lines)
+    Tb, Ja, JT = J.rt_jacobian(alpha, atm.T, paths)
+    chained = np.stack([JT + Ja * dal[0][None], Ja * dal[1][None], Ja * dal[2][None]], axis=1)
+    for p in range(3):
+        scale = np.max(np.abs(Jm[:, p])) + 1e-300
+        assert np.max(np.abs(chained[:, p] - Jm[:, p])) < 1e-10 * scale, p
+    sub = [1, 4]
+    Tb2, Ja2, JT2 = J.rt_jacobian(alpha, atm.T, paths, freq_index=sub)
+    assert np.array_equal(Tb2, Tb[:, sub]) and np.allclose(Ja2, Ja[:, :, sub], rtol=1e-13, atol=0)
