@@ -71,6 +71,7 @@ class Problem:
         self.K = K
         w = synth.workload(name)
         self.w = w
+        self.w_name = name
         atm = w['atm']
         self.atm = atm
         raw = lines.raw_catalog()
@@ -282,46 +283,74 @@ def cpu_baseline(prob, meas):
     return rec
 
 
-def _rows_rate(rb, reps, rebuild=False):
-    """Forward + adjoint Jacobian rows/s of a RetrievalBatch (events on the launch stream);
-    rebuild: each step also rebuilds the atmospheres' shells on the device."""
+def tangent_flop(cat, nl, nf):
+    """Algorithmic FLOP of one tangent pass: per line, layer and frequency the forward count
+    (Ben-Reuven 9, Gross 7, VVW 12) plus 6 per tangent for the two tangents each line has
+    (d/dT and d/dq of its own species: dN 2, dQ 2, (dN - t dQ) / Q 2 with the reciprocal
+    shared), plus the closed forms."""
+    n = {k: cat.lines(k) for k in range(7)}
+    per = (FLOP_BR * (n[0] + n[4] + n[5]) + FLOP_GROSS * (n[1] + n[2]) + FLOP_VVW * n[6]
+           + 12 * (n[0] + n[1] + n[2] + n[4] + n[5] + n[6]) + 2 * CLOSED_FORM_FLOP)
+    return float(per) * nl * nf
+
+
+def _rows_rate(rb, reps, rebuild=False, tune_key=None, args=None):
+    """Forward + adjoint Jacobian rows/s of a RetrievalBatch: the step (with rebuild, the
+    shells rebuilt on the device first) captured in a HIP graph and replayed; per-stage
+    times from HIP events on the launch stream; the tangent pass's split autotuned first."""
     import torch
-    from radiobear_amd import kernels as K
+    from radiobear_amd import autotune as at, kernels as K
+    rec = {}
+    if tune_key is not None and not args.no_tune:
+        cand = (1, 2, 4, 8, 16)
 
-    def step():
-        if rebuild:
-            rb.update()
-        return rb.run()
-
-    step()
+        def tangent(L):
+            K.alpha_tangent(rb.catalog, rb.absorbers, rb.P, rb.T, rb.mix, rb.freqs, precision=rb.precision,
+                            with_alpha=True, launch=L)
+        rb.launch, r = at.tune('tan_split', tune_key, tangent, rb.launch, reps=2, path=args.tune_cache,
+                               candidates=cand)
+        rec['tan_split_ms'] = r['ms']
+    graph, (Tb, J) = rb.capture(rebuild)
+    graph.replay()
     torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ev[0].record()
+    t0 = time.perf_counter()
     for _ in range(reps):
-        Tb, J = step()
-    ev[1].record()
+        graph.replay()
     torch.cuda.synchronize()
-    dt = ev[0].elapsed_time(ev[1]) / reps * 1e-3
-    # one pass split by stage
-    st = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    dt = (time.perf_counter() - t0) / reps
+    # one eager pass split by stage
+    st = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
     st[0].record()
-    alpha, dal = rb.absorption()
+    if rebuild:
+        rb.update()
     st[1].record()
-    hds, ns = rb.paths()
+    alpha, dal = rb.absorption()
     st[2].record()
+    hds, ns = rb.paths()
+    st[3].record()
     K.jacobian_batch(alpha, dal, rb.T, rb.off_d, rb.ray_atm_d, hds, ns, rb.ldj, order=rb.order_d,
                      tau_cut=rb.tau_cut, precision=rb.precision)
-    st[3].record()
+    st[4].record()
     torch.cuda.synchronize()
-    stages = {'rbx_alpha_tangent': st[0].elapsed_time(st[1]), 'rbx_raypaths_batch': st[1].elapsed_time(st[2]),
-              'rbx_jacobian_batch': st[2].elapsed_time(st[3])}
-    return {'rows_per_s': rb.rows() / dt, 'ms_per_step': dt * 1e3, 'rows': rb.rows(), 'stage_ms': stages,
-            'nan_rows': int(torch.isnan(Tb).any(dim=1).sum().item())}
+    stages = {'rbx_atmosphere_batch': st[0].elapsed_time(st[1]), 'rbx_alpha_tangent': st[1].elapsed_time(st[2]),
+              'rbx_raypaths_batch': st[2].elapsed_time(st[3]), 'rbx_jacobian_batch': st[3].elapsed_time(st[4])}
+    if not rebuild:
+        stages.pop('rbx_atmosphere_batch')
+    flop = tangent_flop(rb.catalog, rb.nl_total, len(rb.freqs))
+    tan_s = stages['rbx_alpha_tangent'] * 1e-3
+    roof = {'kernel': 'rbx_alpha_tangent', 'bound': 'valu', 'achieved': flop / tan_s / 1e12, 'peak': PEAK['fp32'],
+            'unit': 'TFLOP/s', 'frac': flop / tan_s / 1e12 / PEAK['fp32'], 'algorithmic_flop_per_launch': flop,
+            'basis': 'forward line-shape FLOP plus 6 per tangent, two tangents per line, every (line, layer, '
+                     'frequency)'}
+    rec.update({'rows_per_s': rb.rows() / dt, 'ms_per_step': dt * 1e3, 'rows': rb.rows(), 'stage_ms': stages,
+                'graph': 'hipGraph replay of the whole step', 'launch': rb.launch, 'roofline': roof,
+                'nan_rows': int(torch.isnan(Tb).any(dim=1).sum().item())})
+    return rec
 
 
 def jacobian_rate(args):
     """Workload 4 (500 layers, 10000 NH3 lines, 64 freqs, 256 rays): forward + adjoint
-    Jacobian rows/s (dTb/dT and dTb/dq_NH3 per layer), fused FP32 alpha+tangent pass."""
+    Jacobian rows/s (dTb/dT, dTb/dq_NH3, dTb/dq_H2O per layer), fused FP32 alpha+tangent pass."""
     from radiobear_amd import kernels as K, lines, synth
     from radiobear_amd.retrieval import RetrievalBatch
     w = synth.workload('c4_jacobian')
@@ -330,8 +359,9 @@ def jacobian_rate(args):
     ab = K.absorbers_struct({'h2': 'h2_jj_ddb', 'h2o': 'h2o_bk', 'h2s': 'h2s_ddb', 'nh3': 'nh3_hs_sjs',
                              'ph3': 'ph3_jh'}, 'e')
     rb = RetrievalBatch([atm], cat, ab, w['freqs'], w['rays'], atm.Req, atm.Rpol, atm.C, precision='fp32')
-    out = _rows_rate(rb, 10)
-    out.update({'workload': 'c4_jacobian', 'entries_per_row': 2 * atm.nl, 'precision': 'fp32 line sums and tangents'})
+    out = _rows_rate(rb, 20, tune_key='c4_jacobian:fp32', args=args)
+    out.update({'workload': 'c4_jacobian', 'entries_per_row': 3 * atm.nl,
+                'precision': 'fp32 line sums and tangents, FP64 tau and adjoint sweep'})
     return out
 
 
@@ -340,12 +370,100 @@ def retrieval_rate(args):
     128 freqs, 16 rays each, forward + adjoint in one batch: Jacobian rows/s."""
     from radiobear_amd import retrieval
     rb = retrieval.synthetic(n_atm=1024, precision='fp32', device_props=True)
-    out = _rows_rate(rb, 2, rebuild=True)
+    # 1024 atmospheres x 2 tiles are a million tangent wavefronts: one part per layer (built in)
+    out = _rows_rate(rb, 3, rebuild=True, args=args)
     out.update({'workload': 'c5_retrieval', 'atmospheres': rb.natm, 'layers_total': rb.nl_total,
                 'rays': rb.nray, 'freqs': len(rb.freqs), 'precision': 'fp32 line sums and tangents',
                 'step': 'atmosphere shells (hydrostatic z, R, N) + alpha and tangents + ray paths + '
                         'forward/adjoint Jacobian, all on the device'})
     return out
+
+
+def adjoint_rate(prob, dt64, args):
+    """The north-star comparison on the large workload: FP32-line-sum forward + adjoint
+    (the gradient of a scalar loss sum g Tb with respect to T, q_NH3 and q_H2O of every layer,
+    through adjoint.ForwardModel: rbx_alpha_tangent + rbx_integrate, then rbx_integrate_adjoint
+    + rbx_alpha_vjp) against the plain FP64 forward + central-difference Jacobian (2 forwards
+    per layer and variable, 3 variables, plus the base forward).  The FD side is timed on
+    2 layers (12 graph-replayed FP64 forwards with T perturbed in place) and extrapolated to
+    all 6 nl + 1 forwards; its dL/dT on those layers is compared with the adjoint's."""
+    import torch
+    from radiobear_amd import _lib
+    from radiobear_amd.adjoint import ForwardModel
+    FP64 = _lib.FP64
+    model = ForwardModel(prob.cat, prob.ab, prob.P, prob.mix, prob.freqs, prob.hds, prob.ns, precision='fp32')
+    gen = torch.Generator(device='cuda').manual_seed(3)
+    g = torch.randn((prob.nray, prob.nf), generator=gen, device='cuda', dtype=torch.float64)
+    g[prob.ns < 0] = 0.0
+    nan_rows = torch.isnan(prob.Tb64).any(dim=1)
+    g[nan_rows] = 0.0                        # the loss leaves out the rays the geometry loses (NaN Tb)
+    NH3, H2O = _lib.SPECIES.index('NH3'), _lib.SPECIES.index('H2O')
+
+    def step():
+        x = [prob.T.clone().requires_grad_(True), prob.mix[NH3].clone().requires_grad_(True),
+             prob.mix[H2O].clone().requires_grad_(True)]
+        Tb = model(*x)
+        (Tb * g).nan_to_num(0.0).sum().backward()
+        return Tb, [v.grad for v in x]
+
+    step()
+    torch.cuda.synchronize()
+    reps = 3
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        Tb32, gr = step()
+    ev[1].record()
+    torch.cuda.synchronize()
+    t_adj = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+    # stages of one pass (HIP events on the launch stream)
+    T0 = prob.T.clone()
+    se = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    se[0].record()
+    alpha, dal = model.absorption(T0, prob.mix[NH3], prob.mix[H2O])
+    se[1].record()
+    model.integrate(alpha, T0)
+    se[2].record()
+    galpha, gT = model.adjoint(alpha, T0, g)
+    se[3].record()
+    from radiobear_amd import kernels as K
+    K.alpha_vjp(dal, galpha, gT)
+    se[4].record()
+    torch.cuda.synchronize()
+    stages = {'rbx_alpha_tangent': se[0].elapsed_time(se[1]), 'rbx_integrate': se[1].elapsed_time(se[2]),
+              'rbx_integrate_adjoint': se[2].elapsed_time(se[3]), 'rbx_alpha_vjp': se[3].elapsed_time(se[4])}
+    # FD side: graph-replayed FP64 forwards with T[l] +- h (prob.T is read by the graph in place)
+    _, graph = timed_graph(prob, FP64, 1, 1)
+    fd_layers = (prob.nl // 4, prob.nl // 2)
+    fd, n_fwd = {}, 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for l in fd_layers:
+        h = 1e-4 * float(T0[l])
+        out = []
+        for sgn in (1.0, -1.0):
+            prob.T.copy_(T0)
+            prob.T[l] += sgn * h
+            graph.replay()
+            out.append((prob.Tb64 * g).nan_to_num(0.0).sum())
+            n_fwd += 1
+        fd[l] = float((out[0] - out[1]) / (2 * h))
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / n_fwd
+    prob.T.copy_(T0)
+    n_total = 6 * prob.nl + 1
+    t_fd = n_total * t_fwd
+    gT = gr[0].cpu().numpy()
+    scale = float(np.max(np.abs(gT)))
+    agree = {str(l): {'adjoint_fp32': float(gT[l]), 'central_fd_fp64': fd[l],
+                      'rel_to_row_max': abs(float(gT[l]) - fd[l]) / scale} for l in fd_layers}
+    ok = ~torch.isnan(prob.Tb64)
+    dtb = float(torch.max(torch.abs(Tb32.detach() - prob.Tb64)[ok]))
+    return {'workload': prob.w_name, 'fp32_forward_adjoint_ms': t_adj * 1e3, 'stage_ms': stages,
+            'fp64_forward_ms': t_fwd * 1e3, 'fd_forwards': n_total, 'fp64_forward_fd_jacobian_s': t_fd,
+            'speedup_vs_fd_jacobian': t_fd / t_adj, 'target_speedup': 10.0,
+            'max_abs_dTb_fp32_vs_fp64_K': dtb, 'dL_dT_adjoint_vs_fd': agree,
+            'fd_timing': 'measured on {} graph-replayed FP64 forwards, extrapolated to {}'.format(n_fwd, n_total)}
 
 
 def main():
@@ -394,7 +512,7 @@ def main():
                              'fp32_vs_fp64_max_abs_dTb_K': float(np.max(np.abs(Tb32[ok] - Tb64[ok]))),
                              'fp32_vs_fp64_max_rel_dalpha': float(np.max(np.abs(a32 - a64) / np.abs(a64))),
                              'nan_rays': int(np.sum(np.isnan(Tb64).any(axis=1)))}
-        # the tau cuts (FP64 e^-50, FP32 e^-30) change Tb below each mode's resolution: show it
+        # the tau cuts change Tb below each mode's resolution: show it
         prob.launch_integrate_nocut(FP64)
         torch.cuda.synchronize()
         d64 = np.abs(prob.Tb.cpu().numpy()[ok] - Tb64[ok])
@@ -413,7 +531,10 @@ def main():
         ev[1].record()
         torch.cuda.synchronize()
         res['integrate_fp32_without_tau_cut_ms'] = ev[0].elapsed_time(ev[1]) / 3
+        res['ms_per_step_without_tau_cut'] = dt32 * 1e3 - kt32['rbx_integrate'] + res['integrate_fp32_without_tau_cut_ms']
         res['speedup_fp32_vs_fp64'] = dt64 / dt32
+        res['adjoint'] = adjoint_rate(prob, dt64, args)
+        log(args, 'adjoint', res['adjoint'])
         res['jacobian'] = jacobian_rate(args)
         log(args, 'jacobian', res['jacobian'])
         res['retrieval'] = retrieval_rate(args)
@@ -458,8 +579,10 @@ def main():
                    'on_planet_rays': prob.on_planet, 'lines': {k: v for k, v in prob.nlines.items()},
                    'graph': 'hipGraph replay of alpha || raypaths (side stream), then integrate',
                    'parallelism': 'single GPU',
-                   'precision': 'FP32 line sums (FP64 outer accumulation), FP64 detuning, geometry and tau; '
-                                'tau cut at 50'},
+                   'precision': 'FP32 line sums (FP64 outer accumulation), FP64 detuning, geometry, half '
+                                'steps and tau prefix sum; FP32 weights and trapezoid sums added into FP64 '
+                                'every batch of steps; tau cut at {:g} (FP64 mode {:g})'.format(
+                                    prob.K.FP32_TAU_CUT, prob.K.FP64_TAU_CUT)},
         'kernel_ms': kt32, 'launch_shapes': tune32, 'roofline': roof, 'cpu_baseline': cpu,
     }
     line.update(res)

@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 6
+#define RBX_ABI_VERSION 7
 
 /* status codes */
 #define RBX_OK          0
@@ -185,9 +185,11 @@ typedef struct rbx_launch {
     int tan_split;      /* parts each layer's lines are split into by rbx_alpha_tangent: 1..16
                            (built in: 1 when layers x 64-column tiles fill the device, else
                            enough parts for about 4 wavefronts per SIMD) */
-    int adj_rays;       /* rays per wavefront of rbx_integrate_adjoint: 1, 2, 4 or 8 (built in: 8);
+    int adj_rays;       /* rays per wavefront of rbx_integrate_adjoint: 1, 2, 4 or 8 (built in: 4);
                            their terms are summed in registers before each atomic add */
-    int reserved[8];    /* must be 0 */
+    int adj_copies;     /* copies of the adjoint's accumulators the atomic adds are spread over:
+                           1..64 (built in: one per 4 ray blocks of an atmosphere, at most 32) */
+    int reserved[7];    /* must be 0 */
 } rbx_launch;
 
 /* ---- line catalogue (device-resident, owned by the library) ---- */
@@ -354,7 +356,8 @@ int rbx_jacobian_batch(const void *alpha, int alpha_precision, const double *dal
  * For a loss L with upstream gradient gTb [nray][nf] = dL/dTb (float64, device):
  *   galpha [nl][nf]  dL/dalpha[l][f] = sum_ray gTb dTb/dalpha[l][f]
  *   gT     [nl]      the direct part of dL/dT[l]: sum_ray,f gTb dTb/dT_l through T_l W_l
- * (both overwritten; float64).  Same arithmetic and tau_cut as rbx_jacobian: a forward pass
+ * (both overwritten; float64).  The atomic adds of many rays go first to copies of
+ * the accumulators in the workspace (rbx_integrate_adjoint_workspace_size), summed at the end.  Same arithmetic and tau_cut as rbx_jacobian: a forward pass
  * per ray, then the derivatives along the path with e^-tau recomputed.  order [nray]
  * (device, or NULL for 0..nray-1) groups rays: launch.adj_rays consecutive entries of it
  * share a wavefront, whose rays should be of one atmosphere and of similar path length
@@ -365,12 +368,17 @@ int rbx_jacobian_batch(const void *alpha, int alpha_precision, const double *dal
 int rbx_integrate_adjoint(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
                           const double *hds, int ld, const int *nsteps, int nray, const int *order,
                           int precision, double tau_cut, const double *gTb, double *galpha, double *gT,
-                          const rbx_launch *launch, void *stream);
+                          const rbx_launch *launch, void *workspace, size_t workspace_bytes, void *stream);
 int rbx_integrate_adjoint_batch(const void *alpha, int alpha_precision, const double *T, int nl_total, int nf,
                                 const int *layer_off, int natm, const double *hds, int ld, const int *nsteps,
                                 const int *ray_atm, const int *order, int nray, int precision, double tau_cut,
                                 const double *gTb, double *galpha, double *gT, const rbx_launch *launch,
-                                void *stream);
+                                void *workspace, size_t workspace_bytes, void *stream);
+/* Workspace bytes of one adjoint call (host call): copies of the accumulators the atomic
+ * adds are spread over, 8 (copies nl nf + copies ceil(nf/64) nl) bytes, or 0 (natm = 1 for
+ * rbx_integrate_adjoint). */
+int rbx_integrate_adjoint_workspace_size(int nl, int nf, int nray, int natm, const rbx_launch *launch,
+                                         size_t *bytes);
 
 /* Chain through the absorption: grad [3][nl] (float64, overwritten) =
  *   dL/dT[l]     = gT[l] + sum_f galpha[l][f] dalpha[0][l][f]

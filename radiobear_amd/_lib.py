@@ -7,7 +7,7 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'librbx.so')
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 FP64 = 0
 FP32 = 1
@@ -34,14 +34,15 @@ CLOUD_FLAGS = ('ice_p', 'water_p', 'nh4sh_p', 'nh3ice_p', 'h2sice_p', 'ch4')   #
 CO_SHAPES = {'voigt': 0, 'vvw': 1, 'diff': 2}                         # any other coshape: 3
 
 LAUNCH_FIELDS = ('alpha_k', 'alpha_series', 'alpha_split', 'far_split', 'int_unroll', 'int_map', 'tan_split',
-                 'adj_rays')
+                 'adj_rays', 'adj_copies')
 
 EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
            'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles',
            'rbx_jacobian', 'rbx_raypaths_batch', 'rbx_integrate_batch', 'rbx_jacobian_batch',
            'rbx_atmosphere_batch', 'rbx_alpha_rescale', 'rbx_alpha_noalloc', 'rbx_integrate_adjoint',
-           'rbx_integrate_adjoint_batch', 'rbx_alpha_vjp', 'rbx_alpha_tangent_workspace_size')
+           'rbx_integrate_adjoint_batch', 'rbx_alpha_vjp', 'rbx_alpha_tangent_workspace_size',
+           'rbx_integrate_adjoint_workspace_size')
 NOALLOC_SCRATCH_BYTES = 64 << 20     # RBX_NOALLOC_SCRATCH_BYTES
 
 
@@ -65,7 +66,7 @@ class Absorbers(ctypes.Structure):
 
 class Launch(ctypes.Structure):
     """rbx_launch: the launch shape of one call (0 = the library's built-in choice)."""
-    _fields_ = [(n, ctypes.c_int) for n in LAUNCH_FIELDS] + [('reserved', ctypes.c_int * 8)]
+    _fields_ = [(n, ctypes.c_int) for n in LAUNCH_FIELDS] + [('reserved', ctypes.c_int * 7)]
 
 
 def launch_struct(launch):
@@ -116,9 +117,11 @@ _SIGS = {
     'rbx_integrate': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _LP, _VP]),
     'rbx_integrate_profiles': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
     'rbx_jacobian': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP]),
-    'rbx_integrate_adjoint': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP, _LP, _VP]),
+    'rbx_integrate_adjoint': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP, _LP,
+                                   _VP, ctypes.c_size_t, _VP]),
     'rbx_integrate_adjoint_batch': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _I, _I, _D, _VP,
-                                         _VP, _VP, _LP, _VP]),
+                                         _VP, _VP, _LP, _VP, ctypes.c_size_t, _VP]),
+    'rbx_integrate_adjoint_workspace_size': (_I, [_I, _I, _I, _I, _LP, ctypes.POINTER(ctypes.c_size_t)]),
     'rbx_alpha_vjp': (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
     'rbx_alpha_rescale': (_I, [_VP, _I, _I, _I, _VP, _I, _VP, _VP]),
     'rbx_atmosphere_batch': (_I, [_VP, _VP, _VP, _VP, _I, _VP, _I, _D, _D, _D, _VP, _VP, _VP, _VP]),

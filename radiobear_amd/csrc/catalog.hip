@@ -41,6 +41,8 @@ int read_launch(const rbx_launch *in, rbx_launch &out, const char *who) {
     RBX_CHECK_ARG(out.adj_rays == 0 || out.adj_rays == 1 || out.adj_rays == 2 || out.adj_rays == 4 ||
                       out.adj_rays == 8,
                   "%s: launch.adj_rays=%d (0, 1, 2, 4 or 8)", who, out.adj_rays);
+    RBX_CHECK_ARG(out.adj_copies >= 0 && out.adj_copies <= 64, "%s: launch.adj_copies=%d (0..64)", who,
+                  out.adj_copies);
     for (int r : out.reserved) RBX_CHECK_ARG(r == 0, "%s: launch.reserved must be 0", who);
     return RBX_OK;
 }

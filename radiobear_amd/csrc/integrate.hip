@@ -174,22 +174,18 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     if (A.intW) A.intW[o] = (S < 0) ? 0.0 : intW;
 }
 
-// ---- FP32 mode, point rays: FP32 weights and trapezoid sums ----
-// tau stays an FP64 prefix sum (its increments (a0 + a1) ds/2 are FP32), and W, T W
-// and the two trapezoid sums are FP32, added into FP64 totals after every batch of U
-// steps (blocked summation).  e^-tau is the hardware exp2 of t = -tau log2(e) rounded
-// to FP32: the rounding moves the exponent by at most |t| 2^-24, so a weight moves by
-// at most tau e^-tau 2^-24 ~ 2e-8 of its alpha (largest at tau = 1), below the FP32
-// sums' own error; past t = -150 the result is 0.  Within a batch of U steps the tau
-// increments add up in FP32 on the batch's FP64 starting tau (U roundings of a sum of U
-// terms, < 1e-6 relative to the batch's own increment).  Each wavefront first copies its
-// ray's half steps and layer temperatures to LDS as FP32 pairs (h_i, T_{i+1}), so a
-// step reads them with one broadcast LDS load.
+// ---- FP32 mode, point rays: FP32 weights and trapezoid sums, FP64 path and tau ----
+// The half steps and the tau prefix sum stay FP64: tau_{i+1} = tau_i + (a0 + a1) h_i in an
+// FP64 FMA at every step, with a0 + a1 (two FP32 table entries) added in FP32 and h_i the
+// FP64 half step, read as a wave-uniform scalar load (U steps per s_load batch).  W, T W
+// and the two trapezoid sums are FP32, added into FP64 totals after every batch of U steps
+// (blocked summation).  e^-tau is the hardware exp2 of t = -tau log2(e) with tau rounded to
+// FP32 first: the rounding moves the exponent by at most |t| 2^-24, so a weight moves by at
+// most tau e^-tau 2^-24 ~ 2e-8 of its alpha (largest at tau = 1), below the FP32 sums' own
+// error; past t = -150 the result is 0.  Each wavefront first copies its ray's half steps
+// rounded to FP32 and layer temperatures to LDS as pairs (h_i, T_{i+1}) for the FP32 sums,
+// so a step reads them with one broadcast LDS load.
 constexpr float kLog2ef = 1.44269504f;
-
-__device__ __forceinline__ float exp_neg_f32(double tau) {
-    return __builtin_amdgcn_exp2f(static_cast<float>(-tau * kLog2e));
-}
 
 template <int U>
 __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs A) {
@@ -225,18 +221,18 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
         int i = 0;
         for (; i + U <= n; i += U) {
             float ab[U];
+            double hb[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) ab[u] = al[(i + u + 1) * nf];
+            for (int u = 0; u < U; ++u) {
+                ab[u] = al[(i + u + 1) * nf];
+                hb[u] = h[i + u];
+            }
             float iW = 0.0f, nm = 0.0f;
-            // the batch's tau increments add up in FP32 (dt) on the FP64 tau at its start,
-            // and the exponent is t0 - dt log2(e) with t0 = -tau log2(e) rounded once
-            const float t0 = static_cast<float>(-tau * kLog2e);
-            float dt = 0.0f;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const float2 ht = hT[i + u];
-                dt = fmaf(a0 + ab[u], ht.x, dt);
-                const float W1 = ab[u] * __builtin_amdgcn_exp2f(fmaf(dt, -kLog2ef, t0));
+                tau = fma(static_cast<double>(a0 + ab[u]), hb[u], tau);
+                const float W1 = ab[u] * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
                 iW = fmaf(W1 + W0, ht.x, iW);
                 const float TW1 = ht.y * W1;
                 nm = fmaf(TW1 + TW0, ht.x, nm);
@@ -244,7 +240,6 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
                 W0 = W1;
                 TW0 = TW1;
             }
-            tau += static_cast<double>(dt);
             intW += static_cast<double>(iW);
             num += static_cast<double>(nm);
             if (cut > 0.0 && __all(!ok || tau > cut)) {
@@ -255,8 +250,8 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
         for (; i < n; ++i) {
             const float a1 = al[(i + 1) * nf];
             const float2 ht = hT[i];
-            tau += static_cast<double>((a0 + a1) * ht.x);
-            const float W1 = a1 * exp_neg_f32(tau);
+            tau = fma(static_cast<double>(a0 + a1), h[i], tau);
+            const float W1 = a1 * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
             intW += static_cast<double>((W1 + W0) * ht.x);
             const float TW1 = ht.y * W1;
             num += static_cast<double>((TW1 + TW0) * ht.x);
@@ -276,6 +271,65 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
         Tb = num / intW;
     A.Tb[o] = Tb;
     if (A.intW) A.intW[o] = (S < 0) ? 0.0 : intW;
+}
+
+// The forward sums of one ray (brightness.py:93-101) for the Jacobian and the adjoint:
+// the integrated weight D = intW, the T-weighted sum num and the last path point reached.
+// The optical-depth cut is checked after every 8 steps; the steps are taken 8 at a time,
+// their loads issued together, so one memory latency covers 8 steps.
+template <bool FAST>
+__device__ __forceinline__ double exp_neg(double tau) {
+    if constexpr (FAST)
+        return exp_neg_mixed(tau);
+    else
+        return exp(-tau);
+}
+
+template <bool FAST>
+__device__ inline void forward_sums(const IntArgs &A, const double *h, const double *Tl, size_t row0, int S, int jj,
+                                    bool ok, double &intW, double &num, int &last) {
+    constexpr int U = 8;
+    double tau = 0.0, W0 = 0.0, TW0 = 0.0;
+    intW = 0.0;
+    num = 0.0;
+    last = 0;
+    auto step = [&](double a0, double a1, double hh, double T1) {
+        tau = tau + (a0 + a1) * hh;
+        const double W1 = a1 * exp_neg<FAST>(tau);
+        intW = intW + (W1 + W0) * hh;
+        const double TW1 = T1 * W1;
+        num = num + (TW1 + TW0) * hh;
+        W0 = W1;
+        TW0 = TW1;
+    };
+    if (S <= 1) return;
+    const int n = S - 1;
+    const double cut = (A.tau_cut > 0.0 && !path_has_nan(h, Tl, n)) ? A.tau_cut : 0.0;
+    const size_t nf = A.nf;
+    double a0 = load_alpha(A.alpha, A.f32, row0 + jj);
+    int i = 0;
+    for (; i + U <= n; i += U) {
+        double ab[U], hb[U], tb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ab[u] = load_alpha(A.alpha, A.f32, row0 + static_cast<size_t>(i + u + 1) * nf + jj);
+            hb[u] = h[i + u];
+            tb[u] = Tl[i + u + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            step(a0, ab[u], hb[u], tb[u]);
+            a0 = ab[u];
+        }
+        last = i + U;
+        if (cut > 0.0 && __all(!ok || tau > cut)) return;
+    }
+    for (; i < n; ++i) {
+        const double a1 = load_alpha(A.alpha, A.f32, row0 + static_cast<size_t>(i + 1) * nf + jj);
+        step(a0, a1, h[i], Tl[i + 1]);
+        a0 = a1;
+        last = i + 1;
+    }
 }
 
 // ---- Jacobian: two forward passes per (ray, freq) ----
@@ -317,29 +371,13 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
     const double *Tl = A.T + base;
     const double *dal = Aj.dalpha + row0;
 
-    // pass 1: Tb and intW (same recurrence and cut as integrate_kernel)
-    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
-    int last = 0;   // last path point reached
-    if (S > 1) {
-        const double cut = (A.tau_cut > 0.0 && !path_has_nan(h, Tl, S - 1)) ? A.tau_cut : 0.0;
-        double a0 = load_alpha(alpha, A.f32, jj);
-        for (int i = 0; i < S - 1; ++i) {
-            const double hh = h[i];
-            const double a1 = load_alpha(alpha, A.f32, static_cast<size_t>(i + 1) * A.nf + jj);
-            tau = tau + (a0 + a1) * hh;
-            const double W1 = a1 * (A.fast_exp ? exp_neg_mixed(tau) : exp(-tau));
-            intW = intW + (W1 + W0) * hh;
-            const double TW1 = Tl[i + 1] * W1;
-            num = num + (TW1 + TW0) * hh;
-            a0 = a1;
-            W0 = W1;
-            TW0 = TW1;
-            last = i + 1;
-            if (cut > 0.0 && (i & 7) == 7) {
-                if (__all(!ok || tau > cut)) break;
-            }
-        }
-    }
+    // pass 1: Tb and intW (the recurrence of integrate_kernel, the cut checked every 8 steps)
+    double intW, num;
+    int last;   // last path point reached
+    if (A.fast_exp)
+        forward_sums<true>(A, h, Tl, row0, S, jj, ok, intW, num, last);
+    else
+        forward_sums<false>(A, h, Tl, row0, S, jj, ok, intW, num, last);
     const bool live = S > 1 && !(num < kTcmb);
     const double Tb = !valid ? __builtin_nan("") : ((S < 0 || num < kTcmb) ? kTcmb : num / intW);
     if (ok && A.Tb) A.Tb[static_cast<size_t>(ray) * A.nf + j] = Tb;
@@ -351,7 +389,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
         double P = 0.0;          // P_{k-1}
         double hprev = 0.0;      // h_{k-1}
         double a_k = load_alpha(alpha, A.f32, jj);
-        tau = 0.0;
+        double tau = 0.0;
         for (k = 0; k <= last; ++k) {
             const double hk = (k < last) ? h[k] : 0.0;   // the cut (or path end) closes the sums
             double e = 1.0, W = 0.0;
@@ -406,185 +444,240 @@ __global__ __launch_bounds__(kWaves * kLanes) void jacobian_kernel(JacArgs Aj) {
 struct AdjArgs {
     IntArgs I;
     const double *gTb;   // [nray][nf]
-    double *galpha;      // [nl][nf], accumulated (zeroed by adjoint_clear_kernel first)
-    double *gT;          // [nl], accumulated
+    double *galpha;      // [nl][nf] result
+    double *gT;          // [nl] result
+    // the atomic adds go to `copies` copies of galpha, [copies][nl][nf] (ray block b adds to
+    // copy b % copies), and of gT per frequency group, [copies][ngroups][nl], zeroed first and
+    // summed into galpha and gT at the end: thousands of wavefronts of one atmosphere add to
+    // the same rows at the same time, and spreading them over copies keeps them from queueing
+    // at the same addresses.  With one copy these are galpha and gT themselves (gT then holds
+    // [ngroups][nl] partial sums only if ngroups is 1; see launch_adjoint).
+    double *acc_alpha, *acc_T;
+    int copies;
 };
 
 constexpr int kAdjChunk = 8;   // layers summed in registers between atomic flushes
 
-template <bool FAST>
-__device__ __forceinline__ double exp_neg(double tau) {
-    if constexpr (FAST)
-        return exp_neg_mixed(tau);
-    else
-        return exp(-tau);
-}
-
-// One ray of the wavefront: its path, atmosphere rows and pass-1 results.
+// One ray of the wavefront: its path and atmosphere rows.
 struct AdjRay {
     const double *h;     // half steps (cm)
-    const double *Tl;    // layer temperatures of its atmosphere
-    size_t row0;         // offset of its atmosphere's first alpha row (base layer x nf)
-    int base;            // its atmosphere's first layer
-    int last;            // last path point reached (pass 1; -1: no gradient from this ray)
-    double Tb, u;        // Tb and gTb / D (0 where the lane contributes nothing)
+    int steps;           // path steps: S - 1 (0 or less: no path)
+    double g;            // gTb of this lane (0: the lane contributes nothing)
 };
 
-// Pass 1 of one ray: Tb and the integrated weight D, and the last path point reached (the
-// cut is checked every 8 steps, as in jacobian_kernel).
-template <bool FAST>
-__device__ inline void adj_forward(const IntArgs &A, AdjRay &R, int S, int jj, bool ok, double g) {
-    double tau = 0.0, W0 = 0.0, TW0 = 0.0, intW = 0.0, num = 0.0;
-    int last = 0;
-    if (S > 1) {
-        const double cut = (A.tau_cut > 0.0 && !path_has_nan(R.h, R.Tl, S - 1)) ? A.tau_cut : 0.0;
-        double a0 = load_alpha(A.alpha, A.f32, R.row0 + jj);
-        for (int i = 0; i < S - 1; ++i) {
-            const double hh = R.h[i];
-            const double a1 = load_alpha(A.alpha, A.f32, R.row0 + static_cast<size_t>(i + 1) * A.nf + jj);
-            tau = tau + (a0 + a1) * hh;
-            const double W1 = a1 * exp_neg<FAST>(tau);
-            intW = intW + (W1 + W0) * hh;
-            const double TW1 = R.Tl[i + 1] * W1;
-            num = num + (TW1 + TW0) * hh;
-            a0 = a1;
-            W0 = W1;
-            TW0 = TW1;
-            last = i + 1;
-            if (cut > 0.0 && (i & 7) == 7) {
-                if (__all(!ok || tau > cut)) break;
-            }
-        }
-    }
-    // no gradient: off the planet or a single point (S <= 1), or Tb at the T_cmb floor
-    const bool live = S > 1 && !(num < kTcmb);
-    R.last = __any(live && ok && g != 0.0) ? last : -1;
-    R.Tb = num / intW;
-    R.u = (live && ok && g != 0.0) ? g / intW : 0.0;
-}
-
-// Pass 2 of NR rays of one atmosphere (row0 shared), layers [0, kend]: per layer and ray
-//   c_k = h_{k-1} + h_k, gl = gTb lambda_k = c_k (T_k - Tb) u,  W_k = alpha_k e^-tau_k,
-//   P_k = P_{k-1} + gl W_k (gTb times the prefix of lambda W),
+// Both passes for NR rays of one atmosphere (first alpha row row0 = base x nf, temperatures Tl),
+// interleaved layer by layer so the rays share each alpha row and temperature load, and the
+// next layer's loads are issued one step ahead.
+// Pass 1, per ray: tau, the integrated weight D and the T-weighted sum (the recurrence of
+// integrate_kernel; the optical-depth cut checked every 8 steps per ray, off for a path with a
+// NaN half step or temperature), and the last path point reached.
+// Pass 2, per layer k and ray, along the same points with e^-tau recomputed:
+//   c_k = h_{k-1} + h_k, gl = gTb lambda_k = c_k (T_k - Tb) u with u = gTb / D,
+//   W_k = alpha_k e^-tau_k,  P_k = P_{k-1} + gl W_k (gTb times the prefix of lambda W),
 //   gTb dTb/dalpha_k = gl e^-tau_k + h_{k-1} P_{k-1} + h_k P_k,   gTb dTb/dT_k = c_k W_k u,
-// with h_{last} = 0 closing each ray's sums (the path end or the cut).
+// with h_last = 0 closing each ray's sums (the path end or the cut); the rays' terms add up in
+// registers and go to the accumulators with one atomic add per layer (gT summed over the
+// wavefront's frequencies first).
 template <int NR, bool FAST>
-__device__ inline void adj_backward(const AdjArgs &Aa, const AdjRay (&R)[NR], int kend, int j, int jj, bool ok) {
+__device__ inline void adj_rays(const AdjArgs &Aa, const AdjRay (&R)[NR], const double *Tl, size_t row0, int base,
+                                int j, int jj, bool ok) {
     const IntArgs &A = Aa.I;
     const size_t nf = A.nf;
-    const size_t row0 = R[0].row0;
-    const int base = R[0].base;
-    double tau[NR], P[NR], hp[NR];   // per ray: tau_k, P_{k-1}, h_{k-1}
+    int kmax = 0;                          // steps of the longest path
+    double cut[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        kmax = max(kmax, R[r].steps);
+        cut[r] = (A.tau_cut > 0.0 && R[r].steps > 0 && !path_has_nan(R[r].h, Tl, R[r].steps)) ? A.tau_cut : 0.0;
+    }
+    if (kmax <= 0) return;
+    // ---- pass 1 ----
+    double tau[NR], W0[NR], TW0[NR], iw[NR], nm[NR];
+    int last[NR];
+    bool run[NR];                          // wavefront-uniform
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        tau[r] = W0[r] = TW0[r] = iw[r] = nm[r] = 0.0;
+        last[r] = 0;
+        run[r] = R[r].steps > 0;
+    }
+    double a0 = load_alpha(A.alpha, A.f32, row0 + jj);
+    double a_n = load_alpha(A.alpha, A.f32, row0 + nf + jj), T_n = Tl[1];
+    double h_n[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) h_n[r] = R[r].steps > 0 ? R[r].h[0] : 0.0;
+    for (int i = 0; i < kmax; ++i) {
+        const double a1 = a_n, T1 = T_n;
+        double hh[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) hh[r] = h_n[r];
+        if (i + 1 < kmax) {   // the next step's loads, one step ahead
+            a_n = load_alpha(A.alpha, A.f32, row0 + static_cast<size_t>(i + 2) * nf + jj);
+            T_n = Tl[i + 2];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) h_n[r] = i + 1 < R[r].steps ? R[r].h[i + 1] : 0.0;
+        }
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (!run[r] || i >= R[r].steps) continue;
+            tau[r] = tau[r] + (a0 + a1) * hh[r];
+            const double W1 = a1 * exp_neg<FAST>(tau[r]);
+            iw[r] = iw[r] + (W1 + W0[r]) * hh[r];
+            const double TW1 = T1 * W1;
+            nm[r] = nm[r] + (TW1 + TW0[r]) * hh[r];
+            W0[r] = W1;
+            TW0[r] = TW1;
+            last[r] = i + 1;
+            if ((i & 7) == 7 && cut[r] > 0.0 && __all(!ok || tau[r] > cut[r])) run[r] = false;
+            any = true;
+        }
+        a0 = a1;
+        if (!any) break;
+    }
+    // per ray: Tb, u = gTb / D, and whether it has a gradient (on the planet, more than one
+    // point, Tb above the T_cmb floor, some lane with gTb != 0)
+    double Tb[NR], u[NR];
+    int kend = -1;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const bool live = R[r].steps > 0 && !(nm[r] < kTcmb) && ok && R[r].g != 0.0;
+        Tb[r] = nm[r] / iw[r];
+        u[r] = live ? R[r].g / iw[r] : 0.0;
+        if (!__any(live)) last[r] = -1;
+        kend = max(kend, last[r]);
+    }
+    if (kend < 0) return;
+    // ---- pass 2 ----
+    const int group = blockIdx.x % A.ngroups;
+    const int copy = static_cast<int>((blockIdx.x / A.ngroups) % Aa.copies);
+    double *acc_a = Aa.acc_alpha + static_cast<size_t>(copy) * A.nl * nf;
+    double *acc_t = Aa.acc_T + (static_cast<size_t>(copy) * A.ngroups + group) * A.nl;
+    double P[NR], hp[NR];   // per ray: P_{k-1}, h_{k-1}
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         tau[r] = 0.0;
         P[r] = 0.0;
         hp[r] = 0.0;
+        h_n[r] = last[r] > 0 ? R[r].h[0] : 0.0;
     }
     double a_prev = 0.0;
-    for (int k0 = 0; k0 <= kend; k0 += kAdjChunk) {
-        double accA[kAdjChunk], accT[kAdjChunk];
+    a_n = load_alpha(A.alpha, A.f32, row0 + jj);
+    T_n = Tl[0];
+    for (int k = 0; k <= kend; ++k) {
+        const double a_k = a_n, T_k = T_n;
+        double hk[NR];
 #pragma unroll
-        for (int u = 0; u < kAdjChunk; ++u) {
-            accA[u] = 0.0;
-            accT[u] = 0.0;
-            const int k = k0 + u;
-            if (k > kend) continue;   // wavefront-uniform
-            const double a_k = load_alpha(A.alpha, A.f32, row0 + static_cast<size_t>(k) * nf + jj);
-            const double T_k = R[0].Tl[k];
+        for (int r = 0; r < NR; ++r) hk[r] = k < last[r] ? h_n[r] : 0.0;   // the path end closes the sums
+        if (k < kend) {
+            a_n = load_alpha(A.alpha, A.f32, row0 + static_cast<size_t>(k + 1) * nf + jj);
+            T_n = Tl[k + 1];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                if (k > R[r].last) continue;   // wavefront-uniform: this ray has ended
-                const double hprev = hp[r];
-                const double hk = k < R[r].last ? R[r].h[k] : 0.0;
-                double e = 1.0, W = 0.0;
-                if (k > 0) {
-                    tau[r] = tau[r] + (a_prev + a_k) * hprev;
-                    e = exp_neg<FAST>(tau[r]);
-                    W = a_k * e;
-                }
-                const double ck = hprev + hk;
-                const double gl = ck * (T_k - R[r].Tb) * R[r].u;
-                const double Pn = P[r] + gl * W;
-                const double ga = (k > 0 ? gl * e : 0.0) + hprev * P[r] + hk * Pn;
-                const bool on = R[r].u != 0.0;
-                accA[u] += on ? ga : 0.0;
-                accT[u] += on ? ck * W * R[r].u : 0.0;
-                P[r] = Pn;
-                hp[r] = hk;
+            for (int r = 0; r < NR; ++r) h_n[r] = k + 1 < last[r] ? R[r].h[k + 1] : 0.0;
+        }
+        double sa = 0.0, st = 0.0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (k > last[r]) continue;   // wavefront-uniform: this ray has ended
+            double e = 1.0, W = 0.0;
+            if (k > 0) {
+                tau[r] = tau[r] + (a_prev + a_k) * hp[r];
+                e = exp_neg<FAST>(tau[r]);
+                W = a_k * e;
             }
-            a_prev = a_k;
+            const double ck = hp[r] + hk[r];
+            const double gl = ck * (T_k - Tb[r]) * u[r];
+            const double Pn = P[r] + gl * W;
+            const double ga = (k > 0 ? gl * e : 0.0) + hp[r] * P[r] + hk[r] * Pn;
+            const bool on = u[r] != 0.0;
+            sa += on ? ga : 0.0;
+            st += on ? ck * W * u[r] : 0.0;
+            P[r] = Pn;
+            hp[r] = hk[r];
         }
-        // flush the chunk: galpha rows by frequency, gT rows summed over the wavefront
-#pragma unroll
-        for (int u = 0; u < kAdjChunk; ++u) {
-            const int k = k0 + u;
-            if (k > kend) continue;
-            if (ok && accA[u] != 0.0) unsafeAtomicAdd(&Aa.galpha[row0 + static_cast<size_t>(k) * nf + j], accA[u]);
-            const double t = wave_sum(ok ? accT[u] : 0.0);
-            if (t != 0.0 && threadIdx.x % kLanes == 0) unsafeAtomicAdd(&Aa.gT[base + k], t);
-        }
+        a_prev = a_k;
+        if (ok && sa != 0.0) unsafeAtomicAdd(&acc_a[row0 + static_cast<size_t>(k) * nf + j], sa);
+        const double t = wave_sum(ok ? st : 0.0);
+        if (t != 0.0 && threadIdx.x % kLanes == 0) unsafeAtomicAdd(&acc_t[base + k], t);
     }
 }
 
-// ---- kernel ----
+// ---- kernel: wavefront w takes schedule slots [w NR, (w + 1) NR) ----
 template <int NR, bool FAST>
 __global__ __launch_bounds__(kWaves * kLanes) void adjoint_kernel(AdjArgs Aa) {
     const IntArgs &A = Aa.I;
     const int wave = threadIdx.x / kLanes;
     const int lane = threadIdx.x % kLanes;
     const int group = blockIdx.x % A.ngroups;
-    const int slot0 = ((blockIdx.x / A.ngroups) * kWaves + wave) * NR;   // first schedule slot
+    const int slot0 = ((blockIdx.x / A.ngroups) * kWaves + wave) * NR;
     const int j = group * kLanes + lane;
     const bool ok = j < A.nf;
     const int jj = ok ? j : A.nf - 1;
     AdjRay R[NR];
-    int kend = -1, base0 = -1;
+    int base0 = -1, bases[NR];
     bool same = true;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         R[r].h = A.hds;
-        R[r].Tl = A.T;
-        R[r].row0 = 0;
-        R[r].base = 0;
-        R[r].last = -1;
-        R[r].Tb = 0.0;
-        R[r].u = 0.0;
+        R[r].steps = 0;
+        R[r].g = 0.0;
+        bases[r] = -1;
         const int ray = ray_of(A, slot0 + r);
         if (ray < 0) continue;
         int base, nlr;
         if (!ray_span(A, ray, base, nlr)) continue;    // bad atmosphere index: Tb NaN, no gradient
-        const int S = min(A.nsteps[ray], nlr - 1);
         R[r].h = A.hds + static_cast<size_t>(ray) * A.ld;
-        R[r].Tl = A.T + base;
-        R[r].row0 = static_cast<size_t>(base) * A.nf;
-        R[r].base = base;
-        const double g = ok ? Aa.gTb[static_cast<size_t>(ray) * A.nf + j] : 0.0;
-        adj_forward<FAST>(A, R[r], S, jj, ok, g);
-        if (R[r].last < 0) continue;
+        R[r].steps = min(A.nsteps[ray], nlr - 1) - 1;
+        R[r].g = ok ? Aa.gTb[static_cast<size_t>(ray) * A.nf + j] : 0.0;
+        bases[r] = base;
+        if (R[r].steps <= 0) continue;
         if (base0 < 0)
             base0 = base;
         else if (base != base0)
             same = false;
-        kend = max(kend, R[r].last);
     }
-    if (kend < 0) return;
+    if (base0 < 0) return;
     if (same) {
-        adj_backward<NR, FAST>(Aa, R, kend, j, jj, ok);
-    } else {   // rays of several atmospheres in one wavefront: one at a time
-#pragma unroll
+        adj_rays<NR, FAST>(Aa, R, A.T + base0, static_cast<size_t>(base0) * A.nf, base0, j, jj, ok);
+    } else {   // rays of several atmospheres in one wavefront: one at a time (one copy of the code)
+#pragma unroll 1
         for (int r = 0; r < NR; ++r) {
-            if (R[r].last < 0) continue;
-            const AdjRay one[1] = {R[r]};
-            adj_backward<1, FAST>(Aa, one, R[r].last, j, jj, ok);
+            AdjRay one[1] = {R[0]};
+            int b = bases[0];
+#pragma unroll
+            for (int q = 1; q < NR; ++q) {
+                if (q == r) {
+                    one[0] = R[q];
+                    b = bases[q];
+                }
+            }
+            if (one[0].steps <= 0) continue;
+            adj_rays<1, FAST>(Aa, one, A.T + b, static_cast<size_t>(b) * A.nf, b, j, jj, ok);
         }
     }
 }
 
-__global__ __launch_bounds__(256) void adjoint_clear_kernel(double *galpha, size_t n, double *gT, int nl) {
+__global__ __launch_bounds__(256) void adjoint_clear_kernel(double *acc_alpha, size_t na, double *acc_T, size_t nt) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) galpha[i] = 0.0;
-    if (i < static_cast<size_t>(nl)) gT[i] = 0.0;
+    if (i < na) acc_alpha[i] = 0.0;
+    if (i < nt) acc_T[i] = 0.0;
+}
+
+// galpha = the sum of the copies (in copy order); gT[l] = the sum over copies and frequency groups
+__global__ __launch_bounds__(256) void adjoint_reduce_kernel(const double *acc_alpha, const double *acc_T, int copies,
+                                                             int parts_T, size_t n, int nl, double *galpha,
+                                                             double *gT) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && galpha != acc_alpha) {
+        double s = 0.0;
+        for (int c = 0; c < copies; ++c) s += acc_alpha[static_cast<size_t>(c) * n + i];
+        galpha[i] = s;
+    }
+    if (i < static_cast<size_t>(nl)) {
+        double s = 0.0;
+        for (int c = 0; c < parts_T; ++c) s += acc_T[static_cast<size_t>(c) * nl + i];
+        gT[i] = s;
+    }
 }
 
 // grad[m][l] = sum_f galpha[l][f] dalpha[m][l][f] (+ gT[l] for m = 0): one wavefront per layer
@@ -851,20 +944,57 @@ extern "C" int rbx_integrate_profiles(const void *alpha, int alpha_precision, co
     return rbx::launch_status("profile_kernel");
 }
 
-static int launch_adjoint(rbx::AdjArgs &Aa, const rbx_launch &L, void *stream) {
-    const int nr = L.adj_rays ? L.adj_rays : 8;
+namespace rbx {
+// Rays per wavefront and copies of the accumulators of one adjoint call: launch.adj_rays
+// (built in 4) and launch.adj_copies, else about one copy per 4 ray blocks of an atmosphere,
+// at most 32 (a 16384-ray image of one atmosphere: 32; retrieval batches of 16 rays per
+// atmosphere: 1, their atmospheres' rows already differ).
+static void adjoint_plan(int nl, int nf, int nray, int natm, const rbx_launch &L, int &nr, int &copies,
+                         size_t &bytes) {
+    nr = L.adj_rays ? L.adj_rays : 4;
+    const long long per_atm = (static_cast<long long>(nray) + natm - 1) / std::max(natm, 1);
+    const long long blocks_per_atm = (per_atm + static_cast<long long>(kWaves) * nr - 1) / (kWaves * nr);
+    copies = L.adj_copies ? L.adj_copies : static_cast<int>(std::min<long long>(32, std::max(1LL, blocks_per_atm / 4)));
+    const int ngroups = (nf + kLanes - 1) / kLanes;
+    const size_t n = static_cast<size_t>(nl) * nf;
+    bytes = 0;
+    if (copies > 1 || ngroups > 1)
+        bytes = ((copies > 1 ? static_cast<size_t>(copies) * n : 0) + static_cast<size_t>(copies) * ngroups * nl) *
+                sizeof(double);
+}
+}  // namespace rbx
+
+static int launch_adjoint(rbx::AdjArgs &Aa, const rbx_launch &L, int natm, void *workspace, size_t workspace_bytes,
+                          void *stream) {
+    int nr, copies;
+    size_t need;
+    rbx::adjoint_plan(Aa.I.nl, Aa.I.nf, Aa.I.nray, natm, L, nr, copies, need);
+    RBX_CHECK_ARG(need == 0 || (workspace && workspace_bytes >= need),
+                  "rbx_integrate_adjoint: workspace of %zu bytes, this call needs %zu "
+                  "(rbx_integrate_adjoint_workspace_size)",
+                  workspace_bytes, need);
+    RBX_CHECK_ARG(need == 0 || reinterpret_cast<uintptr_t>(workspace) % 8 == 0,
+                  "rbx_integrate_adjoint: workspace not 8-byte aligned");
     const long long slots = (static_cast<long long>(Aa.I.nray) + static_cast<long long>(rbx::kWaves) * nr - 1) /
                             (static_cast<long long>(rbx::kWaves) * nr);
     const long long blocks = static_cast<long long>(Aa.I.ngroups) * slots;
     RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_integrate_adjoint: too many blocks");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const size_t n = static_cast<size_t>(Aa.I.nl) * Aa.I.nf;
-    const size_t nclear = std::max(n, static_cast<size_t>(Aa.I.nl));
+    const int parts_T = copies * Aa.I.ngroups;
+    Aa.copies = copies;
+    double *ws = static_cast<double *>(workspace);
+    Aa.acc_alpha = copies > 1 ? ws : Aa.galpha;
+    Aa.acc_T = need > 0 ? ws + (copies > 1 ? static_cast<size_t>(copies) * n : 0) : Aa.gT;
+    const size_t na = (copies > 1 ? static_cast<size_t>(copies) : 1) * n;
+    const size_t nt = static_cast<size_t>(parts_T) * Aa.I.nl;
+    const size_t nclear = std::max(na, nt);
     hipLaunchKernelGGL(rbx::adjoint_clear_kernel, dim3(static_cast<unsigned>((nclear + 255) / 256)), dim3(256), 0, st,
-                       Aa.galpha, n, Aa.gT, Aa.I.nl);
+                       Aa.acc_alpha, na, Aa.acc_T, nt);
     int rc = rbx::launch_status("adjoint_clear_kernel");
-    if (rc || blocks == 0) return rc;
-    const dim3 grid(static_cast<unsigned>(blocks)), block(rbx::kWaves * rbx::kLanes);
+    if (rc) return rc;
+    if (blocks > 0) {
+        const dim3 grid(static_cast<unsigned>(blocks)), block(rbx::kWaves * rbx::kLanes);
 #define RBX_ADJ_LAUNCH(NR)                                                                   \
     do {                                                                                     \
         if (Aa.I.fast_exp)                                                                   \
@@ -872,20 +1002,41 @@ static int launch_adjoint(rbx::AdjArgs &Aa, const rbx_launch &L, void *stream) {
         else                                                                                 \
             hipLaunchKernelGGL((rbx::adjoint_kernel<NR, false>), grid, block, 0, st, Aa);    \
     } while (0)
-    switch (nr) {
-        case 1: RBX_ADJ_LAUNCH(1); break;
-        case 2: RBX_ADJ_LAUNCH(2); break;
-        case 4: RBX_ADJ_LAUNCH(4); break;
-        default: RBX_ADJ_LAUNCH(8); break;
-    }
+        switch (nr) {
+            case 1: RBX_ADJ_LAUNCH(1); break;
+            case 2: RBX_ADJ_LAUNCH(2); break;
+            case 4: RBX_ADJ_LAUNCH(4); break;
+            default: RBX_ADJ_LAUNCH(8); break;
+        }
 #undef RBX_ADJ_LAUNCH
-    return rbx::launch_status("adjoint_kernel");
+        rc = rbx::launch_status("adjoint_kernel");
+        if (rc) return rc;
+    }
+    if (need == 0) return RBX_OK;   // one copy, one frequency group: the sums are galpha and gT
+    const size_t nred = std::max(n, static_cast<size_t>(Aa.I.nl));
+    hipLaunchKernelGGL(rbx::adjoint_reduce_kernel, dim3(static_cast<unsigned>((nred + 255) / 256)), dim3(256), 0, st,
+                       Aa.acc_alpha, Aa.acc_T, copies, parts_T, n, Aa.I.nl, Aa.galpha, Aa.gT);
+    return rbx::launch_status("adjoint_reduce_kernel");
+}
+
+extern "C" int rbx_integrate_adjoint_workspace_size(int nl, int nf, int nray, int natm, const rbx_launch *launch,
+                                                    size_t *bytes) {
+    RBX_CHECK_ARG(bytes, "rbx_integrate_adjoint_workspace_size: null output");
+    RBX_CHECK_ARG(nl >= 2 && nf > 0 && nray >= 0 && natm >= 1, "rbx_integrate_adjoint_workspace_size: nl=%d nf=%d "
+                  "nray=%d natm=%d", nl, nf, nray, natm);
+    rbx_launch L;
+    int rc = rbx::read_launch(launch, L, "rbx_integrate_adjoint_workspace_size");
+    if (rc) return rc;
+    int nr, copies;
+    rbx::adjoint_plan(nl, nf, nray, natm, L, nr, copies, *bytes);
+    return RBX_OK;
 }
 
 extern "C" int rbx_integrate_adjoint(const void *alpha, int alpha_precision, const double *T, int nl, int nf,
                                      const double *hds, int ld, const int *nsteps, int nray, const int *order,
                                      int precision, double tau_cut, const double *gTb, double *galpha, double *gT,
-                                     const rbx_launch *launch, void *stream) {
+                                     const rbx_launch *launch, void *workspace, size_t workspace_bytes,
+                                     void *stream) {
     int rc = rbx::check_int_args(alpha, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
     if (rc) return rc;
     rbx_launch L;
@@ -899,14 +1050,15 @@ extern "C" int rbx_integrate_adjoint(const void *alpha, int alpha_precision, con
     Aa.gTb = gTb;
     Aa.galpha = galpha;
     Aa.gT = gT;
-    return launch_adjoint(Aa, L, stream);
+    return launch_adjoint(Aa, L, 1, workspace, workspace_bytes, stream);
 }
 
 extern "C" int rbx_integrate_adjoint_batch(const void *alpha, int alpha_precision, const double *T, int nl_total,
                                            int nf, const int *layer_off, int natm, const double *hds, int ld,
                                            const int *nsteps, const int *ray_atm, const int *order, int nray,
                                            int precision, double tau_cut, const double *gTb, double *galpha,
-                                           double *gT, const rbx_launch *launch, void *stream) {
+                                           double *gT, const rbx_launch *launch, void *workspace,
+                                           size_t workspace_bytes, void *stream) {
     int rc = rbx::check_int_args(alpha, alpha_precision, T, nl_total, nf, hds, ld, nsteps, nray, precision);
     if (rc) return rc;
     rbx_launch L;
@@ -925,7 +1077,7 @@ extern "C" int rbx_integrate_adjoint_batch(const void *alpha, int alpha_precisio
     Aa.gTb = gTb;
     Aa.galpha = galpha;
     Aa.gT = gT;
-    return launch_adjoint(Aa, L, stream);
+    return launch_adjoint(Aa, L, natm, workspace, workspace_bytes, stream);
 }
 
 extern "C" int rbx_alpha_vjp(const double *dalpha, const double *galpha, const double *gT, int nl, int nf,

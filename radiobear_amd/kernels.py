@@ -40,6 +40,25 @@ def as_device(x, dtype=torch.float64):
     return t.contiguous()
 
 
+_CONSTS = {}          # small host arrays already copied to the device (request frequencies)
+_CONSTS_MAX = 64
+
+
+def device_const(a):
+    """A float64 device copy of a small host array, kept and reused for equal contents, so
+    that repeated calls (and calls under HIP graph capture after a first eager one) copy
+    nothing from the host."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    key = (torch.cuda.current_device(), a.shape, a.tobytes())
+    t = _CONSTS.get(key)
+    if t is None:
+        if len(_CONSTS) >= _CONSTS_MAX:
+            _CONSTS.pop(next(iter(_CONSTS)))
+        t = torch.as_tensor(a, device=device())
+        _CONSTS[key] = t
+    return t
+
+
 # ------------------------------------------------------------------ catalogue
 class Catalog:
     """Device-resident line lists (rbx_catalog)."""
@@ -187,8 +206,8 @@ def _alpha_inputs(P, T, mix, freqs, scale):
     if T.shape != (nl,) or mix.shape != (len(_lib.SPECIES), nl):
         raise ValueError('shape mismatch: P {}, T {}, mix {}'.format(tuple(P.shape), tuple(T.shape),
                                                                      tuple(mix.shape)))
-    fd = as_device(f)
-    fn = None if np.array_equal(order, np.arange(len(f))) else as_device(f[order])
+    fd = device_const(f)
+    fn = None if np.array_equal(order, np.arange(len(f))) else device_const(f[order])
     sc = None
     if scale is not None:
         sc = as_device(scale)
@@ -412,6 +431,14 @@ def path_order(nsteps):
     return torch.argsort(nsteps.to(torch.int64), descending=True, stable=True).to(torch.int32)
 
 
+def _adjoint_workspace(nl, nf, nray, natm, launch):
+    nb = ctypes.c_size_t(0)
+    check(lib().rbx_integrate_adjoint_workspace_size(nl, nf, nray, natm, _lib.launch_ref(launch), ctypes.byref(nb)))
+    if nb.value == 0:
+        return None, 0
+    return torch.empty(((nb.value + 7) // 8,), dtype=torch.float64, device=device()), nb.value
+
+
 def integrate_adjoint(alpha_table, T, hds, nsteps, gTb, order=None, precision=None, tau_cut=0.0, launch=None):
     """Vector-Jacobian product of integrate(): (galpha (nl, nf), gT (nl,)) float64, the gradient
     of L = sum gTb * Tb with respect to the alpha table and (directly) the layer temperatures."""
@@ -426,9 +453,10 @@ def integrate_adjoint(alpha_table, T, hds, nsteps, gTb, order=None, precision=No
     od = path_order(nsteps) if order is None else (order if isinstance(order, torch.Tensor) else _i32(order))
     galpha = torch.empty((nl, nf), dtype=torch.float64, device=device())
     gT = torch.empty((nl,), dtype=torch.float64, device=device())
+    ws, nb = _adjoint_workspace(nl, nf, nray, 1, launch)
     check(lib().rbx_integrate_adjoint(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds), ld, _ptr(nsteps), nray, _ptr(od),
                                       prec, float(tau_cut), _ptr(g), _ptr(galpha), _ptr(gT),
-                                      _lib.launch_ref(launch), _stream()))
+                                      _lib.launch_ref(launch), _ptr(ws), nb, _stream()))
     return galpha, gT
 
 
@@ -450,10 +478,11 @@ def integrate_adjoint_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, gTb
     od = None if order is None else (order if isinstance(order, torch.Tensor) else _i32(order))
     galpha = torch.empty((nl, nf), dtype=torch.float64, device=device())
     gT = torch.empty((nl,), dtype=torch.float64, device=device())
+    ws, nb = _adjoint_workspace(nl, nf, nray, off.shape[0] - 1, launch)
     check(lib().rbx_integrate_adjoint_batch(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(off), off.shape[0] - 1,
                                             _ptr(hds), ld, _ptr(nsteps), _ptr(ra), _ptr(od), nray, prec,
                                             float(tau_cut), _ptr(g), _ptr(galpha), _ptr(gT),
-                                            _lib.launch_ref(launch), _stream()))
+                                            _lib.launch_ref(launch), _ptr(ws), nb, _stream()))
     return galpha, gT
 
 
@@ -592,9 +621,10 @@ def integrate_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, order=None,
     return Tb
 
 
-def atmosphere_batch(P, T, mix, layer_off, req, gm, p_ref=1.0, Z=None):
+def atmosphere_batch(P, T, mix, layer_off, req, gm, p_ref=1.0, Z=None, off_dev=None):
     """Z, R, N (each (nl_total,) float64 on the device) of atmospheres laid end to end;
-    Z None: hydrostatic z (rbx_atmosphere_batch)."""
+    Z None: hydrostatic z (rbx_atmosphere_batch).  layer_off: host offsets (checked here);
+    off_dev: their int32 device copy if the caller holds one (then nothing is copied)."""
     P, T, mix = as_device(P), as_device(T), as_device(mix)
     nlt = P.shape[0]
     if T.shape != (nlt,) or mix.shape != (len(_lib.SPECIES), nlt):
@@ -603,7 +633,7 @@ def atmosphere_batch(P, T, mix, layer_off, req, gm, p_ref=1.0, Z=None):
     off = np.asarray(layer_off.cpu() if isinstance(layer_off, torch.Tensor) else layer_off, dtype=np.int64)
     if off[0] != 0 or off[-1] != nlt or np.any(np.diff(off) < 1):
         raise ValueError('layer offsets must run from 0 to nl_total')
-    off_d = _i32(off)
+    off_d = _i32(off) if off_dev is None else off_dev
     Zin = None if Z is None else as_device(Z)
     Zo, R, N = (torch.empty((nlt,), dtype=torch.float64, device=device()) for _ in range(3))
     check(lib().rbx_atmosphere_batch(_ptr(P), _ptr(T), _ptr(mix), _ptr(Zin), nlt, _ptr(off_d), len(off) - 1,

@@ -84,7 +84,7 @@ class RetrievalBatch:
         if mix is not None:
             self.mix = K.as_device(mix)
         self.Z, self.R, self.N = K.atmosphere_batch(self.P, self.T, self.mix, self.layer_off, self.req, self.gm,
-                                                    self.p_ref)
+                                                    self.p_ref, off_dev=self.off_d)
 
     def absorption(self):
         """alpha (nl_total, nf) and dalpha (3, nl_total, nf), float64, one fused pass."""
@@ -113,6 +113,30 @@ class RetrievalBatch:
         J, Tb = K.jacobian_batch(alpha, dalpha, self.T, self.off_d, self.ray_atm_d, hds, ns, self.ldj,
                                  order=self.order_d, tau_cut=self.tau_cut, precision=self.precision)
         return Tb, J
+
+    def step(self, rebuild=False):
+        """One retrieval step: (optionally) the shells rebuilt on the device for the current
+        T and mix, then run().  Returns (Tb, J)."""
+        if rebuild:
+            self.update()
+        return self.run()
+
+    def capture(self, rebuild=False):
+        """The whole step captured once in a HIP graph (torch.cuda.CUDAGraph): the kernels of
+        step() replay with no host work per launch.  Returns (graph, (Tb, J)); each
+        graph.replay() recomputes Tb and J in place from the batch's current device inputs
+        (T, mix and, with rebuild, the shells), which a sampler updates with copy_()."""
+        import torch
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            self.step(rebuild)                     # eager warm-up: allocations and the side stream
+        torch.cuda.current_stream().wait_stream(stream)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            out = self.step(rebuild)
+        return graph, out
 
     def model(self, hds=None, nsteps=None):
         """A differentiable forward model of the batch (adjoint.ForwardModel): Tb (nray, nf)

@@ -270,13 +270,77 @@ def fixture_benchmark():
         os.chdir(cwd)
 
 
+FREQ_REQUESTS = [([1.0, 2.5, 30.0], 'GHz'), ('1,2,3.5', 'GHz'), ('1:96:5', 'GHz'), ('0.3:2.0:0.1', 'GHz'),
+                 ('5', 'GHz'), (7, 'GHz'), (2.5, 'GHz'), ('100:3000:250', 'MHz'), ('0.5,1.25', 'cm'),
+                 ('0.3;300;7', 'GHz')]
+B_REQUESTS = [[0.0, 0.0], [[0.0, 0.0], [0.2, 0.1]], [[0.1 * i, 0.0] for i in range(7)], 'disc', 'Disk',
+              'stamp:0.1:-0.2,0.2,-0.3,0.1', 'stamp:0.25:0,0.5,-0.5,0.5', '0:0.9:0.1', '0:1.2:0.05<30',
+              '0.1,0.5,0.99<90', '0.2,0.98,0.999,1.1', '0:1:0.2<45', '0.3;0.9;4']
+
+
+def fixture_requests():
+    """F8: the reference's request parsing (set_freq, set_b) and result files (FileIO) as text:
+    tests/golden/requests.json holds each request with the reference's output (or the name of
+    the exception it raised), and the three result layouts written for fixed Data."""
+    import json
+    import tempfile
+    cwd = os.getcwd()
+    os.chdir(RB)
+    try:
+        from radiobear import set_utils, fileIO, data_handling
+        out = {'set_freq': [], 'set_b': [], 'files': {}}
+        for req, unit in FREQ_REQUESTS:
+            try:
+                f, u = set_utils.set_freq(req if not isinstance(req, list) else list(req), unit)
+                out['set_freq'].append({'request': req, 'unit': unit, 'freqs': [float(x) for x in f], 'out_unit': u})
+            except Exception as e:   # recorded: the build documents each deviation
+                out['set_freq'].append({'request': req, 'unit': unit, 'raises': type(e).__name__})
+        for req in B_REQUESTS + [0.25]:
+            rec = {'request': req, 'block': [1, 1]}
+            try:
+                r = set_utils.set_b(req, block=[1, 1], Rpol=66854.0, Req=71492.0)
+                rec.update({'b': [[float(v) for v in p] if not isinstance(p, str) else p for p in r.b],
+                            'data_type': r.data_type, 'imSize': r.imSize})
+            except Exception as e:
+                rec['raises'] = type(e).__name__
+            out['set_b'].append(rec)
+        tb = np.array([[134.2567, 151.5, 172.25001], [133.0, 150.75, 171.5], [131.125, 149.875, 170.0625]])
+        with tempfile.TemporaryDirectory() as tmp:
+            os.chdir(tmp)
+            os.mkdir('Scratch')
+            for kind, b in (('spectrum', [[0.0, 0.0], [0.45, 0.1], [0.7, -0.2]]), ('profile', [[0.0, 0.0],
+                            [0.45, 0.1], [0.7, -0.2]]), ('image', [[0.0, 0.0], [0.45, 0.1], [0.7, -0.2]]),
+                            ('spectrum-disc', 'disc')):
+                for xaxis in ('frequency', 'wavelength'):
+                    d = data_handling.Data()
+                    d.set('header', {'a_planet': '# jupiter', 'b_freqs': '# freqs: 1.5 22.2 40', 'c_x': '# x'})
+                    d.set('type', kind.split('-')[0])
+                    d.set('f', [1.5, 22.2, 40.0])
+                    d.set('freqUnit', 'GHz')
+                    d.set('b', b)
+                    d.set('Tb', tb.tolist() if kind != 'spectrum-disc' else tb[:1].tolist())
+                    name = '{}_{}.dat'.format(kind, xaxis)
+                    fileIO.FileIO().write(name, d, xaxis=xaxis)
+                    with open(name) as fp:
+                        out['files'][name] = fp.read()
+            os.chdir(RB)
+        with open(os.path.join(HERE, 'requests.json'), 'w') as fp:
+            json.dump(out, fp, indent=1)
+    finally:
+        os.chdir(cwd)
+
+
 if __name__ == '__main__':
     if sys.argv[1:] == ['other']:
         fixture_other_absorbers()
+        sys.exit(0)
+    if sys.argv[1:] == ['requests']:
+        fixture_requests()
         sys.exit(0)
     fixture_absorbers()
     fixture_other_absorbers()
     fixture_synthetic_lines()
     fixture_geometry()
     fixture_benchmark()
+    fixture_requests()
     print('fixtures written to', HERE)

@@ -153,13 +153,16 @@ def test_masked_rays_and_nan_paths(K, case):
 @pytest.mark.parametrize('prec', ['fp64', 'fp32'])
 def test_rays_per_wavefront_agree(K, case, prec):
     """1, 2, 4 or 8 rays summed per wavefront before the atomic adds, rays in path order or
-    in input order: the same gradient up to the order of FP64 additions."""
+    in input order, the adds spread over 1, 3 or 32 copies of the accumulators: the same
+    gradient up to the order of FP64 additions."""
     import torch
     atm, cat, ab, f, rays, hds, ns, g = case
     out = []
     for nr in (1, 2, 4, 8):
         for order in (None, torch.arange(len(rays), dtype=torch.int32, device='cuda')):
             out.append(grads(K, model_of(K, case, prec, launch={'adj_rays': nr}, order=order), atm, g)[1])
+    for copies in (1, 3, 32):                      # accumulator copies: summed at the end
+        out.append(grads(K, model_of(K, case, prec, launch={'adj_rays': 1, 'adj_copies': copies}), atm, g)[1])
     for o in out[1:]:
         for m in range(3):
             assert row_err(o[m], out[0][m]) < 1e-12, m

@@ -132,3 +132,22 @@ def test_truncation_matches_oracle(raw_lines):
         assert np.array_equal(a[k], b[k])
     assert len(lines.truncate(raw_lines['h2s'], ('f0', 'I0', 'E', 'GH2S'), 1e-22)['f0']) == 121
     assert len(lines.truncate_ph3(raw_lines['ph3jh'], raw_lines['PH3WGT'], 1e-22)['f0']) == 33
+
+
+def test_adjoint_workspace_plan():
+    """The adjoint's accumulator copies (host planning, no GPU work): a large image of one
+    atmosphere spreads its atomic adds over 32 copies, a retrieval batch of 16 rays per
+    atmosphere over one; a single 64-column frequency group needs no workspace at one copy."""
+    lib = _lib.lib()
+    nb = ctypes.c_size_t(0)
+
+    def size(nl, nf, nray, natm, launch=None):
+        assert lib.rbx_integrate_adjoint_workspace_size(nl, nf, nray, natm, _lib.launch_ref(launch),
+                                                        ctypes.byref(nb)) == 0
+        return nb.value
+
+    assert size(800, 1024, 16384, 1) == (32 * 800 * 1024 + 32 * 16 * 800) * 8
+    assert size(500, 128, 1024 * 16, 1024) == (1 * 2 * 500) * 8          # one copy; gT per frequency group
+    assert size(100, 64, 8, 1) == 0                                       # one copy, one group
+    assert size(100, 64, 8, 1, {'adj_copies': 4}) == (4 * 100 * 64 + 4 * 100) * 8
+    assert lib.rbx_integrate_adjoint_workspace_size(1, 64, 8, 1, None, ctypes.byref(nb)) == -1
