@@ -158,7 +158,8 @@ def test_integrate_fixture(K, golden, tag):
     assert np.array_equal(np.isnan(Tb), np.isnan(ref))
     ok = ~np.isnan(ref)
     assert np.max(np.abs(Tb[ok] - ref[ok])) < 1e-8
-    Tb32 = K.integrate(tab.float(), G['gas'][C['T']], hds, ns, precision='fp32', tau_cut=50.0).cpu().numpy()
+    Tb32 = K.integrate(tab.float(), G['gas'][C['T']], hds, ns, precision='fp32',
+                       tau_cut=K.FP32_TAU_CUT).cpu().numpy()
     assert np.max(np.abs(Tb32[ok] - ref[ok])) < 1e-3
 
 
@@ -187,7 +188,7 @@ def test_benchmark_T_rb_on_gpu(K, jcat, bench_atm, golden, precision):
     pl = K.planet_desc(pipeline.JUPITER_REQ, pipeline.JUPITER_RPOL)
     hds, ns = K.raypaths(pl, prop['R'], prop['N'], pipeline.BENCH_B)
     Tb = K.integrate(tab, gas[C['T']], hds, ns, precision=precision,
-                     tau_cut=50.0 if precision == 'fp32' else 0.0).cpu().numpy()
+                     tau_cut=K.FP32_TAU_CUT if precision == 'fp32' else K.FP64_TAU_CUT).cpu().numpy()
     assert np.max(np.abs(Tb - T_RB)) < 0.01
     ref = golden('benchmark.npz')['Tb']
     assert np.max(np.abs(Tb - ref)) < (1e-6 if precision == 'fp64' else 2e-3)
@@ -290,11 +291,18 @@ def test_jacobian_vs_torch_autograd(K, raw_lines, jupiter_lines, alpha_precision
         assert err < 1e-3, (slot, err)
 
 
+# the launch shape bench.py's autotuner picks on the large workload (both precisions)
+BENCH_LAUNCH = {'alpha_series': 2, 'alpha_k': 2, 'far_split': 4, 'int_unroll': 8, 'int_map': 2}
+
+
 @pytest.mark.parametrize('name', ['c2_image', 'c3_large'])
 def test_workload_full_shape_sampled(K, raw_lines, jupiter_lines, name):
     """The image and large workloads at their full shapes (all layers, lines, freqs, rays on
-    the GPU), checked on a sample the oracle finishes quickly: 3 layers of alpha at every
-    frequency, and 4 rays integrated over the GPU FP64 table with the oracle's geometry."""
+    the GPU) with the launch shapes the bench's autotuner picks for the large workload and the
+    production optical-depth cuts (FP32 20, FP64 40), checked on a sample the oracle finishes
+    quickly: 3 layers of alpha at every frequency (both precisions, from the FP32 and FP64
+    tables the bench times), and 4 rays integrated over those tables with the oracle's
+    geometry.  Also the built-in launch shapes, FP32 table within 1e-6 of the tuned one."""
     import torch
     w = synth.workload(name)
     atm = w['atm']
@@ -309,8 +317,11 @@ def test_workload_full_shape_sampled(K, raw_lines, jupiter_lines, name):
     ab = K.absorbers_struct(JUPITER_ALPHA, 'e')
     mix = K.mix_table(atm.gas, C)
     f = w['freqs']
-    a64 = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp64')
-    a32 = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp32')
+    L = BENCH_LAUNCH
+    a64 = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp64', launch=L)
+    a32 = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp32', launch=L)
+    built_in = K.alpha(cat, ab, atm.P, atm.T, mix, f, 'fp32').double().cpu().numpy()
+    assert rel(built_in, a32.double().cpu().numpy()) < 1e-6
     sel = [0, atm.nl // 2, atm.nl - 1]
     ref = oracle_alpha(atm.gas[:, sel], f, lines).T                      # (3, nf)
     assert rel(a64.cpu().numpy()[sel], ref) < 1e-9
@@ -318,8 +329,8 @@ def test_workload_full_shape_sampled(K, raw_lines, jupiter_lines, name):
     rays = w['rays']
     pl = K.planet_desc(atm.Req, atm.Rpol)
     hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
-    Tb = K.integrate(a64, atm.T, hds, ns, precision='fp64').cpu().numpy()
-    Tb32 = K.integrate(a32, atm.T, hds, ns, precision='fp32', tau_cut=50.0).cpu().numpy()
+    Tb = K.integrate(a64, atm.T, hds, ns, precision='fp64', tau_cut=K.FP64_TAU_CUT, launch=L).cpu().numpy()
+    Tb32 = K.integrate(a32, atm.T, hds, ns, precision='fp32', tau_cut=K.FP32_TAU_CUT, launch=L).cpu().numpy()
     tab = a64.cpu().numpy().T                                            # (nf, nl)
     nsn = ns.cpu().numpy()
     on = np.flatnonzero(nsn > 0)
