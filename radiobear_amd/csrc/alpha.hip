@@ -135,6 +135,20 @@ struct AlphaArgs {
     int tan_seg0[6];
 };
 
+// One line's catalogue columns held in registers, read by the coefficient sets below the
+// way they read a DevBlock (B.col[c][i] gives column c of this line whatever i is).
+template <int NC>
+struct LineRow {
+    struct Col {
+        double v;
+        __device__ double operator[](int) const { return v; }
+    } col[NC];
+    __device__ void load(const DevBlock &B, int i) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) col[c].v = B.col[c][i];
+    }
+};
+
 // ---- per-layer coefficient sets: each derives (D, a, c, g, G) of one line ----
 template <class S>
 struct InvPrep {   // nh3_hs inversion lines (nh3_hs.py:198-255)
@@ -153,11 +167,13 @@ struct InvPrep {   // nh3_hs inversion lines (nh3_hs.py:198-255)
     }
     // the two coefficient sets share eT, so a line's weight before pref is common to both
     static constexpr bool kShared = true;
-    __device__ S weight(const DevBlock &B, int i) const {
+    template <class Blk>
+    __device__ S weight(const Blk &B, int i) const {
         const double fo = B.col[0][i];
         return exp(eT * B.col[2][i]) * (B.col[1][i] / (fo * fo));
     }
-    __device__ CoefT<S> coef(const DevBlock &B, int i, const S &w) const {
+    template <class Blk>
+    __device__ CoefT<S> coef(const Blk &B, int i, const S &w) const {
         const double fo = B.col[0][i], go = B.col[3][i];
         S gam = Ag + Bg * go;
         S zet = Az + Bz * go;
@@ -166,7 +182,8 @@ struct InvPrep {   // nh3_hs inversion lines (nh3_hs.py:198-255)
         S Dd = fd * fd + gam * gam - zet * zet;
         return {Dd, W * (gam - zet), W * (gam + zet) * Dd, 4.0 * gam * gam, S(0.0)};
     }
-    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const { return coef(B, i, weight(B, i)); }
+    template <class Blk>
+    __device__ CoefT<S> operator()(const Blk &B, int i, S &) const { return coef(B, i, weight(B, i)); }
 };
 
 template <class S>
@@ -181,7 +198,8 @@ struct RotPrep {   // nh3_hs rotational lines, Gross (nh3_hs.py:257-291)
         eT = (1.0 / 300.0 - 1.0 / L.T) * kHsHcKb;
         pref = 2.4268 * kHsCoef * (L.nh3 / 300.0) * pow(300.0 / L.T, 3.5) * ((1.0 / kGHzToInvCm) * (4.0 / kPi));
     }
-    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+    template <class Blk>
+    __device__ CoefT<S> operator()(const Blk &B, int i, S &) const {
         const double fo = B.col[0][i];
         S gam = c1 * B.col[4][i] + c2 * B.col[5][i] + c3 * B.col[3][i];
         S W = pref * B.col[1][i] * exp(eT * B.col[2][i]);
@@ -199,7 +217,8 @@ struct V2Prep {    // nh3_hs v2 roto-vibrational lines, Gross (nh3_hs.py:293-328
         eT = (1.0 / 300.0 - 1.0 / L.T) * kHsHcKb;
         pref = 1.1206 * kHsCoef * (L.nh3 / 300.0) * pow(300.0 / L.T, 3.5) * ((1.0 / kGHzToInvCm) * (4.0 / kPi));
     }
-    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+    template <class Blk>
+    __device__ CoefT<S> operator()(const Blk &B, int i, S &) const {
         const double fo = B.col[0][i];
         S W = pref * B.col[1][i] * exp(eT * B.col[2][i]);
         return {S(fo * fo), S(0.0), W * gam, 4.0 * gam * gam, S(0.0)};
@@ -219,7 +238,8 @@ struct H2sPrep {   // h2s_ddb, Ben-Reuven with zeta = gamma (h2s_ddb.py:42-87)
         eT = -((1.0 / L.T) - (1.0 / T0)) * kHck;
         pref = kGeisa * (L.h2s / T0) * pow(T0 / L.T, 3.5) * (kGHz * 2.0 / kPi);
     }
-    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+    template <class Blk>
+    __device__ CoefT<S> operator()(const Blk &B, int i, S &) const {
         const double f0 = B.col[0][i];
         S gam = Ag + Bg * B.col[3][i];
         S W = pref * B.col[1][i] * exp(eT * B.col[2][i]) / (f0 * f0);
@@ -240,7 +260,8 @@ struct Ph3Prep {   // ph3_jh, Ben-Reuven with zeta = delta = 0 (ph3_jh.py:64-108
         eT = -((1.0 / L.T) - (1.0 / T0)) * kHck;
         pref = kGeisa * (L.ph3 / T0) * pow(T0 / L.T, 3.5) * (kGHz * 2.0 / kPi);
     }
-    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &) const {
+    template <class Blk>
+    __device__ CoefT<S> operator()(const Blk &B, int i, S &) const {
         const double f0 = B.col[0][i];
         S gam = A * B.col[4][i] + Bs * B.col[5][i];
         S W = pref * (B.col[1][i] * B.col[3][i]) * exp(eT * B.col[2][i]) / (f0 * f0);
@@ -266,7 +287,8 @@ struct H2oPrep {   // h2o_bk lines, VVW with base (h2o_bk.py:83-99, 130-189)
         he = L.he;
     }
     // also adds the line's base contribution (-2 W base) to `base`
-    __device__ CoefT<S> operator()(const DevBlock &B, int i, S &base) const {
+    template <class Blk>
+    __device__ CoefT<S> operator()(const Blk &B, int i, S &base) const {
         const double fo = B.col[0][i];
         S Sl = B.col[1][i] * Th25 * exp(B.col[2][i] * (1.0 - Th));
         S df = B.col[3][i] * h2o * exp(B.col[4][i] * lnTh) + B.col[5][i] * h2 * exp(B.col[6][i] * lnTh) +

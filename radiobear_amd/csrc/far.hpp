@@ -21,13 +21,14 @@
 // lines in FP64 from their (D, a, c, g, G), as the direct FP64 sums do.
 //
 // alpha_far_kernel (block = layer x group of kFarTiles tiles x part of the layer's
-// staging rounds): each thread derives one line's coefficients per round (once per
-// line and layer), then thread groups of 16 classify the round's lines against one
-// tile each, add the far lines' series coefficients into per-tile moments (FP64,
-// in registers, reduced per segment) and write one bit per near line into the tile's
-// near mask.  The h2o base term is summed here.
-// It also stores every line's staged form per layer: FP32 mode (Ds, h; scaled a, c),
-// FP64 mode (D, a, c, g, G).
+// staging rounds; each wavefront a contiguous run of 64-line blocks): each lane derives
+// one line's coefficients per coefficient set (once per line and layer, its catalogue
+// columns fetched a block ahead), classifies it against every tile of the group (one
+// ballot per tile gives the tiles' near-mask words) and adds its series to running sums
+// keyed by the first tile it is below for and the last it is above for (prefix and
+// suffix sums over the tiles give each tile's moments).  The h2o base term is summed
+// here.  It also stores every line's staged form per layer: FP32 mode (Ds, h; scaled a,
+// c), FP64 mode (D, a, c, g, G).
 // alpha_near_kernel (block = tile x layer, all four wavefronts on the tile's
 // columns): lists the tile's near lines from its masks, gathers and sums them as the
 // line-sum kernel does with the staged lines split between the wavefronts, adds the
@@ -42,8 +43,6 @@ __host__ __device__ constexpr double far_ratio() { return F32 ? 1.0 / 3.0 : 0.15
 constexpr int kFarTiles = 8;                          // tiles per alpha_far_kernel block
 constexpr int kFarSets = 6;                           // h2o, h2s, inversion set 0, set 1, rot + v2, ph3
 constexpr int kFarMom = kFarSets * 2 * kFarTerms;     // moments per (layer, part, tile)
-constexpr int kRedChunk = 8;                          // accumulator terms reduced per LDS pass
-static_assert((2 * kFarTerms) % kRedChunk == 0, "series terms must fill whole reduction passes");
 constexpr int kSegStream[kSegs] = {0, 1, 2, 4, 5, 6};
 constexpr int kSegFarSet[kSegs] = {0, 1, 2, 4, 4, 5};
 constexpr int kSegVar[kSegs] = {0, 0, 1, 1, 1, 0};    // columns' x (0) or x for NH3 (1)
@@ -91,100 +90,172 @@ __device__ inline void tile_ranges(const AlphaArgs &A, int tile, double (&lo)[kF
     }
 }
 
-// moduli of the two roots of x^2 - 2 Ds x + (Ds^2 + h): complex pair (h >= 0) or real
-__device__ inline void pole_moduli(double Ds, double h, double &rs, double &rb) {
-    if (h >= 0.0) {
-        rs = rb = sqrt(fma(Ds, Ds, h));
-    } else {
-        const double q = sqrt(-h), m = fabs(Ds);
-        rb = m + q;
-        rs = fabs(m - q);
-    }
+// moduli (smaller rs, larger rb) of the two roots of x^2 - 2 Ds x + P0, P0 = Ds^2 + h:
+// a complex pair of modulus sqrt(P0) when h >= 0, else |Ds| -+ sqrt(-h); in FP32 from the
+// FP64 coefficients (the class tests compare FP32 values; NaN coefficients give NaN
+// moduli, near everywhere)
+__device__ inline void pole_moduli_f32(double Ds, double h, double P0, float &rs, float &rb) {
+    const float m = fabsf(static_cast<float>(Ds)), q = __builtin_amdgcn_sqrtf(fabsf(static_cast<float>(h)));
+    const float r = __builtin_amdgcn_sqrtf(static_cast<float>(P0));
+    const bool complex_pair = h >= 0.0;
+    rb = complex_pair ? r : m + q;
+    rs = complex_pair ? r : fabsf(m - q);
+    if (!(h == h)) rs = rb = NAN;
+}
+
+// a copy of a wavefront-uniform value held in scalar registers (each 32-bit word taken from
+// the first active lane)
+template <class T>
+__device__ __forceinline__ T uniform(const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    T out = v;
+    const int *src = reinterpret_cast<const int *>(&v);
+    int *dst = reinterpret_cast<int *>(&out);
+#pragma unroll
+    for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) dst[w] = __builtin_amdgcn_readfirstlane(src[w]);
+    return out;
+}
+
+// 1 / x to within an ulp or two: the hardware estimate refined by two Newton steps
+__device__ inline double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
 // the segment's coefficient deriver(s) for a layer and their FP32 weight scales (FP64 mode
-// uses none)
+// uses none), in scalar registers: f(segment as a compile-time constant, set 0 deriver,
+// set 1 deriver (the same one outside the nh3_hs inversion lines), scale 0, scale 1)
+template <int S>
+using SegC = std::integral_constant<int, S>;
 template <class F>
 __device__ __forceinline__ void with_prep(const AlphaArgs &A, const LayerT<double> &L, int seg, F &&f) {
     const DevBlock *blk = A.cat.blk;
     switch (seg) {
         case 0: {
-            const H2oPrep<double> p(L);
-            f(&p, pow2_scale(p.pref * blk[RBX_LINES_H2O].wscale), 1.0);
+            const H2oPrep<double> p = uniform(H2oPrep<double>(L));
+            f(SegC<0>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_H2O].wscale), 1.0);
             break;
         }
         case 1: {
-            const H2sPrep<double> p(L);
-            f(&p, pow2_scale(p.pref * blk[RBX_LINES_H2S].wscale), 1.0);
+            const H2sPrep<double> p = uniform(H2sPrep<double>(L));
+            f(SegC<1>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_H2S].wscale), 1.0);
             break;
         }
         case 2: {
-            const InvPrep<double> p[2] = {InvPrep<double>(L, nh3_low_set(A.abs.nh3)), InvPrep<double>(L, 1)};
+            const InvPrep<double> p0 = uniform(InvPrep<double>(L, nh3_low_set(A.abs.nh3)));
+            const InvPrep<double> p1 = uniform(InvPrep<double>(L, 1));
             const double w = blk[RBX_LINES_NH3_INV].wscale;
-            f(p, pow2_scale(p[0].pref * w), pow2_scale(p[1].pref * w));
+            f(SegC<2>{}, p0, p1, pow2_scale(p0.pref * w), pow2_scale(p1.pref * w));
             break;
         }
         case 3: {
-            const RotPrep<double> p(L);
-            f(&p, pow2_scale(p.pref * blk[RBX_LINES_NH3_ROT].wscale), 1.0);
+            const RotPrep<double> p = uniform(RotPrep<double>(L));
+            f(SegC<3>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_NH3_ROT].wscale), 1.0);
             break;
         }
         case 4: {
-            const V2Prep<double> p(L);
-            f(&p, pow2_scale(p.pref * blk[RBX_LINES_NH3_V2].wscale), 1.0);
+            const V2Prep<double> p = uniform(V2Prep<double>(L));
+            f(SegC<4>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_NH3_V2].wscale), 1.0);
             break;
         }
         default: {
-            const Ph3Prep<double> p(L);
-            f(&p, pow2_scale(p.pref * blk[RBX_LINES_PH3].wscale), 1.0);
+            const Ph3Prep<double> p = uniform(Ph3Prep<double>(L));
+            f(SegC<5>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_PH3].wscale), 1.0);
             break;
         }
     }
 }
 
-// ---- pass 1: classification, moments, near masks ----
+// ---- pass 1: far-line series and near masks ----
+//
+// A line's series coefficients (m_k below, n_k above) do not depend on the tile, only its
+// class does: the line is below for the tiles with q lo >= its larger pole modulus and
+// above for those with hi / q <= its smaller one.  Ranked by q lo, the tiles a line is
+// below for are the last ones (all ranks >= kb, kb = the number of tiles with q lo under
+// the modulus); ranked by hi / q, the tiles it is above for are the first ka.  So each
+// thread derives its own line's series once per coefficient set and adds them into a
+// difference array over the ranks (entry kb of the below array, entry ka of the above
+// one); the moments of a tile are the prefix (below) or suffix (above) sums at its rank.
+// The line lists are sorted by centre, so a thread's successive lines mostly share kb
+// and ka: it keeps running sums in registers for its current kb and ka and adds them to
+// the block's difference arrays in LDS (FP64 atomics) only when they change.  Near bits
+// come from one wavefront ballot per tile.
 
-constexpr int kFarPairs = kFarTiles + 1;   // (tile, coefficient set) pairs: one tile may need both sets
+constexpr int kFarWaves = kThreads / 64;
 
 struct FarShared {
-    union {
-        struct {   // one staging round, per coefficient set and line
-            double a[2][kThreads], c[2][kThreads], Ds[2][kThreads], P0[2][kThreads];
-            float2 r[2][kThreads];   // pole moduli (smaller, larger)
-        } st;
-        double red[kThreads][kRedChunk];   // accumulator reduction, kRedChunk terms at a time
-    };
+    // per wavefront: difference arrays, below (entry kb) and above (entry ka)
+    double dB[kFarWaves][kFarTiles + 1][kFarTerms];
+    double dA[kFarWaves][kFarTiles + 1][kFarTerms];
     double mom[kFarTiles][kFarMom];
     double lo[kFarTiles][kFarRanges], hi[kFarTiles][kFarRanges];
-    unsigned int maskw[kFarPairs][kThreads / 32];
     double bsum[kThreads / 64];
+    alignas(16) float thr[2][kFarTiles];   // current coefficient set: q lo, hi / q (+inf: unused)
 };
 
-// the (tile, set) pairs of a segment in a group of tiles, in order: every tile of the
-// group, with each inversion set its columns use (one set for the other segments)
-struct FarPairs {
-    int n;
-    int j[kFarPairs], set[kFarPairs];
-    __device__ FarPairs(const AlphaArgs &A, int g, int ntiles, bool inv) {
-        n = 0;
-        for (int jj = 0; jj < ntiles; ++jj) {
-            int c0, c1;
-            tile_cols(A, g * kFarTiles + jj, c0, c1);
-            const int ts = inv ? tile_sets(A, c0, c1) : 1;
-            for (int v = 0; v < 2; ++v)
-                if ((ts >> v) & 1) {
-                    j[n] = jj;
-                    set[n] = v;
-                    ++n;
-                }
-        }
-    }
-};
-
-// three blocks per CU (168 VGPRs, a few spills) run the pass in 1.00 ms against 1.18 ms with two
-// blocks and no spills; four (128 VGPRs) gain 3 % more for twice the spill traffic
+// a used tile's class thresholds for range rv: q lo (below) and hi / q (above), as FP32
 template <bool F32>
-__global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
+__device__ inline float far_threshold(const FarShared &sh, int j, int rv, bool above) {
+    const double q = far_ratio<F32>();
+    return above ? static_cast<float>(sh.hi[j][rv] / q) : static_cast<float>(q * sh.lo[j][rv]);
+}
+
+// rank of used tile j among the used tiles by its threshold (ties by index)
+template <bool F32>
+__device__ inline int tile_rank(const FarShared &sh, unsigned used, int j, int rv, bool above) {
+    const float vj = far_threshold<F32>(sh, j, rv, above);
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < kFarTiles; ++i) {
+        const float vi = far_threshold<F32>(sh, i, rv, above);
+        r += ((used >> i) & 1u) && (vi < vj || (vi == vj && i < j)) ? 1 : 0;
+    }
+    return r;
+}
+
+
+// The lanes with `pred` add their running sums acc into entry e (their own) of the
+// wavefront's difference array d: for each entry among them, the 16 terms are summed over
+// its lanes by a fixed reduce-scatter (halving the terms each lane carries at xor 32, 16, 8
+// and 4, then adding over xor 2 and 1: term 8 b5 + 4 b4 + 2 b3 + b2 ends in the lanes with
+// those lane bits), so the result does not depend on timing.  Called by the whole wavefront.
+__device__ __forceinline__ void far_flush(double (&d)[kFarTiles + 1][kFarTerms], bool pred, int e,
+                                          const double (&acc)[kFarTerms]) {
+    static_assert(kFarTerms == 16, "the reduce-scatter below is written for 16 terms");
+    unsigned long long rem = __ballot(pred);
+    const int lane = threadIdx.x % 64;
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+    while (rem != 0ull) {
+        const int b = __builtin_amdgcn_readlane(e, __builtin_ctzll(rem));
+        const bool sel = pred && e == b;
+        rem &= ~__ballot(sel);
+        double w8[8], w4[4], w2[2];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const double lo = sel ? acc[t] : 0.0, hi = sel ? acc[t + 8] : 0.0;
+            w8[t] = (b5 ? hi : lo) + __shfl_xor(b5 ? lo : hi, 32);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w4[t] = (b4 ? w8[t + 4] : w8[t]) + __shfl_xor(b4 ? w8[t] : w8[t + 4], 16);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) w2[t] = (b3 ? w4[t + 2] : w4[t]) + __shfl_xor(b3 ? w4[t] : w4[t + 2], 8);
+        double w1 = (b2 ? w2[1] : w2[0]) + __shfl_xor(b2 ? w2[0] : w2[1], 4);
+        w1 += __shfl_xor(w1, 2);
+        w1 += __shfl_xor(w1, 1);
+        if ((lane & 3) == 0) d[b][(b5 ? 8 : 0) + (b4 ? 4 : 0) + (b3 ? 2 : 0) + (b2 ? 1 : 0)] += w1;
+    }
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) {
+    // the arguments read where the launch put them (explicit kernel arguments open the
+    // kernarg segment): a private copy of the struct would be spilled to scratch per lane
+#if defined(__HIP_DEVICE_COMPILE__)
+    const AlphaArgs &A = *reinterpret_cast<const AlphaArgs *>(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+    const AlphaArgs &A = args;   // (host pass: never run)
+#endif
     __shared__ FarShared sh;
     const int ngroups = (A.tiles_total + kFarTiles - 1) / kFarTiles;
     const int layer = blockIdx.x / ngroups, g = blockIdx.x % ngroups;
@@ -204,149 +275,212 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs A) {
         }
     }
     for (int e = tid; e < kFarTiles * kFarMom; e += kThreads) (&sh.mom[0][0])[e] = 0.0;
+    for (int e = tid; e < kFarWaves * (kFarTiles + 1) * kFarTerms; e += kThreads) {
+        (&sh.dB[0][0][0])[e] = 0.0;
+        (&sh.dA[0][0][0])[e] = 0.0;
+    }
     int gc0, gc1;
     tile_cols(A, g * kFarTiles, gc0, gc1);
     gc1 = min(A.nf, (g + 1) * kFarTiles * A.tile_width);
     const int gsets = tile_sets(A, gc0, gc1);   // inversion sets any tile of the group needs
     const int ntiles = min(kFarTiles, A.tiles_total - g * kFarTiles);
-    const double q = far_ratio<F32>();
+    unsigned tile_set_bits[2] = {0u, 0u};       // tiles of the group whose columns use set 0 / 1
+    for (int jj = 0; jj < ntiles; ++jj) {
+        int c0, c1;
+        tile_cols(A, g * kFarTiles + jj, c0, c1);
+        const int ts = tile_sets(A, c0, c1);
+        tile_set_bits[0] |= (ts & 1) ? 1u << jj : 0u;
+        tile_set_bits[1] |= (ts & 2) ? 1u << jj : 0u;
+    }
     __syncthreads();
 
     const LinePlan plan(A, L.P);
     const int r0 = plan.begin(part, nsplit), r1 = plan.begin(part + 1, nsplit);
     double base = 0.0;
     int off = 0;
+#pragma unroll 1
     for (int seg = 0; seg < kSegs; ++seg) {
         const int a0 = max(r0, off), b0 = min(r1, off + plan.nr[seg]);
         const int seg_off = off;
         off += plan.nr[seg];
         if (a0 >= b0) continue;
-        const DevBlock &B = A.cat.blk[kSegKind[seg]];
-        const bool inv = seg == 2;
-        const int need = inv ? gsets : 1;
-        // thread groups of tpp consecutive threads, one per (tile, set) pair; each takes
-        // every tpp-th line of a round
-        const FarPairs pairs(A, g, ntiles, inv);
-        const int tpp = kThreads / pairs.n;
-        const int myp = tid / tpp, sidx = tid % tpp;
-        const bool active = myp < pairs.n;
-        const int myj = active ? pairs.j[myp] : 0, myset = active ? pairs.set[myp] : 0;
-        const int rv = kSegVar[seg] == 0 ? 0 : (inv ? 2 + myset : 1);
-        const float lo_q = static_cast<float>(q * sh.lo[myj][rv]), hi_q = static_cast<float>(sh.hi[myj][rv] / q);
-        double accB[kFarTerms], accA[kFarTerms];
-#pragma unroll
-        for (int k = 0; k < kFarTerms; ++k) accB[k] = accA[k] = 0.0;
-
-        with_prep(A, L, seg, [&](const auto *prep, double s0, double s1) {
-            const double scale[2] = {F32 ? s0 : 1.0, F32 ? s1 : 1.0};
+        with_prep(A, L, seg, [&](auto segc, const auto &p0, const auto &p1, double s0, double s1) {
+            constexpr int SEG = decltype(segc)::value;
+            constexpr bool inv = SEG == 2;
+            const DevBlock &B = A.cat.blk[kSegKind[SEG]];
+            const int need = inv ? gsets : 1;
+            const double scale[2] = {uniform(F32 ? s0 : 1.0), uniform(F32 ? s1 : 1.0)};
             if (g == 0 && tid < 2)   // (every part that has the segment stores the same value)
-                A.far_scale[(static_cast<size_t>(layer) * kSegs + seg) * 2 + tid] = scale[tid];
-            for (int r = a0; r < b0; ++r) {
-                const int l0 = (r - seg_off) * kThreads;
-                const int cnt = min(kThreads, B.n - l0);
-                {   // phase 1: thread tid derives line l0 + tid and stores its staged form
-                    CoefT<double> qs[2] = {{0.0, 0.0, 0.0, 0.0, 1.0}, {0.0, 0.0, 0.0, 0.0, 1.0}};
-                    if (tid < cnt) line_coefs(prep, need, B, l0 + tid, base, qs);
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        if (!(need & (1 << s))) continue;
-                        double Ds, h, rs, rb;
-                        shifted_denominator(qs[s], Ds, h);
-                        pole_moduli(Ds, h, rs, rb);
-                        // one group stores each stream: the first whose tiles use the set
-                        const int writer = (inv && s == 1) ? A.nf_low / A.tile_width / kFarTiles : 0;
-                        if (g == writer && tid < cnt) {
-                            const size_t o = static_cast<size_t>(layer) * A.far_words * 32 +
-                                             static_cast<size_t>(A.far_off[kSegStream[seg] + (inv ? s : 0)]) * 32 +
-                                             l0 + tid;
-                            if constexpr (F32) {
-                                A.near_dh[o] = make_double2(Ds, h);
-                                A.near_ac[o] = make_float2(static_cast<float>(qs[s].a * scale[s]),
-                                                           static_cast<float>(qs[s].c * scale[s]));
-                            } else {
-                                const size_t pl = static_cast<size_t>(A.nl) * A.far_words * 32;
-                                A.near_c64[o] = qs[s].D;
-                                A.near_c64[pl + o] = qs[s].a;
-                                A.near_c64[2 * pl + o] = qs[s].c;
-                                A.near_c64[3 * pl + o] = qs[s].g;
-                                A.near_c64[4 * pl + o] = qs[s].G;
-                            }
-                        }
-                        sh.st.a[s][tid] = qs[s].a;
-                        sh.st.c[s][tid] = qs[s].c;
-                        sh.st.Ds[s][tid] = Ds;
-                        sh.st.P0[s][tid] = fma(Ds, Ds, h);
-                        sh.st.r[s][tid] = make_float2(static_cast<float>(rs), static_cast<float>(rb));
-                    }
-                    if (tid < kFarPairs * (kThreads / 32)) (&sh.maskw[0][0])[tid] = 0u;
+                A.far_scale[(static_cast<size_t>(layer) * kSegs + SEG) * 2 + tid] = tid ? scale[1] : scale[0];
+#pragma unroll 1
+            for (int v = 0; v < 2; ++v) {
+                if (!((need >> v) & 1)) continue;
+                const unsigned used = inv ? tile_set_bits[v] : (1u << ntiles) - 1u;
+                const int nv = __popc(used);
+                const int rv = kSegVar[SEG] == 0 ? 0 : (inv ? 2 + v : 1);
+                // thresholds of the used tiles; unused ones never count (+inf: a finite
+                // modulus is below it and never at or above it)
+                if (tid < 2 * kFarTiles) {
+                    const int j = tid % kFarTiles;
+                    sh.thr[tid / kFarTiles][j] = (used >> j) & 1u ? far_threshold<F32>(sh, j, rv, tid >= kFarTiles) : INFINITY;
                 }
                 __syncthreads();
-                // phase 2: classify against the pair's tile; far lines add their series
-                // coefficients, near lines set their bit
-                if (active) {
-                    for (int i = sidx; i < cnt; i += tpp) {
-                        const float2 rr = sh.st.r[myset][i];
-                        if (rr.y <= lo_q) {
-                            const double a = sh.st.a[myset][i], c = sh.st.c[myset][i];
-                            const double Ds2 = 2.0 * sh.st.Ds[myset][i], P0 = sh.st.P0[myset][i];
-                            double m0 = a, m1 = fma(Ds2, a, c);
-                            accB[0] += m0;
-                            accB[1] += m1;
+                const int writer = (inv && v == 1) ? A.nf_low / A.tile_width / kFarTiles : 0;
+                const int soff = (inv && v) ? A.far_off[kSegStream[SEG] + 1] : A.far_off[kSegStream[SEG]];
+                const size_t so = static_cast<size_t>(layer) * A.far_words * 32 + static_cast<size_t>(soff) * 32;
+                unsigned int *mrow = A.near + (static_cast<size_t>(layer) * A.tiles_total + g * kFarTiles) * A.far_words + soff;
+                double accB[kFarTerms], accA[kFarTerms];
 #pragma unroll
-                            for (int k = 2; k < kFarTerms; ++k) {
-                                const double m2 = fma(Ds2, m1, -P0 * m0);
-                                accB[k] += m2;
-                                m0 = m1;
-                                m1 = m2;
-                            }
-                        } else if (rr.x >= hi_q) {
-                            const double a = sh.st.a[myset][i], c = sh.st.c[myset][i];
-                            const double Ds2 = 2.0 * sh.st.Ds[myset][i], iP = 1.0 / sh.st.P0[myset][i];
-                            double n0 = c * iP, n1 = fma(Ds2, n0, a) * iP;
-                            accA[0] += n0;
-                            accA[1] += n1;
+                for (int k = 0; k < kFarTerms; ++k) accB[k] = accA[k] = 0.0;
+                int cbB = -1, cbA = -1;   // entries the running sums belong to (-1: none yet)
+                // wavefront w takes a contiguous run of the part's 64-line blocks (its lanes'
+                // successive lines are 64 apart, so their entries change seldom)
+                const int nblk = b0 - a0;   // blocks per wavefront (kThreads / 64 per round)
+                const int blk0 = (a0 - seg_off) * kFarWaves + wave * nblk;
+                // each line's catalogue columns are fetched one block ahead
+                LineRow<kNumCols[kSegKind[SEG]]> next;
+                next.load(B, min(blk0 * 64 + lane, B.n - 1));
+#pragma unroll 1
+                for (int blk = blk0; blk < blk0 + nblk; ++blk) {
+                    const int i = blk * 64 + lane;
+                    const bool valid = i < B.n;
+                    const auto row = next;
+                    if (blk + 1 < blk0 + nblk) next.load(B, min(i + 64, B.n - 1));
+                    CoefT<double> c = {0.0, 0.0, 0.0, 0.0, 1.0};
+                    if (valid) {
+                        using Prep = std::decay_t<decltype(p0)>;
+                        if constexpr (Prep::kShared) {
+                            const double w = p0.weight(row, i);
+                            c = v ? p1.coef(row, i, w) : p0.coef(row, i, w);
+                        } else {
+                            c = p0(row, i, base);
+                        }
+                    }
+                    const double sc = v ? scale[1] : scale[0];
+                    double Ds, h;
+                    shifted_denominator(c, Ds, h);
+                    const double P0 = fma(Ds, Ds, h);
+                    if (g == writer && valid) {
+                        if constexpr (F32) {
+                            A.near_dh[so + i] = make_double2(Ds, h);
+                            A.near_ac[so + i] = make_float2(static_cast<float>(c.a * sc),
+                                                            static_cast<float>(c.c * sc));
+                        } else {
+                            const size_t pl = static_cast<size_t>(A.nl) * A.far_words * 32;
+                            A.near_c64[so + i] = c.D;
+                            A.near_c64[pl + so + i] = c.a;
+                            A.near_c64[2 * pl + so + i] = c.c;
+                            A.near_c64[3 * pl + so + i] = c.g;
+                            A.near_c64[4 * pl + so + i] = c.G;
+                        }
+                    }
+                    // class against every used tile: below when rb <= q lo, else above when
+                    // rs >= hi / q, else near (NaN moduli: near everywhere)
+                    float frs, frb;
+                    pole_moduli_f32(Ds, h, P0, frs, frb);
+                    float loq[kFarTiles], hiq[kFarTiles];
 #pragma unroll
-                            for (int k = 2; k < kFarTerms; ++k) {
-                                const double n2 = fma(Ds2, n1, -n0) * iP;
-                                accA[k] += n2;
-                                n0 = n1;
-                                n1 = n2;
-                            }
-                        } else {   // near (also NaN coefficients: summed directly)
-                            atomicOr(&sh.maskw[myp][i >> 5], 1u << (i & 31));
+                    for (int j = 0; j < kFarTiles; j += 4) {
+                        const float4 l4 = *reinterpret_cast<const float4 *>(&sh.thr[0][j]);
+                        const float4 h4 = *reinterpret_cast<const float4 *>(&sh.thr[1][j]);
+                        loq[j] = l4.x; loq[j + 1] = l4.y; loq[j + 2] = l4.z; loq[j + 3] = l4.w;
+                        hiq[j] = h4.x; hiq[j + 1] = h4.y; hiq[j + 2] = h4.z; hiq[j + 3] = h4.w;
+                    }
+                    // (unused tiles: both thresholds +inf, never counted for finite moduli; their
+                    // near bits are not stored)
+                    int kb = 0, ka = 0;
+                    unsigned long long near[kFarTiles];
+#pragma unroll
+                    for (int j = 0; j < kFarTiles; ++j) {
+                        const bool below = frb <= loq[j], above = frs >= hiq[j];
+                        kb += below ? 0 : 1;
+                        ka += above ? 1 : 0;
+                        near[j] = __builtin_amdgcn_ballot_w64(valid && !below && !above);
+                    }
+                    if (lane == 0) {   // the block's two mask words of every used tile
+#pragma unroll
+                        for (int j = 0; j < kFarTiles; ++j)
+                            if ((used >> j) & 1u)
+                                *reinterpret_cast<uint2 *>(mrow + static_cast<size_t>(j) * A.far_words + 2 * blk) =
+                                    make_uint2(static_cast<unsigned int>(near[j]), static_cast<unsigned int>(near[j] >> 32));
+                    }
+                    // below: entry kb (< nv), series m_k; above: entry ka (> 0), series n_k.
+                    // A thread whose entry changes first adds its running sums to the block's
+                    // array; lanes with no share carry zero coefficients.
+                    const bool hasB = valid && kb < nv, hasA = valid && ka > 0 && ka <= nv;
+                    const bool swB = hasB && kb != cbB, swA = hasA && ka != cbA;
+                    if (__builtin_amdgcn_ballot_w64(swB)) {
+                        far_flush(sh.dB[wave], swB && cbB >= 0, cbB, accB);
+#pragma unroll
+                        for (int k = 0; k < kFarTerms; ++k) accB[k] = swB ? 0.0 : accB[k];
+                        cbB = swB ? kb : cbB;
+                    }
+                    if (__builtin_amdgcn_ballot_w64(swA)) {
+                        far_flush(sh.dA[wave], swA && cbA >= 0, cbA, accA);
+#pragma unroll
+                        for (int k = 0; k < kFarTerms; ++k) accA[k] = swA ? 0.0 : accA[k];
+                        cbA = swA ? ka : cbA;
+                    }
+                    {
+                        double a = hasB ? c.a : 0.0, cc = hasB ? c.c : 0.0;
+                        double Ds2 = hasB ? 2.0 * Ds : 0.0, p0 = hasB ? P0 : 0.0;
+                        // (the series start after the flushes: their terms are not held
+                        // live across them)
+                        asm volatile("" : "+v"(a), "+v"(cc), "+v"(Ds2), "+v"(p0));
+                        double m0 = a, m1 = fma(Ds2, a, cc);
+                        accB[0] += m0;
+                        accB[1] += m1;
+#pragma unroll
+                        for (int k = 2; k < kFarTerms; ++k) {
+                            const double m2 = fma(Ds2, m1, -p0 * m0);
+                            accB[k] += m2;
+                            m0 = m1;
+                            m1 = m2;
+                        }
+                    }
+                    {
+                        double a = hasA ? c.a : 0.0, cc = hasA ? c.c : 0.0;
+                        double Ds2 = hasA ? 2.0 * Ds : 0.0, iP = hasA ? recip(P0) : 0.0;
+                        asm volatile("" : "+v"(a), "+v"(cc), "+v"(Ds2), "+v"(iP));
+                        double n0 = cc * iP, n1 = fma(Ds2, n0, a) * iP;
+                        accA[0] += n0;
+                        accA[1] += n1;
+#pragma unroll
+                        for (int k = 2; k < kFarTerms; ++k) {
+                            const double n2 = fma(Ds2, n1, -n0) * iP;
+                            accA[k] += n2;
+                            n0 = n1;
+                            n1 = n2;
                         }
                     }
                 }
+                far_flush(sh.dB[wave], cbB >= 0, cbB, accB);
+                far_flush(sh.dA[wave], cbA >= 0, cbA, accA);
                 __syncthreads();
-                if (tid < pairs.n * (kThreads / 32)) {   // the round's 8 mask words of each pair
-                    const int p = tid / (kThreads / 32), w = tid % (kThreads / 32);
-                    const int stream = kSegStream[seg] + (inv ? pairs.set[p] : 0);
-                    A.near[(static_cast<size_t>(layer) * A.tiles_total + g * kFarTiles + pairs.j[p]) * A.far_words +
-                           A.far_off[stream] + l0 / 32 + w] = sh.maskw[p][w];
+                // moments of each used tile: prefix sums of the below array up to its rank
+                // by q lo, suffix sums of the above array past its rank by hi / q
+                const int ss = kSegFarSet[SEG] + (inv ? v : 0);
+                for (int e = tid; e < kFarTiles * kFarTerms; e += kThreads) {
+                    const int j = e / kFarTerms, k = e % kFarTerms;
+                    if (!((used >> j) & 1u)) continue;
+                    const int pb = tile_rank<F32>(sh, used, j, rv, false), pa = tile_rank<F32>(sh, used, j, rv, true);
+                    double sb = 0.0, sa = 0.0;
+                    for (int w = 0; w < kFarWaves; ++w) {
+                        for (int p = 0; p <= pb; ++p) sb += sh.dB[w][p][k];
+                        for (int p = pa + 1; p <= nv; ++p) sa += sh.dA[w][p][k];
+                    }
+                    sh.mom[j][ss * 2 * kFarTerms + k] += sb;
+                    sh.mom[j][ss * 2 * kFarTerms + kFarTerms + k] += sa;
                 }
+                __syncthreads();
+                for (int e = tid; e < kFarWaves * (kFarTiles + 1) * kFarTerms; e += kThreads) {
+                    (&sh.dB[0][0][0])[e] = 0.0;
+                    (&sh.dA[0][0][0])[e] = 0.0;
+                }
+                __syncthreads();
             }
         });
-        // the segment's moments: each pair's threads' accumulators summed through LDS
-        // (the staging area), kRedChunk terms at a time, into the pair's tile and set
-#pragma unroll
-        for (int part = 0; part < 2 * kFarTerms / kRedChunk; ++part) {
-#pragma unroll
-            for (int e = 0; e < kRedChunk; ++e) {
-                const int t = part * kRedChunk + e;
-                sh.red[tid][e] = t < kFarTerms ? accB[t] : accA[t - kFarTerms];
-            }
-            __syncthreads();
-            if (tid < pairs.n * kRedChunk) {
-                const int p = tid / kRedChunk, e = tid % kRedChunk;
-                double sum = 0.0;
-                for (int k = 0; k < tpp; ++k) sum += sh.red[p * tpp + k][e];
-                const int ss = kSegFarSet[seg] + (inv ? pairs.set[p] : 0);
-                sh.mom[pairs.j[p]][ss * 2 * kFarTerms + part * kRedChunk + e] += sum;
-            }
-            __syncthreads();
-        }
     }
-    __syncthreads();
     for (int e = tid; e < kFarTiles * kFarMom; e += kThreads) {
         const int jj = e / kFarMom, r = e % kFarMom;
         const int t = g * kFarTiles + jj;
