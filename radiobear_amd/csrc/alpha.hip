@@ -119,14 +119,10 @@ struct AlphaArgs {
     double *ws;        // forward: partial line sums [kSlots][nsplit][nl][nf]
     double *ws_base;   // forward: partial h2o base terms [nsplit][nl]
     double *ws_extra;  // forward: optional absorbers [kNumExtra][nl][nf] (when any is selected)
-    // FP32 far-line series (far.hpp): near-line masks [nl][tiles_total][far_words], moments
+    // far-line series (far.hpp): near-line masks [nl][tiles_total][far_words], moments
     // [nl][far_split][tiles_total][kFarMom], h2o base terms [far_split][nl]
     unsigned int *near;
-    double2 *near_dh;   // FP32 staged lines per layer and stream line [nl][far_words * 32]: Ds, h
-    float2 *near_ac;    // and the scaled a, c
-    double *near_c64;   // FP64: D, a, c, g, G planes [5][nl * far_words * 32]
     double *far_mom, *far_base;
-    double *far_scale;  // FP32 line-weight scale per layer, segment and coefficient set [nl][kSegs][2]
     int far_words, far_split;
     int far_off[kFarStreams];   // first mask word of each stream
     // tangent pass split into tan_parts parts of a layer's tan_rounds 64-line rounds;
@@ -1546,11 +1542,6 @@ static bool series_on(const AlphaArgs &A, const rbx_launch &L) {
     return t == 2 || (t != 1 && A.tiles_total >= kSeriesMinTiles);
 }
 
-// bytes of one line's staged form per layer: FP32 (Ds, h) + scaled (a, c); FP64 (D, a, c, g, G)
-static size_t staged_line_bytes(int precision) {
-    return precision == RBX_FP32 ? sizeof(double2) + sizeof(float2) : 5 * sizeof(double);
-}
-
 // near-mask streams: one bit per line of every staging round of each selected segment
 static void far_layout(AlphaArgs &A) {
     const DevBlock *blk = A.cat.blk;
@@ -1589,10 +1580,9 @@ static size_t alpha_ws_bytes(const AlphaArgs &A, int precision, int nsplit, bool
     const size_t extra = has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0;
     size_t b = (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl + extra) *
                sizeof(double);
-    if (series)   // (+16: the staged lines start 16-byte aligned)
-        b += 16 + static_cast<size_t>(A.nl) * A.far_words * 32 * staged_line_bytes(precision) +
-             (static_cast<size_t>(A.nl) * A.far_split * A.tiles_total * kFarMom +
-              static_cast<size_t>(A.far_split) * A.nl + static_cast<size_t>(A.nl) * kSegs * 2) * sizeof(double) +
+    if (series)
+        b += (static_cast<size_t>(A.nl) * A.far_split * A.tiles_total * kFarMom + static_cast<size_t>(A.far_split) * A.nl) *
+                 sizeof(double) +
              static_cast<size_t>(A.nl) * A.tiles_total * A.far_words * sizeof(unsigned int);
     return b;
 }
@@ -1660,7 +1650,7 @@ static int plan_alpha(AlphaArgs &A, const rbx_launch &L, int precision, int &K, 
     if (series) {
         A.far_split = far_split(A, L);
         nsplit = 1;
-        // the staged lines take 24 B (FP32) or 40 B (FP64) per line and layer: past
+        // the moments and near masks grow with layers x tiles x lines: past
         // kSeriesMaxWorkspace the built-in choice is the direct sums (launch.alpha_series = 2
         // forces the series)
         if (L.alpha_series != 2 && alpha_ws_bytes(A, precision, 1, true) > kSeriesMaxWorkspace)
@@ -1682,23 +1672,9 @@ static int run_planned(AlphaArgs &A, int precision, int K, int nsplit, bool seri
     A.ws_extra = A.ws_base + static_cast<size_t>(nsplit) * nl;
     if (series) {
         const size_t plane = static_cast<size_t>(nl) * nf;
-        const size_t lines = static_cast<size_t>(nl) * A.far_words * 32;
-        const uintptr_t at = reinterpret_cast<uintptr_t>(
-            A.ws_extra + (has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0));
-        const uintptr_t lines_at = (at + 15) / 16 * 16;
-        double *after;
-        if (precision == RBX_FP32) {
-            A.near_dh = reinterpret_cast<double2 *>(lines_at);
-            A.near_ac = reinterpret_cast<float2 *>(A.near_dh + lines);
-            after = reinterpret_cast<double *>(A.near_ac + lines);
-        } else {
-            A.near_c64 = reinterpret_cast<double *>(lines_at);
-            after = A.near_c64 + 5 * lines;
-        }
-        A.far_mom = after;
+        A.far_mom = A.ws_extra + (has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0);
         A.far_base = A.far_mom + static_cast<size_t>(nl) * A.far_split * A.tiles_total * kFarMom;
-        A.far_scale = A.far_base + static_cast<size_t>(A.far_split) * nl;
-        A.near = reinterpret_cast<unsigned int *>(A.far_scale + static_cast<size_t>(nl) * kSegs * 2);
+        A.near = reinterpret_cast<unsigned int *>(A.far_base + static_cast<size_t>(A.far_split) * nl);
         switch (K) {
             case 4: return launch_series<4>(A, precision, st);
             case 2: return launch_series<2>(A, precision, st);

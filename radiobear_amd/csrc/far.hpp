@@ -27,13 +27,13 @@
 // ballot per tile gives the tiles' near-mask words) and adds its series to running sums
 // keyed by the first tile it is below for and the last it is above for (prefix and
 // suffix sums over the tiles give each tile's moments).  The h2o base term is summed
-// here.  It also stores every line's staged form per layer: FP32 mode (Ds, h; scaled a,
-// c), FP64 mode (D, a, c, g, G).
+// here.
 // alpha_near_kernel (block = tile x layer, all four wavefronts on the tile's
-// columns): lists the tile's near lines from its masks, gathers and sums them as the
-// line-sum kernel does with the staged lines split between the wavefronts, adds the
-// wavefronts in order and the series of the summed moments, and writes the same
-// per-slot partials the finish kernel reads (one part).
+// columns): lists the tile's near lines from its masks, derives each listed line's
+// coefficients again from its catalogue row (the catalogue stays in L2; no per-layer copy
+// of the lines goes through HBM), sums them as the line-sum kernel does with the lines
+// split between the wavefronts, adds the wavefronts in order and the series of the summed
+// moments, and writes the same per-slot partials the finish kernel reads (one part).
 
 constexpr int kFarTerms = 16;
 // pole ratio of a far line: 1/3 in FP32 mode, 0.15 in FP64 mode (at most ~8e-12 of a far
@@ -164,6 +164,19 @@ __device__ __forceinline__ void with_prep(const AlphaArgs &A, const LayerT<doubl
             f(SegC<5>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_PH3].wscale), 1.0);
             break;
         }
+    }
+}
+
+// coefficient set v of line i of a segment from its catalogue row (both passes derive a
+// line's coefficients this way; `base` collects the h2o base term)
+template <class Prep, class Row>
+__device__ __forceinline__ CoefT<double> seg_coef(const Prep &p0, const Prep &p1, int v, const Row &row, int i,
+                                                  double &base) {
+    if constexpr (Prep::kShared) {
+        const double w = p0.weight(row, i);
+        return v ? p1.coef(row, i, w) : p0.coef(row, i, w);
+    } else {
+        return p0(row, i, base);
     }
 }
 
@@ -309,9 +322,8 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) 
             constexpr bool inv = SEG == 2;
             const DevBlock &B = A.cat.blk[kSegKind[SEG]];
             const int need = inv ? gsets : 1;
-            const double scale[2] = {uniform(F32 ? s0 : 1.0), uniform(F32 ? s1 : 1.0)};
-            if (g == 0 && tid < 2)   // (every part that has the segment stores the same value)
-                A.far_scale[(static_cast<size_t>(layer) * kSegs + SEG) * 2 + tid] = tid ? scale[1] : scale[0];
+            (void)s0;
+            (void)s1;
 #pragma unroll 1
             for (int v = 0; v < 2; ++v) {
                 if (!((need >> v) & 1)) continue;
@@ -325,9 +337,7 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) 
                     sh.thr[tid / kFarTiles][j] = (used >> j) & 1u ? far_threshold<F32>(sh, j, rv, tid >= kFarTiles) : INFINITY;
                 }
                 __syncthreads();
-                const int writer = (inv && v == 1) ? A.nf_low / A.tile_width / kFarTiles : 0;
                 const int soff = (inv && v) ? A.far_off[kSegStream[SEG] + 1] : A.far_off[kSegStream[SEG]];
-                const size_t so = static_cast<size_t>(layer) * A.far_words * 32 + static_cast<size_t>(soff) * 32;
                 unsigned int *mrow = A.near + (static_cast<size_t>(layer) * A.tiles_total + g * kFarTiles) * A.far_words + soff;
                 double accB[kFarTerms], accA[kFarTerms];
 #pragma unroll
@@ -347,33 +357,10 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) 
                     const auto row = next;
                     if (blk + 1 < blk0 + nblk) next.load(B, min(i + 64, B.n - 1));
                     CoefT<double> c = {0.0, 0.0, 0.0, 0.0, 1.0};
-                    if (valid) {
-                        using Prep = std::decay_t<decltype(p0)>;
-                        if constexpr (Prep::kShared) {
-                            const double w = p0.weight(row, i);
-                            c = v ? p1.coef(row, i, w) : p0.coef(row, i, w);
-                        } else {
-                            c = p0(row, i, base);
-                        }
-                    }
-                    const double sc = v ? scale[1] : scale[0];
+                    if (valid) c = seg_coef(p0, p1, v, row, i, base);
                     double Ds, h;
                     shifted_denominator(c, Ds, h);
                     const double P0 = fma(Ds, Ds, h);
-                    if (g == writer && valid) {
-                        if constexpr (F32) {
-                            A.near_dh[so + i] = make_double2(Ds, h);
-                            A.near_ac[so + i] = make_float2(static_cast<float>(c.a * sc),
-                                                            static_cast<float>(c.c * sc));
-                        } else {
-                            const size_t pl = static_cast<size_t>(A.nl) * A.far_words * 32;
-                            A.near_c64[so + i] = c.D;
-                            A.near_c64[pl + so + i] = c.a;
-                            A.near_c64[2 * pl + so + i] = c.c;
-                            A.near_c64[3 * pl + so + i] = c.g;
-                            A.near_c64[4 * pl + so + i] = c.G;
-                        }
-                    }
                     // class against every used tile: below when rb <= q lo, else above when
                     // rs >= hi / q, else near (NaN moduli: near everywhere)
                     float frs, frb;
@@ -522,10 +509,11 @@ struct NearShared {
         unsigned short list[kListWords * 32];   // a window's near lines
         double red[kThreads / 64][K][64];       // the wavefronts' partial sums at a slot's end
     };
-    double scale[kSegs][2];
     double far[kFarMom];
     WaveRanges rng;
     int wsum[kThreads / 64];
+    alignas(16) double prep[2][8];   // the segment's coefficient sets (read from LDS while deriving)
+    double scale[2];                 // and their FP32 weight scales
 };
 
 // sum of the far-line series of one moment set at x
@@ -545,18 +533,54 @@ __device__ inline double far_eval(const double *mom, double x) {
     return below + s;
 }
 
+// A listed line's coefficients for the near sums, derived from its catalogue row as pass 1
+// derives them, with the segment's coefficient sets read from LDS: FP32 (Ds, h, a s, c s),
+// FP64 (D, a, c, g, G)
+template <class Prep, int NC>
+__device__ __forceinline__ CoefT<double> near_coef(const DevBlock &B, int i, int v, const double (&prep)[2][8]) {
+    LineRow<NC> row;
+    row.load(B, i);
+    double unused = 0.0;
+    return seg_coef(*reinterpret_cast<const Prep *>(prep[0]), *reinterpret_cast<const Prep *>(prep[1]), v, row, i,
+                    unused);
+}
+
+template <bool F32>
+__device__ __forceinline__ void derive_near(int seg, int v, const DevBlock &B, int i, const double (&prep)[2][8],
+                                            double sv, double (&c5)[5], float2 &ac) {
+    CoefT<double> c;
+    switch (seg) {
+        case 0: c = near_coef<H2oPrep<double>, kNumCols[RBX_LINES_H2O]>(B, i, v, prep); break;
+        case 1: c = near_coef<H2sPrep<double>, kNumCols[RBX_LINES_H2S]>(B, i, v, prep); break;
+        case 2: c = near_coef<InvPrep<double>, kNumCols[RBX_LINES_NH3_INV]>(B, i, v, prep); break;
+        case 3: c = near_coef<RotPrep<double>, kNumCols[RBX_LINES_NH3_ROT]>(B, i, v, prep); break;
+        case 4: c = near_coef<V2Prep<double>, kNumCols[RBX_LINES_NH3_V2]>(B, i, v, prep); break;
+        default: c = near_coef<Ph3Prep<double>, kNumCols[RBX_LINES_PH3]>(B, i, v, prep); break;
+    }
+    if constexpr (F32) {
+        shifted_denominator(c, c5[0], c5[1]);
+        ac = make_float2(static_cast<float>(c.a * sv), static_cast<float>(c.c * sv));
+    } else {
+        c5[0] = c.D;
+        c5[1] = c.a;
+        c5[2] = c.c;
+        c5[3] = c.g;
+        c5[4] = c.G;
+    }
+}
+
 // near lines of one segment and coefficient set v, listed from the tile's mask words; each
 // wavefront sums its own batches of kWaveBatch listed lines (batches w, w + 4, ...), staged
 // in its own LDS and fetched one batch ahead into registers.  `so` = the stream's first line
 // in the staged-line arrays.
-template <int K, bool F32>
-__device__ void near_sum(const AlphaArgs &A, int n, size_t so, int v, bool mixed, bool uni, const unsigned int *mask,
+template <int K, bool F32, class Derive>
+__device__ void near_sum(const AlphaArgs &A, int n, const Derive &derive, int v, bool mixed, bool uni,
+                         const unsigned int *mask,
                          int nwords, int var, const int (&set)[K], const double (&x)[K], const float (&xf)[K],
                          double (&acc)[K], NearShared<K, F32> &sh) {
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     auto &st = sh.st[wave];
     const double lo = sh.rng.lo[var][0], hi = sh.rng.hi[var][0];
-    const size_t pl = static_cast<size_t>(A.nl) * A.far_words * 32;   // FP64 coefficient plane
     for (int w0 = 0; w0 < nwords; w0 += kListWords) {
         const int wi = w0 + tid;
         const unsigned int bits = (tid < kListWords && wi < nwords) ? mask[wi] : 0u;
@@ -578,33 +602,16 @@ __device__ void near_sum(const AlphaArgs &A, int n, size_t so, int v, bool mixed
         for (unsigned int b = bits; b != 0u; b &= b - 1u)
             sh.list[pos++] = static_cast<unsigned short>(tid * 32 + __builtin_ctz(b));
         __syncthreads();
-        // the staged form pass 1 stored for listed line e (past the list: a zero term):
-        // FP32 (Ds, h, a s, c s), FP64 (D, a, c, g, G)
-        auto fetch = [&](int e, double (&c5)[5], float2 &ac) {
-            const int i = e < total ? w0 * 32 + sh.list[e] : n;
-            c5[0] = 0.0; c5[1] = 0.0; c5[2] = 0.0; c5[3] = 0.0; c5[4] = 1.0;
-            ac = make_float2(0.0f, 0.0f);
-            if (i < n) {
-                if constexpr (F32) {
-                    const double2 dh = A.near_dh[so + i];
-                    c5[0] = dh.x;
-                    c5[1] = dh.y;
-                    ac = A.near_ac[so + i];
-                } else {
-#pragma unroll
-                    for (int m = 0; m < 5; ++m) c5[m] = A.near_c64[m * pl + so + i];
-                }
-            }
-            if constexpr (F32) {
-                if (!(i < n)) c5[1] = 1.0;   // padding h
-            }
-        };
+        // listed line e's coefficients, derived from its catalogue row when its batch is
+        // staged (past the list: a zero term): FP32 (Ds, h, a s, c s), FP64 (D, a, c, g, G)
         const int nb = (total + kWaveBatch - 1) / kWaveBatch;
-        double c5[5];
-        float2 ac;
-        if (wave < nb) fetch(wave * kWaveBatch + lane, c5, ac);
         for (int b = wave; b < nb; b += kThreads / 64) {
             const int cntp = (min(kWaveBatch, total - b * kWaveBatch) + kSub - 1) / kSub * kSub;
+            const int e = b * kWaveBatch + lane;
+            const int i = e < total ? w0 * 32 + sh.list[e] : n;
+            double c5[5] = {0.0, F32 ? 1.0 : 0.0, 0.0, 0.0, 1.0};
+            float2 ac = make_float2(0.0f, 0.0f);
+            if (i < n) derive(i, c5, ac);
             if constexpr (F32) {
                 st.D[v][lane] = c5[0];
                 st.h[v][lane] = c5[1];
@@ -633,10 +640,6 @@ __device__ void near_sum(const AlphaArgs &A, int n, size_t so, int v, bool mixed
                 }
             }
             wave_sync();
-            const int bn = b + kThreads / 64;
-            double c5n[5] = {0.0, 1.0, 0.0, 0.0, 1.0};
-            float2 ac2 = make_float2(0.0f, 0.0f);
-            if (bn < nb) fetch(bn * kWaveBatch + lane, c5n, ac2);
             if constexpr (F32) {
                 if (uni)
                     sum_staged_f32<K, true>(st, set, 0, cntp, x, xf, acc);
@@ -649,16 +652,18 @@ __device__ void near_sum(const AlphaArgs &A, int n, size_t so, int v, bool mixed
                     sum_staged_f64<K, false>(st, set, 0, cntp, x, acc);
             }
             wave_sync();
-#pragma unroll
-            for (int m = 0; m < 5; ++m) c5[m] = c5n[m];
-            ac = ac2;
         }
         __syncthreads();   // every wavefront is done with this window's list
     }
 }
 
 template <int K, bool F32>
-__global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(AlphaArgs A) {
+__global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(AlphaArgs args) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const AlphaArgs &A = *reinterpret_cast<const AlphaArgs *>(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+    const AlphaArgs &A = args;   // (host pass: never run)
+#endif
     __shared__ NearShared<K, F32> sh;
     // layer fastest: blocks are dealt to the XCDs in turn, so each XCD gets every tile (the
     // tiles' near fractions differ several-fold) rather than one tile of every layer.  Last
@@ -691,7 +696,6 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(Al
         xf[k] = static_cast<float>(x[k]);
         xnf[k] = static_cast<float>(xn[k]);
     }
-    const int one[K] = {};
     const bool uni_set = !mixed;   // every column of the tile reads one inversion set
 
     if (wave == 0) {
@@ -705,7 +709,6 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(Al
             }
         }
     }
-    if (tid < kSegs * 2) sh.scale[tid / 2][tid % 2] = A.far_scale[static_cast<size_t>(layer) * kSegs * 2 + tid];
     for (int e = tid; e < kFarMom; e += kThreads) {
         double s = 0.0;
         for (int p = 0; p < nsplit; ++p)
@@ -757,26 +760,45 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(Al
     };
     for (int seg = 0; seg < kSegs; ++seg) {
         if (plan.nr[seg] > 0) {
-            const DevBlock &B = A.cat.blk[kSegKind[seg]];
+            // the segment's coefficient sets and weight scales go to LDS and are read there
+            // while deriving (held in registers across the line sums they would crowd out the
+            // sums' own)
+            __syncthreads();   // (the previous segment's derivations are done)
+            if (tid == 0)
+                with_prep(A, L, seg, [&](auto, const auto &p0, const auto &p1, double s0, double s1) {
+                    using Prep = std::decay_t<decltype(p0)>;
+                    static_assert(sizeof(Prep) <= sizeof(sh.prep[0]), "coefficient set larger than its LDS slot");
+                    *reinterpret_cast<Prep *>(sh.prep[0]) = p0;
+                    *reinterpret_cast<Prep *>(sh.prep[1]) = p1;
+                    sh.scale[0] = F32 ? s0 : 1.0;
+                    sh.scale[1] = F32 ? s1 : 1.0;
+                });
+            __syncthreads();
             const bool inv = seg == 2;
-            const int nwords = plan.nr[seg] * (kThreads / 32);
-            const double s[2] = {sh.scale[seg][0], sh.scale[seg][1]};
-            double a2[K] = {};
             const int var = kSegVar[seg];
+            const DevBlock &B = A.cat.blk[kSegKind[seg]];
+            const int nwords = plan.nr[seg] * (kThreads / 32);
+            double a2[K] = {}, xv[K];
+            float xfv[K];
+            int sets[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                xv[k] = var ? xn[k] : x[k];
+                xfv[k] = var ? xnf[k] : xf[k];
+                sets[k] = inv ? set[k] : 0;
+            }
+#pragma unroll 1
             for (int v = 0; v < 2; ++v) {
                 if (inv ? !((tsets >> v) & 1) : v > 0) continue;
-                const int stream = kSegStream[seg] + (inv ? v : 0);
-                const unsigned int *mk = masks + A.far_off[stream];
-                const size_t so = static_cast<size_t>(layer) * A.far_words * 32 + static_cast<size_t>(A.far_off[stream]) * 32;
-                if (inv)
-                    near_sum<K, F32>(A, B.n, so, v, mixed, uni_set, mk, nwords, var, set, xn, xnf, a2, sh);
-                else
-                    near_sum<K, F32>(A, B.n, so, 0, false, true, mk, nwords, var, one, var ? xn : x, var ? xnf : xf,
-                                     a2, sh);
+                const unsigned int *mk = masks + A.far_off[kSegStream[seg] + (inv ? v : 0)];
+                const double sv = sh.scale[v];
+                auto derive = [&](int i, double (&c5)[5], float2 &ac) { derive_near<F32>(seg, v, B, i, sh.prep, sv, c5, ac); };
+                near_sum<K, F32>(A, B.n, derive, inv ? v : 0, inv && mixed, inv ? uni_set : true, mk, nwords, var, sets,
+                                 xv, xfv, a2, sh);
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const double sc = inv ? s[set[k]] : s[0];
+                const double sc = sh.scale[inv ? set[k] : 0];
                 if (seg == 0 || seg == 1 || seg == 5)
                     acc[k] = a2[k] / sc;
                 else

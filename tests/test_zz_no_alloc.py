@@ -33,8 +33,9 @@ def cat(K, raw_lines):
 
 @pytest.fixture(scope='module')
 def big_cat(K, raw_lines):
-    """20000 synthetic NH3 inversion lines: the series' staged lines of 160 layers then take
-    about 80 MB, more than the scratch maximum."""
+    """20000 synthetic NH3 inversion lines: the series' workspace (partial sums, moments and
+    near masks) of 320 layers x 1024 frequencies then takes about 87 MB, more than the scratch
+    maximum."""
     return K.jupiter_catalog(raw_lines, nh3_inversion=synth.nh3_inversion_lines(20000, 5))
 
 
@@ -72,13 +73,13 @@ def test_noalloc_per_constituent_rows(K, cat):
 
 @pytest.mark.parametrize('prec,tol', [('fp32', 1e-6), ('fp64', 1e-11)])
 def test_noalloc_falls_back_to_direct_sums(K, big_cat, prec, tol):
-    """The built-in plan of 160 layers x 1024 frequencies is the far-line series, whose staged
-    lines need more scratch than the maximum: the call takes the direct sums instead (bit-identical
+    """The built-in plan of 320 layers x 1024 frequencies is the far-line series, whose workspace
+    needs more scratch than the maximum: the call takes the direct sums instead (bit-identical
     to rbx_alpha with the direct sums split into the parts the fallback chose, and within the
     series' own error of the built-in table)."""
     from radiobear_amd import _lib
     cat = big_cat
-    atm = synth.jupiter_like(160)
+    atm = synth.jupiter_like(320)
     f = np.geomspace(0.3, 300.0, 1024)
     order, nf_low = K.nh3_order(f)
     ab = K.absorbers_struct(JUPITER_ALPHA)
@@ -101,11 +102,11 @@ def test_noalloc_falls_back_to_direct_sums(K, big_cat, prec, tol):
 
 
 def test_noalloc_refuses_what_cannot_fit(K, cat, big_cat):
-    """Forcing the series on a request whose staged lines exceed the scratch maximum, or a
+    """Forcing the series on a request whose workspace exceeds the scratch maximum, or a
     request too large even for one part of the direct sums, returns RBX_ERR_ARG before any
     kernel runs; the next call still works."""
     from radiobear_amd import _lib
-    atm = synth.jupiter_like(160)
+    atm = synth.jupiter_like(320)
     f = np.geomspace(0.3, 300.0, 1024)
     with pytest.raises(_lib.RbxError) as e:
         K.alpha_noalloc(big_cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), f,
