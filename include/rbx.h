@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 7
+#define RBX_ABI_VERSION 8
 
 /* status codes */
 #define RBX_OK          0
@@ -387,7 +387,7 @@ int rbx_integrate_adjoint_workspace_size(int nl, int nf, int nray, int natm, con
 int rbx_alpha_vjp(const double *dalpha, const double *galpha, const double *gT, int nl, int nf,
                   double *grad, void *stream);
 
-/* ---- atmosphere construction (computeProp on the device, atm_base.py:120-181) ----
+/* ---- atmosphere construction (computeProp on the device, atm_base.py:120-194) ----
  * For natm atmospheres laid end to end (layer_off as above), each ordered top
  * (lowest P) to bottom, from P [nl_total] bar, T [nl_total] K and
  * mix [RBX_NUM_SPECIES][nl_total]:
@@ -400,6 +400,33 @@ int rbx_alpha_vjp(const double *dalpha, const double *galpha, const double *gT, 
 int rbx_atmosphere_batch(const double *P, const double *T, const double *mix, const double *Z_in,
                          int nl_total, const int *layer_off, int natm, double req, double gm,
                          double p_ref, double *Z, double *R, double *N, void *stream);
+
+/* computeProp's whole property table (atm_base.py:120-194; replaces the property array
+ * AtmosphereBase.computeProp builds), for the same atmospheres from their layers' P, T, mix
+ * and Z (rbx_atmosphere_batch's Z, or the caller's): props [RBX_NUM_PROPS][nl_total] with the
+ * rows in the reference's order (config LP, default_config.json):
+ *   Z km, R km, P bar, GM km^3/s^2, AMU g/mol, REFR, N, H km, LAPSE K/km, LAPSEP K/bar,
+ *   g km/s^2.
+ * GM is the mass above each layer summed from the top (dM = 1e11 rho 4 pi R^2 |dz|, rho =
+ * AMU P / (R_gas T)) and referred to gm_ref at the layer nearest p_ref; g = GM / R^2,
+ * H = R_gas T / (g AMU) / 1000; LAPSE, LAPSEP are |dT/dz|, |dT/dP| to the layer above (0 at
+ * the top; a repeated z or P gives inf or NaN there, as IEEE division does).
+ */
+#define RBX_NUM_PROPS   11
+#define RBX_PROP_Z      0
+#define RBX_PROP_R      1
+#define RBX_PROP_P      2
+#define RBX_PROP_GM     3
+#define RBX_PROP_AMU    4
+#define RBX_PROP_REFR   5
+#define RBX_PROP_N      6
+#define RBX_PROP_H      7
+#define RBX_PROP_LAPSE  8
+#define RBX_PROP_LAPSEP 9
+#define RBX_PROP_G      10
+int rbx_atmosphere_props(const double *P, const double *T, const double *mix, const double *Z,
+                         int nl_total, const int *layer_off, int natm, double req, double gm_ref,
+                         double p_ref, double *props, void *stream);
 
 #ifdef __cplusplus
 }

@@ -7,7 +7,9 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'librbx.so')
-ABI_VERSION = 7
+ABI_VERSION = 8
+# rows of rbx_atmosphere_props (RBX_PROP_*: the reference's property-table order)
+PROP_ROWS = ('Z', 'R', 'P', 'GM', 'AMU', 'REFR', 'N', 'H', 'LAPSE', 'LAPSEP', 'g')
 
 FP64 = 0
 FP32 = 1
@@ -40,7 +42,7 @@ EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_catalog
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
            'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_profiles',
            'rbx_jacobian', 'rbx_raypaths_batch', 'rbx_integrate_batch', 'rbx_jacobian_batch',
-           'rbx_atmosphere_batch', 'rbx_alpha_rescale', 'rbx_alpha_noalloc', 'rbx_integrate_adjoint',
+           'rbx_atmosphere_batch', 'rbx_atmosphere_props', 'rbx_alpha_rescale', 'rbx_alpha_noalloc', 'rbx_integrate_adjoint',
            'rbx_integrate_adjoint_batch', 'rbx_alpha_vjp', 'rbx_alpha_tangent_workspace_size',
            'rbx_integrate_adjoint_workspace_size')
 NOALLOC_SCRATCH_BYTES = 64 << 20     # RBX_NOALLOC_SCRATCH_BYTES
@@ -125,6 +127,7 @@ _SIGS = {
     'rbx_alpha_vjp': (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
     'rbx_alpha_rescale': (_I, [_VP, _I, _I, _I, _VP, _I, _VP, _VP]),
     'rbx_atmosphere_batch': (_I, [_VP, _VP, _VP, _VP, _I, _VP, _I, _D, _D, _D, _VP, _VP, _VP, _VP]),
+    'rbx_atmosphere_props': (_I, [_VP, _VP, _VP, _VP, _I, _VP, _I, _D, _D, _D, _VP, _VP]),
     'rbx_raypaths_batch': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _VP, _I, _VP, _I, _VP, _I,
                                 _VP, _VP]),
     'rbx_integrate_batch': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _I, _I, _I, _D, _VP,

@@ -621,6 +621,25 @@ def integrate_batch(alpha_table, T, layer_off, ray_atm, hds, nsteps, order=None,
     return Tb
 
 
+def atmosphere_props(P, T, mix, Z, layer_off, req, gm, p_ref=1.0, off_dev=None):
+    """computeProp's property table (rbx_atmosphere_props): a dict of (nl_total,) float64
+    device tensors keyed by _lib.PROP_ROWS (Z, R, P, GM, AMU, REFR, N, H, LAPSE, LAPSEP, g)
+    of atmospheres laid end to end, from their P, T, mix and Z."""
+    P, T, mix, Z = as_device(P), as_device(T), as_device(mix), as_device(Z)
+    nlt = P.shape[0]
+    if T.shape != (nlt,) or Z.shape != (nlt,) or mix.shape != (len(_lib.SPECIES), nlt):
+        raise ValueError('shape mismatch: P {}, T {}, Z {}, mix {}'.format(tuple(P.shape), tuple(T.shape),
+                                                                           tuple(Z.shape), tuple(mix.shape)))
+    off = np.asarray(layer_off.cpu() if isinstance(layer_off, torch.Tensor) else layer_off, dtype=np.int64)
+    if off[0] != 0 or off[-1] != nlt or np.any(np.diff(off) < 1):
+        raise ValueError('layer offsets must run from 0 to nl_total')
+    off_d = _i32(off) if off_dev is None else off_dev
+    props = torch.empty((len(_lib.PROP_ROWS), nlt), dtype=torch.float64, device=device())
+    check(lib().rbx_atmosphere_props(_ptr(P), _ptr(T), _ptr(mix), _ptr(Z), nlt, _ptr(off_d), len(off) - 1,
+                                     float(req), float(gm), float(p_ref), _ptr(props), _stream()))
+    return {k: props[i] for i, k in enumerate(_lib.PROP_ROWS)}
+
+
 def atmosphere_batch(P, T, mix, layer_off, req, gm, p_ref=1.0, Z=None, off_dev=None):
     """Z, R, N (each (nl_total,) float64 on the device) of atmospheres laid end to end;
     Z None: hydrostatic z (rbx_atmosphere_batch).  layer_off: host offsets (checked here);

@@ -270,6 +270,27 @@ def fixture_benchmark():
         os.chdir(cwd)
 
 
+def fixture_props():
+    """F9: computeProp's whole property table (atm_base.py:120-194, rows in the config's LP
+    order) of the benchmark atmosphere, with the config values it uses."""
+    cwd = os.getcwd()
+    os.chdir(RB)
+    try:
+        import radiobear as rb
+        with contextlib.redirect_stdout(io.StringIO()), np.errstate(all='ignore'):
+            j = rb.planet.Planet('jupiter', plot_atm=False, plot_bright=False, verbose=False,
+                                 config_file='config_benchmark.par')
+            j.run([5.0], b=[[0.0, 0.0]], reuse_override='False')
+        atm = j.atmos[0]
+        lp = atm.config.LP
+        rows = sorted(lp, key=lp.get)
+        np.savez_compressed(os.path.join(HERE, 'props.npz'), gas=atm.gas, prop=np.array(atm.property),
+                            rows=np.array(rows), Req=atm.config.Req, GM_ref=atm.config.GM_ref,
+                            p_ref=atm.config.p_ref)
+    finally:
+        os.chdir(cwd)
+
+
 FREQ_REQUESTS = [([1.0, 2.5, 30.0], 'GHz'), ('1,2,3.5', 'GHz'), ('1:96:5', 'GHz'), ('0.3:2.0:0.1', 'GHz'),
                  ('5', 'GHz'), (7, 'GHz'), (2.5, 'GHz'), ('100:3000:250', 'MHz'), ('0.5,1.25', 'cm'),
                  ('0.3;300;7', 'GHz')]
@@ -337,10 +358,14 @@ if __name__ == '__main__':
     if sys.argv[1:] == ['requests']:
         fixture_requests()
         sys.exit(0)
+    if sys.argv[1:] == ['props']:
+        fixture_props()
+        sys.exit(0)
     fixture_absorbers()
     fixture_other_absorbers()
     fixture_synthetic_lines()
     fixture_geometry()
     fixture_benchmark()
     fixture_requests()
+    fixture_props()
     print('fixtures written to', HERE)

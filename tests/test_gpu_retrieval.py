@@ -168,6 +168,41 @@ def test_device_atmosphere_given_z_matches_oracle_compute_prop(K, bench_gas_text
     assert np.max(np.abs(R.cpu().numpy() - prop['R']) / prop['R']) < 1e-14
 
 
+def test_device_property_table_matches_oracle(K, bench_gas_text, golden):
+    """rbx_atmosphere_props on the benchmark atmosphere (its own z) against the oracle's
+    computeProp (pinned to the reference-written table), and on a batch of perturbed
+    atmospheres with the hydrostatic z built on the device, each against the oracle."""
+    from oracle import pipeline
+    from oracle import atmosphere as OA
+    F = golden('props.npz')
+    req, gm, pref = float(F['Req']), float(F['GM_ref']), float(F['p_ref'])
+    gas, _ = pipeline.benchmark_atmosphere(bench_gas_text)
+    got = K.atmosphere_props(gas[C['P']], gas[C['T']], K.mix_table(gas, C), gas[C['Z']], [0, gas.shape[1]],
+                             req, gm, pref)
+    ref = OA.compute_prop(gas, C, req, pref, GM_ref=gm)
+    for r in OA.PROP_ROWS:
+        g, e = got[r].cpu().numpy(), ref[r]
+        assert np.max(np.abs(g - e) / np.maximum(np.abs(e), 1e-300)) < 1e-13, r
+        assert np.max(np.abs(g - F['prop'][OA.PROP_ROWS.index(r)]) / np.maximum(np.abs(e), 1e-300)) < 1e-13, r
+    atms = synth.retrieval_batch(7, seed=5, nl_range=(30, 120))
+    off = np.concatenate([[0], np.cumsum([a.nl for a in atms])])
+    cat = lambda f: np.concatenate([f(a) for a in atms])
+    mix = np.concatenate([K.mix_table(a.gas, C) for a in atms], axis=1)
+    P, T = cat(lambda a: a.P), cat(lambda a: a.T)
+    Z, _, _ = K.atmosphere_batch(P, T, mix, off, req, gm)
+    got = K.atmosphere_props(P, T, mix, Z, off, req, gm)
+    Zh = Z.cpu().numpy()
+    for a, atm in enumerate(atms):
+        sl = slice(off[a], off[a + 1])
+        g2 = atm.gas.copy()
+        g2[C['Z']] = Zh[sl]
+        ref = OA.compute_prop(g2, C, req, 1.0, GM_ref=gm)
+        for r in OA.PROP_ROWS:
+            e = ref[r]
+            d = got[r].cpu().numpy()[sl]
+            assert np.max(np.abs(d - e) / np.maximum(np.abs(e), 1e-300)) < 1e-13, (a, r)
+
+
 def test_retrieval_with_device_atmosphere(K, small_batch):
     """The batch rebuilt on the device (hydrostatic z) integrates to the same Tb and J as
     with the host-built shells."""

@@ -54,6 +54,44 @@ def test_benchmark_fixture(bench, golden, bench_gas_text):
     assert np.array_equal(prop['N'], ref['N'])
 
 
+def test_property_table_fixture(golden, bench_gas_text):
+    """computeProp's whole property table of the benchmark atmosphere against the table the
+    reference wrote (props.npz): bit-identical but for g and H, which round once differently."""
+    from oracle import atmosphere as OA
+    F = golden('props.npz')
+    gas, _ = pipeline.benchmark_atmosphere(bench_gas_text)
+    assert np.array_equal(gas, F['gas'])
+    rows = [str(r) for r in F['rows']]
+    assert tuple(rows) == OA.PROP_ROWS
+    got = OA.compute_prop(gas, OA.C_JUPITER, float(F['Req']), float(F['p_ref']), GM_ref=float(F['GM_ref']))
+    for i, r in enumerate(rows):
+        ref = F['prop'][i]
+        if r in ('g', 'H'):
+            assert np.max(np.abs(got[r] - ref) / np.abs(ref)) < 1e-15, r
+        else:
+            assert np.array_equal(got[r], ref), r
+
+
+def test_property_table_identities(bench_gas_text):
+    """What the definitions fix: GM equals GM_ref at the layer nearest p_ref and drops by the
+    mass of each deeper shell; g R^2 = GM; H m g = R_gas T / 1000; a planted slip (the shell
+    volume without its 4 pi) moves GM off the reference table."""
+    from oracle import atmosphere as OA
+    gas, _ = pipeline.benchmark_atmosphere(bench_gas_text)
+    C, GM_ref = OA.C_JUPITER, 126686538.0
+    p = OA.compute_prop(gas, C, 71492.0, 1.0, GM_ref=GM_ref)
+    i_ref = int(np.argmin(np.abs(gas[C['P']] - 1.0)))
+    assert p['GM'][i_ref] == GM_ref
+    assert np.all(np.diff(p['GM']) < 0)                   # layers run top to bottom: mass inside shrinks
+    assert np.max(np.abs(p['g'] * p['R'] ** 2 - p['GM']) / p['GM']) < 1e-15
+    assert np.max(np.abs(p['H'] * p['g'] * p['AMU'] * 1000.0 - OA.R_GAS * gas[C['T']]) / gas[C['T']]) < 1e-13
+    rho = p['AMU'][1:] * gas[C['P']][1:] / (OA.R_GAS * gas[C['T']][1:])
+    shell = 4.0 * np.pi * p['R'][1:] ** 2 * np.abs(np.diff(gas[C['Z']]))
+    # (GM ~1.3e8 km^3/s^2 against shell terms down to 2e-3: its differences keep ~1e-5)
+    assert np.allclose(-np.diff(p['GM']), OA.GRAV_CONST * 1.0e11 * rho * shell, rtol=1e-4, atol=0)
+    assert not np.allclose(-np.diff(p['GM']), OA.GRAV_CONST * 1.0e11 * rho * shell / (4.0 * np.pi), rtol=1e-3)
+
+
 ABS = {
     'h2_e': lambda L, f, T, P, X: A.h2_jj_ddb(f, T, P, X, C, {'h2state': 'e'}, units='invcm'),
     'h2_n': lambda L, f, T, P, X: A.h2_jj_ddb(f, T, P, X, C, {'h2state': 'n'}, units='invcm'),
