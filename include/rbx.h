@@ -63,10 +63,11 @@ extern "C" {
 /* arithmetic modes */
 #define RBX_FP64        0   /* plain FP64 everywhere (IEEE division, FP64 exp) */
 #define RBX_FP32        1   /* FP32 line sums, compensated (FP64 outer accumulation);
-                               FP64 detuning, tau prefix sum and geometry; exp(-tau)
-                               from an FP64 range reduction and an FP32 exp2; point-ray
-                               weights and trapezoid sums in FP32, added into FP64 totals
-                               every few steps */
+                               FP64 detuning, geometry, half steps and tau prefix sum;
+                               point rays: e^-tau as the FP32 exp2 of the FP32-rounded
+                               exponent, weights and trapezoid sums in FP32 added into
+                               FP64 totals every batch of steps; disc rays and batches:
+                               e^-tau from an FP64 range reduction and an FP32 exp2 */
 
 /* line-block kinds and their columns (float64, one value per line) */
 #define RBX_LINES_NH3_INV  0  /* fo, Io, Eo, gammaNH3o                     (ammonia.npz) */
@@ -170,8 +171,8 @@ typedef struct rbx_launch {
                            series (at most 8e-7 / 8e-12 of a far line's value: FP32 tables within
                            3e-7 of FP64 as the direct sums, FP64 tables within 1e-11 of the direct
                            FP64 sums); built in: 2 for requests of 4 or more wavefront tiles while
-                           the workspace (24 B FP32, 40 B FP64 per line and layer more) stays
-                           under 4 GiB, else 1 */
+                           the workspace (the tiles' moments and near-line masks: about 0.1 MB per
+                           layer at 1024 frequencies and 25553 lines) stays under 4 GiB, else 1 */
     int alpha_split;    /* parts each layer's direct line sums are split into: 1..16 (built in: a
                            model of the grid's last dispatch wave) */
     int far_split;      /* parts each layer's lines are split into in the series' classification
