@@ -109,6 +109,12 @@ class Problem:
         self.ws = None
         self.side = None
         self.launch = {}    # precision -> launch shape ({field: value}) used by step()
+        # integration in the order the frequencies pass the tau cut (rbx_integrate_order):
+        # precision -> on / off, and its buffers
+        self.ordered = {}
+        self.cols = torch.empty((self.nf,), dtype=torch.int32, device=dev)
+        self.alpha32_cols = torch.empty_like(self.alpha32)
+        self.alpha64_cols = torch.empty_like(self.alpha64)
 
     # the three launches of one step, on torch's current stream
     def launch_alpha(self, prec, launch=None):
@@ -132,7 +138,7 @@ class Problem:
                                            K._ptr(self.b), self.nray, 1, K._ptr(self.hds), self.ld,
                                            K._ptr(self.ns), K._stream()))
 
-    def launch_integrate(self, prec, launch=None, tau_cut=None, Tb=None):
+    def launch_integrate(self, prec, launch=None, tau_cut=None, Tb=None, ordered=None):
         from radiobear_amd import _lib
         K = self.K
         a = self.alpha32 if prec == _lib.FP32 else self.alpha64
@@ -140,7 +146,19 @@ class Problem:
             Tb = self.Tb if prec == _lib.FP32 else self.Tb64
         if tau_cut is None:
             tau_cut = K.FP32_TAU_CUT if prec == _lib.FP32 else K.FP64_TAU_CUT
+        if ordered is None:
+            ordered = self.ordered.get(prec, False)
         ls = _lib.launch_struct(self.launch.get(prec) if launch is None else launch)
+        if ordered and tau_cut > 0.0:
+            ac = self.alpha32_cols if prec == _lib.FP32 else self.alpha64_cols
+            _lib.check(_lib.lib().rbx_integrate_order(K._ptr(a), prec, self.nl, self.nf, K._ptr(self.hds), self.ld,
+                                                      K._ptr(self.ns), self.nray, tau_cut, K._ptr(self.cols),
+                                                      K._ptr(ac), K._stream()))
+            _lib.check(_lib.lib().rbx_integrate_cols(K._ptr(ac), prec, K._ptr(self.cols), K._ptr(self.T), self.nl,
+                                                     self.nf, K._ptr(self.hds), self.ld, K._ptr(self.ns), self.nray,
+                                                     0, prec, tau_cut, K._ptr(Tb), None, _lib.launch_ref(ls),
+                                                     K._stream()))
+            return
         _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
                                             self.ld, K._ptr(self.ns), self.nray, 0, prec, tau_cut, K._ptr(Tb), None,
                                             _lib.launch_ref(ls), K._stream()))
@@ -190,12 +208,25 @@ def autotune(prob, prec, args):
         la, rec_s = at.tune(split, key, lambda L: prob.launch_alpha(prec, L), la, path=args.tune_cache)
         li, rec_i = at.tune_all(('int_unroll', 'int_map'), key, lambda L: prob.launch_integrate(prec, L),
                                 path=args.tune_cache)
+        # the integration in request order or in cut order (order + reordered table included)
+        prob.launch[prec] = dict(la, **li)
+        order_ms = {}
+        for on in (False, True):
+            prob.launch_integrate(prec, ordered=on)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(5):
+                prob.launch_integrate(prec, ordered=on)
+            ev[1].record()
+            torch.cuda.synchronize()
+            order_ms[str(int(on))] = ev[0].elapsed_time(ev[1]) / 5
     torch.cuda.synchronize()
-    prob.launch[prec] = dict(la, **li)
+    prob.ordered[prec] = order_ms['1'] < order_ms['0']
     return {'alpha_path': 'far-line series' if la['alpha_series'] == 2 else 'direct line sums',
             'launch': prob.launch[prec], 'alpha_series_ms': rec['alpha_series']['ms'],
             'alpha_k_ms': rec['alpha_k']['ms'], split + '_ms': rec_s['ms'],
-            'int_unroll_ms': rec_i['int_unroll']['ms'], 'int_map_ms': rec_i['int_map']['ms']}
+            'int_unroll_ms': rec_i['int_unroll']['ms'], 'int_map_ms': rec_i['int_map']['ms'],
+            'int_order': int(prob.ordered[prec]), 'int_order_ms': order_ms}
 
 
 def other_workloads(args):

@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define RBX_ABI_VERSION 8
+#define RBX_ABI_VERSION 9
 
 /* status codes */
 #define RBX_OK          0
@@ -308,6 +308,23 @@ int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int n
                   const double *hds, int ld, const int *nsteps, int nray,
                   int mode, int precision, double tau_cut, double *Tb, double *intW,
                   const rbx_launch *launch, void *stream);
+
+/* The same integration with the frequencies taken in the order of cols (no counterpart in
+ * the reference; Brightness.single, brightness.py:30-126, integrates every frequency to the
+ * end of the path).  A wavefront of 64 frequencies stops at the tau cut only once all of them
+ * are past it, so frequencies that pass it at similar depths should share wavefronts.
+ * rbx_integrate_order writes such an order, cols [nf] (column j of the reordered table is
+ * request frequency cols[j]; a permutation), ranking the frequencies by the path step at which
+ * the on-planet ray with the longest path passes tau_cut (stable; at most 16384 frequencies
+ * and 131072 layers),
+ * and the table in that order, alpha_cols [nl][nf] (alpha's precision).  rbx_integrate_cols
+ * integrates alpha_cols and writes Tb and intW to column cols[j] of each row, so its outputs
+ * are rbx_integrate's up to what the cut itself changes (e^-tau_cut of a ray's weight). */
+int rbx_integrate_order(const void *alpha, int alpha_precision, int nl, int nf, const double *hds, int ld,
+                        const int *nsteps, int nray, double tau_cut, int *cols, void *alpha_cols, void *stream);
+int rbx_integrate_cols(const void *alpha_cols, int alpha_precision, const int *cols, const double *T, int nl,
+                       int nf, const double *hds, int ld, const int *nsteps, int nray, int mode, int precision,
+                       double tau_cut, double *Tb, double *intW, const rbx_launch *launch, void *stream);
 
 /* ---- per-point profiles (Brightness.single's .tau, .W, .Tb_lyr, brightness.py:53-120) ----
  * tau, W, Tb_lyr  [nray][nl][nf] float64 at path points 0..nsteps-1 (0 beyond);

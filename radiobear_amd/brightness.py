@@ -12,6 +12,8 @@ import torch
 
 from . import kernels, raypath, utils
 
+ORDER_MIN_RAY_FREQS = 1 << 23   # point ray x frequency pairs from which batch() orders for the cut
+
 
 class Brightness:
     """Brightness temperature of the planet."""
@@ -71,8 +73,11 @@ class Brightness:
         point = [i for i, d in enumerate(disc) if not d]
         if point:
             sel = torch.as_tensor(point, device=hds.device)
+            # frequencies in the order they pass the cut (a fixed ~0.07 ms that saves ~10 % of
+            # the integration): for large batches only
+            big = len(point) * alpha.table.shape[1] >= ORDER_MIN_RAY_FREQS
             Tb[point] = kernels.integrate(alpha.table, T, hds[sel], ns[sel], precision=prec,
-                                          tau_cut=tau_cut).cpu().numpy()
+                                          tau_cut=tau_cut, ordered=big).cpu().numpy()
         discs = [i for i, d in enumerate(disc) if d]
         if discs:
             sel = torch.as_tensor(discs, device=hds.device)

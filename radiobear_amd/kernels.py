@@ -367,8 +367,16 @@ FP32_TAU_CUT = 20.0
 FP64_TAU_CUT = 40.0
 
 
-def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False, launch=None):
-    """Tb (nray, nf) float64 from an alpha table (nl, nf) and ray paths."""
+MAX_ORDER_FREQS = 16384   # rbx_integrate_order's limits
+MAX_ORDER_LAYERS = 131072
+
+
+def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0.0, with_intW=False, launch=None,
+              ordered=False):
+    """Tb (nray, nf) float64 from an alpha table (nl, nf) and ray paths.  ordered: integrate the
+    frequencies in the order in which a reference ray passes tau_cut (rbx_integrate_order +
+    rbx_integrate_cols; with a cut only): the wavefronts then stop together, and Tb differs
+    from the request order's only by what the cut itself changes."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
     prec = aprec if precision is None else _lib.PRECISIONS[precision]
     a = alpha_table.contiguous()
@@ -378,6 +386,15 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
     hds, nsteps = hds.contiguous(), nsteps.contiguous()   # held here: the C call reads their memory
     Tb = torch.empty((nray, nf), dtype=torch.float64, device=device())
     iw = torch.empty((nray, nf), dtype=torch.float64, device=device()) if with_intW else None
+    if ordered and tau_cut > 0.0 and nf <= MAX_ORDER_FREQS and nl <= MAX_ORDER_LAYERS and nray > 0:
+        cols = torch.empty((nf,), dtype=torch.int32, device=device())
+        a_cols = torch.empty_like(a)
+        check(lib().rbx_integrate_order(_ptr(a), aprec, nl, nf, _ptr(hds), ld, _ptr(nsteps), nray, float(tau_cut),
+                                        _ptr(cols), _ptr(a_cols), _stream()))
+        check(lib().rbx_integrate_cols(_ptr(a_cols), aprec, _ptr(cols), _ptr(T), nl, nf, _ptr(hds), ld,
+                                       _ptr(nsteps), nray, 1 if disc else 0, prec, float(tau_cut),
+                                       _ptr(Tb), _ptr(iw), _lib.launch_ref(launch), _stream()))
+        return (Tb, iw) if with_intW else Tb
     check(lib().rbx_integrate(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds), ld,
                               _ptr(nsteps), nray, 1 if disc else 0, prec, float(tau_cut),
                               _ptr(Tb), _ptr(iw), _lib.launch_ref(launch), _stream()))
