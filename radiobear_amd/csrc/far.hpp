@@ -337,7 +337,7 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) 
                     sh.thr[tid / kFarTiles][j] = (used >> j) & 1u ? far_threshold<F32>(sh, j, rv, tid >= kFarTiles) : INFINITY;
                 }
                 __syncthreads();
-                const int soff = (inv && v) ? A.far_off[kSegStream[SEG] + 1] : A.far_off[kSegStream[SEG]];
+                const int soff = A.far_off[kSegStream[SEG] + ((inv && v) ? 1 : 0)];
                 unsigned int *mrow = A.near + (static_cast<size_t>(layer) * A.tiles_total + g * kFarTiles) * A.far_words + soff;
                 double accB[kFarTerms], accA[kFarTerms];
 #pragma unroll

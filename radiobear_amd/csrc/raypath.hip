@@ -102,6 +102,14 @@ __device__ inline void shell_at(const Geo &G, V3 v, double &f, V3 &n) {
     n = {G.ratio * v.x * inv, v.y * inv, G.ratio * v.z * inv};
 }
 
+// Rays per block.  One thread per ray: a ray's path is one dependent chain of FP64 steps, so
+// the kernel is latency-bound and holds few wavefronts (256 for 16384 rays).  It runs beside the
+// line sums (rbx_alpha) when a caller puts it on a second stream or graph branch; four
+// wavefronts per block keep it on a quarter as many CUs as one per block, leaving the other
+// CUs' slots whole for the line sums: 800 layers x 1024 frequencies x 16384 rays, graph step
+// 3.71 -> 3.47 ms (the paths alone 0.42 -> 0.45 ms; 2 and 8 wavefronts per block: 3.59, 3.53 ms).
+constexpr int kRayThreads = 256;
+
 struct RayArgs {
     Geo G;
     const double *R, *N;
@@ -119,7 +127,7 @@ struct RayArgs {
     int natm;
 };
 
-__global__ __launch_bounds__(64) void raypath_kernel(RayArgs A) {
+__global__ __launch_bounds__(kRayThreads) void raypath_kernel(RayArgs A) {
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= A.nray) return;
     const double b0 = A.b[2 * ray], b1 = A.b[2 * ray + 1];
@@ -267,7 +275,7 @@ static int fill_ray_args(rbx::RayArgs &A, const rbx_planet *planet, const double
 }
 
 static int launch_rays(const rbx::RayArgs &A, void *stream) {
-    const int threads = 64;
+    const int threads = rbx::kRayThreads;
     hipLaunchKernelGGL(rbx::raypath_kernel, dim3((A.nray + threads - 1) / threads), dim3(threads), 0,
                        static_cast<hipStream_t>(stream), A);
     return rbx::launch_status("raypath_kernel");
