@@ -185,12 +185,20 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
 // (blocked summation).  e^-tau is the hardware exp2 of t = -tau log2(e) with tau rounded to
 // FP32 first: the rounding moves the exponent by at most |t| 2^-24, so a weight moves by at
 // most tau e^-tau 2^-24 ~ 2e-8 of its alpha (largest at tau = 1), below the FP32 sums' own
-// error; past t = -150 the result is 0.  Each wavefront first copies its ray's half steps
-// rounded to FP32 and layer temperatures to LDS as pairs (h_i, T_{i+1}) for the FP32 sums,
-// so a step reads them with one broadcast LDS load.
+// error; past t = -150 the result is 0.  The trapezoid sums are taken by path point:
+// sum_i (W_i + W_{i+1}) h_i = sum_p W_p (h_{p-1} + h_p) (W_0 = 0, h_n = 0), and the same with
+// T_p W_p, so a step adds W_p times two staged factors (two FMAs, where the pairwise form took
+// two adds, two FMAs and a multiply).  Each wavefront first copies its ray's factors to LDS as
+// FP32 pairs (h_{p-1} + h_p, T_p (h_{p-1} + h_p)), so a step reads them with one broadcast LDS
+// load; a wavefront stopped by the cut at point p takes W_p h_p and T_p W_p h_p back out (its
+// sums end at p, as the pairwise form's do).
+// BUF: the table rows are read as raw buffer loads whose row offset is a scalar register
+// (the row's byte offset computed on the scalar unit), so a step's load costs no vector
+// address arithmetic; for tables under 2 GiB.
 constexpr float kLog2ef = 1.44269504f;
+constexpr int kBufWord3 = 0x00020000;   // buffer descriptor word 3: 32-bit data format, no swizzle
 
-template <int U>
+template <int U, bool BUF>
 __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs A) {
     extern __shared__ float2 hT_all[];   // [kWaves][ld]
     const int wave = threadIdx.x / kLanes;
@@ -210,24 +218,38 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
     const float *al = static_cast<const float *>(A.alpha) + static_cast<size_t>(base) * nf + jj;
     const double *Tl = A.T + base;
     float2 *hT = hT_all + static_cast<size_t>(wave) * A.ld;
+    [[maybe_unused]] __amdgpu_buffer_rsrc_t rows;
+    [[maybe_unused]] const int nfb = static_cast<int>(nf) * 4, lane_b = jj * 4;
+    if constexpr (BUF)
+        rows = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(static_cast<const float *>(A.alpha) + static_cast<size_t>(base) * nf), 0,
+            max(nlr, 1) * nfb, kBufWord3);
+    auto row = [&](int r) -> float {   // table row r of the ray's atmosphere, this lane's column
+        if constexpr (BUF)
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rows, lane_b, r * nfb, 0));
+        else
+            return al[static_cast<size_t>(r) * nf];
+    };
     double tau = 0.0, intW = 0.0, num = 0.0;
     if (S > 1) {
         const int n = S - 1;
         bool bad = false;   // a NaN half step or temperature: no cut (path_has_nan)
         for (int k = lane; k < n; k += kLanes) {
-            hT[k] = make_float2(static_cast<float>(h[k]), static_cast<float>(Tl[k + 1]));
+            const double hh = h[k] + (k + 1 < n ? h[k + 1] : 0.0);   // point k + 1's factor
+            hT[k] = make_float2(static_cast<float>(hh), static_cast<float>(Tl[k + 1] * hh));
             bad = bad || __builtin_isnan(h[k]) || __builtin_isnan(Tl[k + 1]);
         }
         const double cut = (A.tau_cut > 0.0 && !__any(bad)) ? A.tau_cut : 0.0;
+        const unsigned long long idle = __builtin_amdgcn_ballot_w64(!ok);   // lanes past the last column
         wave_sync();
-        float a0 = al[0], W0 = 0.0f, TW0 = 0.0f;
+        float a0 = row(0), W1 = 0.0f;
         int i = 0;
         for (; i + U <= n; i += U) {
             float ab[U];
             double hb[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                ab[u] = al[(i + u + 1) * nf];
+                ab[u] = row(i + u + 1);
                 hb[u] = h[i + u];
             }
             float iW = 0.0f, nm = 0.0f;
@@ -235,32 +257,32 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
             for (int u = 0; u < U; ++u) {
                 const float2 ht = hT[i + u];
                 tau = fma(static_cast<double>(a0 + ab[u]), hb[u], tau);
-                const float W1 = ab[u] * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
-                iW = fmaf(W1 + W0, ht.x, iW);
-                const float TW1 = ht.y * W1;
-                nm = fmaf(TW1 + TW0, ht.x, nm);
+                W1 = ab[u] * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
+                iW = fmaf(W1, ht.x, iW);
+                nm = fmaf(W1, ht.y, nm);
                 a0 = ab[u];
-                W0 = W1;
-                TW0 = TW1;
             }
             intW += static_cast<double>(iW);
             num += static_cast<double>(nm);
-            if (cut > 0.0 && __all(!ok || tau > cut)) {
+            if (cut > 0.0 && (__builtin_amdgcn_ballot_w64(tau > cut) | idle) == __builtin_amdgcn_read_exec()) {
+                const int p = i + U;   // the sums end at point p
+                if (p < n) {
+                    const double hp = h[p] * static_cast<double>(W1);
+                    intW -= hp;
+                    num -= hp * Tl[p];
+                }
                 i = n;
                 break;
             }
         }
         for (; i < n; ++i) {
-            const float a1 = al[(i + 1) * nf];
+            const float a1 = row(i + 1);
             const float2 ht = hT[i];
             tau = fma(static_cast<double>(a0 + a1), h[i], tau);
-            const float W1 = a1 * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
-            intW += static_cast<double>((W1 + W0) * ht.x);
-            const float TW1 = ht.y * W1;
-            num += static_cast<double>((TW1 + TW0) * ht.x);
+            W1 = a1 * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
+            intW += static_cast<double>(W1 * ht.x);
+            num += static_cast<double>(W1 * ht.y);
             a0 = a1;
-            W0 = W1;
-            TW0 = TW1;
         }
     }
     if (!ok) return;
@@ -927,10 +949,15 @@ static int launch_integrate(rbx::IntArgs A, const rbx_launch &L, void *stream) {
     } else if (A.f32) {
         const size_t lds = sizeof(float2) * rbx::kWaves * static_cast<size_t>(A.ld);
         if (A.fast_exp && lds <= 64 * 1024) {
-            if (u4)
-                hipLaunchKernelGGL(rbx::integrate_f32_kernel<4>, grid, block, lds, st, A);
+            const bool buf = static_cast<size_t>(A.nl) * static_cast<size_t>(A.nf) * 4 < (size_t(1) << 31);
+            if (u4 && buf)
+                hipLaunchKernelGGL((rbx::integrate_f32_kernel<4, true>), grid, block, lds, st, A);
+            else if (u4)
+                hipLaunchKernelGGL((rbx::integrate_f32_kernel<4, false>), grid, block, lds, st, A);
+            else if (buf)
+                hipLaunchKernelGGL((rbx::integrate_f32_kernel<8, true>), grid, block, lds, st, A);
             else
-                hipLaunchKernelGGL(rbx::integrate_f32_kernel<8>, grid, block, lds, st, A);
+                hipLaunchKernelGGL((rbx::integrate_f32_kernel<8, false>), grid, block, lds, st, A);
         } else if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(float, 0, true, 4); else RBX_INT_LAUNCH(float, 0, true, 8); }
         else { if (u4) RBX_INT_LAUNCH(float, 0, false, 4); else RBX_INT_LAUNCH(float, 0, false, 8); }
     } else {
