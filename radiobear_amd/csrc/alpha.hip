@@ -123,6 +123,7 @@ struct AlphaArgs {
     // [nl][far_split][tiles_total][kFarMom], h2o base terms [far_split][nl]
     unsigned int *near;
     double *far_mom, *far_base;
+    double *far_prep;   // the segments' coefficient derivers per layer [nl][kSegs][kPrepWords]
     int far_words, far_split;
     int far_off[kFarStreams];   // first mask word of each stream
     // tangent pass split into tan_parts parts of a layer's tan_rounds 64-line rounds;
@@ -1581,7 +1582,8 @@ static size_t alpha_ws_bytes(const AlphaArgs &A, int precision, int nsplit, bool
     size_t b = (static_cast<size_t>(kSlots) * nsplit * plane + static_cast<size_t>(nsplit) * A.nl + extra) *
                sizeof(double);
     if (series)
-        b += (static_cast<size_t>(A.nl) * A.far_split * A.tiles_total * kFarMom + static_cast<size_t>(A.far_split) * A.nl) *
+        b += (static_cast<size_t>(A.nl) * A.far_split * A.tiles_total * kFarMom + static_cast<size_t>(A.far_split) * A.nl +
+              static_cast<size_t>(A.nl) * kSegs * kPrepWords) *
                  sizeof(double) +
              static_cast<size_t>(A.nl) * A.tiles_total * A.far_words * sizeof(unsigned int);
     return b;
@@ -1614,9 +1616,13 @@ static int launch_alpha(AlphaArgs &A, int precision, int nsplit, hipStream_t st)
 template <int K, bool F32>
 static int launch_series(AlphaArgs &A, hipStream_t st) {
     const long long groups = (A.tiles_total + kFarTiles - 1) / kFarTiles;
+    hipLaunchKernelGGL(alpha_prep_kernel<F32>, dim3(static_cast<unsigned>((static_cast<long long>(A.nl) * kSegs + 255) / 256)),
+                       dim3(256), 0, st, A);
+    int rc = launch_status("alpha_prep_kernel");
+    if (rc) return rc;
     dim3 g1(static_cast<unsigned>(groups * A.nl), static_cast<unsigned>(A.far_split));
     hipLaunchKernelGGL(alpha_far_kernel<F32>, g1, dim3(kThreads), 0, st, A);
-    int rc = launch_status("alpha_far_kernel");
+    rc = launch_status("alpha_far_kernel");
     if (rc) return rc;
     dim3 g2(static_cast<unsigned>(static_cast<long long>(A.tiles_total) * A.nl));
     hipLaunchKernelGGL((alpha_near_kernel<K, F32>), g2, dim3(kThreads), 0, st, A);
@@ -1674,7 +1680,8 @@ static int run_planned(AlphaArgs &A, int precision, int K, int nsplit, bool seri
         const size_t plane = static_cast<size_t>(nl) * nf;
         A.far_mom = A.ws_extra + (has_extra(A.abs) ? static_cast<size_t>(kNumExtra) * plane : 0);
         A.far_base = A.far_mom + static_cast<size_t>(nl) * A.far_split * A.tiles_total * kFarMom;
-        A.near = reinterpret_cast<unsigned int *>(A.far_base + static_cast<size_t>(A.far_split) * nl);
+        A.far_prep = A.far_base + static_cast<size_t>(A.far_split) * nl;
+        A.near = reinterpret_cast<unsigned int *>(A.far_prep + static_cast<size_t>(nl) * kSegs * kPrepWords);
         switch (K) {
             case 4: return launch_series<4>(A, precision, st);
             case 2: return launch_series<2>(A, precision, st);

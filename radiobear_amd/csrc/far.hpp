@@ -123,45 +123,115 @@ __device__ inline double recip(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
-// the segment's coefficient deriver(s) for a layer and their FP32 weight scales (FP64 mode
-// uses none), in scalar registers: f(segment as a compile-time constant, set 0 deriver,
-// set 1 deriver (the same one outside the nh3_hs inversion lines), scale 0, scale 1)
+// The segments' coefficient derivers per layer (the layer's powers of T and partial pressures,
+// computed once by alpha_prep_kernel): [nl][kSegs][kPrepWords] doubles, set 0's deriver in
+// words 0-7, set 1's (the same one outside the nh3_hs inversion lines) in 8-15, and their FP32
+// weight scales (1 in FP64 mode) in 16 and 17.  The passes read a layer's row with uniform loads.
+constexpr int kPrepWords = 18;
 template <int S>
 using SegC = std::integral_constant<int, S>;
-template <class F>
-__device__ __forceinline__ void with_prep(const AlphaArgs &A, const LayerT<double> &L, int seg, F &&f) {
+
+template <bool F32>
+__global__ __launch_bounds__(256) void alpha_prep_kernel(AlphaArgs A) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.nl * kSegs) return;
+    const int layer = e / kSegs, seg = e % kSegs;
+    const LayerT<double> L = layer_of(A, layer);
     const DevBlock *blk = A.cat.blk;
+    double *row = A.far_prep + static_cast<size_t>(e) * kPrepWords;
+    auto put = [&](const auto &p0, const auto &p1, double s0, double s1) {
+        using Prep = std::decay_t<decltype(p0)>;
+        static_assert(sizeof(Prep) <= 8 * sizeof(double) && sizeof(Prep) % sizeof(double) == 0,
+                      "a deriver is at most 8 doubles");
+        const double *w0 = reinterpret_cast<const double *>(&p0), *w1 = reinterpret_cast<const double *>(&p1);
+        for (int k = 0; k < 8; ++k) {
+            row[k] = k < static_cast<int>(sizeof(Prep) / sizeof(double)) ? w0[k] : 0.0;
+            row[8 + k] = k < static_cast<int>(sizeof(Prep) / sizeof(double)) ? w1[k] : 0.0;
+        }
+        row[16] = F32 ? s0 : 1.0;
+        row[17] = F32 ? s1 : 1.0;
+    };
     switch (seg) {
         case 0: {
-            const H2oPrep<double> p = uniform(H2oPrep<double>(L));
-            f(SegC<0>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_H2O].wscale), 1.0);
+            const H2oPrep<double> p(L);
+            put(p, p, pow2_scale(p.pref * blk[RBX_LINES_H2O].wscale), 1.0);
             break;
         }
         case 1: {
-            const H2sPrep<double> p = uniform(H2sPrep<double>(L));
-            f(SegC<1>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_H2S].wscale), 1.0);
+            const H2sPrep<double> p(L);
+            put(p, p, pow2_scale(p.pref * blk[RBX_LINES_H2S].wscale), 1.0);
             break;
         }
         case 2: {
-            const InvPrep<double> p0 = uniform(InvPrep<double>(L, nh3_low_set(A.abs.nh3)));
-            const InvPrep<double> p1 = uniform(InvPrep<double>(L, 1));
+            const InvPrep<double> p0(L, nh3_low_set(A.abs.nh3)), p1(L, 1);
             const double w = blk[RBX_LINES_NH3_INV].wscale;
-            f(SegC<2>{}, p0, p1, pow2_scale(p0.pref * w), pow2_scale(p1.pref * w));
+            put(p0, p1, pow2_scale(p0.pref * w), pow2_scale(p1.pref * w));
             break;
         }
         case 3: {
-            const RotPrep<double> p = uniform(RotPrep<double>(L));
-            f(SegC<3>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_NH3_ROT].wscale), 1.0);
+            const RotPrep<double> p(L);
+            put(p, p, pow2_scale(p.pref * blk[RBX_LINES_NH3_ROT].wscale), 1.0);
             break;
         }
         case 4: {
-            const V2Prep<double> p = uniform(V2Prep<double>(L));
-            f(SegC<4>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_NH3_V2].wscale), 1.0);
+            const V2Prep<double> p(L);
+            put(p, p, pow2_scale(p.pref * blk[RBX_LINES_NH3_V2].wscale), 1.0);
             break;
         }
         default: {
-            const Ph3Prep<double> p = uniform(Ph3Prep<double>(L));
-            f(SegC<5>{}, p, p, pow2_scale(p.pref * blk[RBX_LINES_PH3].wscale), 1.0);
+            const Ph3Prep<double> p(L);
+            put(p, p, pow2_scale(p.pref * blk[RBX_LINES_PH3].wscale), 1.0);
+            break;
+        }
+    }
+}
+
+// a deriver from its table words (in scalar registers)
+template <class Prep>
+__device__ __forceinline__ Prep load_prep(const double *w) {
+    struct Raw {
+        double v[sizeof(Prep) / sizeof(double)];
+    } r;
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(sizeof(Prep) / sizeof(double)); ++k) r.v[k] = w[k];
+    return uniform(__builtin_bit_cast(Prep, r));
+}
+
+// f(segment as a compile-time constant, set 0 deriver, set 1 deriver, scale 0, scale 1) with
+// the layer's derivers of the segment from the table
+template <class F>
+__device__ __forceinline__ void with_prep(const AlphaArgs &A, int layer, int seg, F &&f) {
+    const double *row = A.far_prep + (static_cast<size_t>(layer) * kSegs + seg) * kPrepWords;
+    const double s0 = __builtin_amdgcn_readfirstlane(row[16]), s1 = __builtin_amdgcn_readfirstlane(row[17]);
+    switch (seg) {
+        case 0: {
+            const auto p = load_prep<H2oPrep<double>>(row);
+            f(SegC<0>{}, p, p, s0, s1);
+            break;
+        }
+        case 1: {
+            const auto p = load_prep<H2sPrep<double>>(row);
+            f(SegC<1>{}, p, p, s0, s1);
+            break;
+        }
+        case 2: {
+            const auto p0 = load_prep<InvPrep<double>>(row), p1 = load_prep<InvPrep<double>>(row + 8);
+            f(SegC<2>{}, p0, p1, s0, s1);
+            break;
+        }
+        case 3: {
+            const auto p = load_prep<RotPrep<double>>(row);
+            f(SegC<3>{}, p, p, s0, s1);
+            break;
+        }
+        case 4: {
+            const auto p = load_prep<V2Prep<double>>(row);
+            f(SegC<4>{}, p, p, s0, s1);
+            break;
+        }
+        default: {
+            const auto p = load_prep<Ph3Prep<double>>(row);
+            f(SegC<5>{}, p, p, s0, s1);
             break;
         }
     }
@@ -317,7 +387,7 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) 
         const int seg_off = off;
         off += plan.nr[seg];
         if (a0 >= b0) continue;
-        with_prep(A, L, seg, [&](auto segc, const auto &p0, const auto &p1, double s0, double s1) {
+        with_prep(A, layer, seg, [&](auto segc, const auto &p0, const auto &p1, double s0, double s1) {
             constexpr int SEG = decltype(segc)::value;
             constexpr bool inv = SEG == 2;
             const DevBlock &B = A.cat.blk[kSegKind[SEG]];
@@ -764,15 +834,13 @@ __global__ __launch_bounds__(kThreads, K >= 4 ? 2 : 4) void alpha_near_kernel(Al
             // while deriving (held in registers across the line sums they would crowd out the
             // sums' own)
             __syncthreads();   // (the previous segment's derivations are done)
-            if (tid == 0)
-                with_prep(A, L, seg, [&](auto, const auto &p0, const auto &p1, double s0, double s1) {
-                    using Prep = std::decay_t<decltype(p0)>;
-                    static_assert(sizeof(Prep) <= sizeof(sh.prep[0]), "coefficient set larger than its LDS slot");
-                    *reinterpret_cast<Prep *>(sh.prep[0]) = p0;
-                    *reinterpret_cast<Prep *>(sh.prep[1]) = p1;
-                    sh.scale[0] = F32 ? s0 : 1.0;
-                    sh.scale[1] = F32 ? s1 : 1.0;
-                });
+            if (tid < kPrepWords) {
+                const double w = A.far_prep[(static_cast<size_t>(layer) * kSegs + seg) * kPrepWords + tid];
+                if (tid < 16)
+                    sh.prep[tid / 8][tid % 8] = w;
+                else
+                    sh.scale[tid - 16] = w;
+            }
             __syncthreads();
             const bool inv = seg == 2;
             const int var = kSegVar[seg];
