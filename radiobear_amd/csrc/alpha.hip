@@ -453,18 +453,35 @@ __device__ __forceinline__ void sum_staged_f32(const ST &st, const int (&set)[K]
     }
 }
 
+// 1 / x to within an ulp or two: the hardware estimate refined by two Newton steps
+__device__ inline double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// FP64 sums: two lines over one denominator, n1/e1 + n2/e2 = (n1 e2 + n2 e1) / (e1 e2), and
+// the reciprocal as the hardware estimate refined twice (a few ulps, where the IEEE division
+// took twice the instructions); the denominators stay within 1e-17..1e27, so their products
+// are far from FP64's range limits.  An odd last line pairs with a zero term.
 template <int K, bool UNI, class ST = Staged64>
 __device__ __forceinline__ void sum_staged_f64(const ST &st, const int (&set)[K], int jb, int je,
                                                const double (&x)[K], double (&acc)[K]) {
-    for (int j = jb; j < je; ++j) {
+    for (int j = jb; j < je; j += 2) {
+        const bool two = j + 1 < je;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int m = UNI ? set[0] : set[k];
-            const double D = st.D[m][j], a = st.a[m][j], c = st.c[m][j], g = st.g[m][j], G = st.G[m][j];
-            const double d = x[k] - D;
-            const double num = fma(a, x[k], c);
-            const double den = fma(d, d, fma(g, x[k], G));
-            acc[k] += num / den;
+            const double d1 = x[k] - st.D[m][j];
+            const double n1 = fma(st.a[m][j], x[k], st.c[m][j]);
+            const double e1 = fma(d1, d1, fma(st.g[m][j], x[k], st.G[m][j]));
+            double n2 = 0.0, e2 = 1.0;
+            if (two) {
+                const double d2 = x[k] - st.D[m][j + 1];
+                n2 = fma(st.a[m][j + 1], x[k], st.c[m][j + 1]);
+                e2 = fma(d2, d2, fma(st.g[m][j + 1], x[k], st.G[m][j + 1]));
+            }
+            acc[k] += fma(n1, e2, n2 * e1) * recip(e1 * e2);
         }
     }
 }

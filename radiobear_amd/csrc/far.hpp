@@ -116,13 +116,6 @@ __device__ __forceinline__ T uniform(const T &v) {
     return out;
 }
 
-// 1 / x to within an ulp or two: the hardware estimate refined by two Newton steps
-__device__ inline double recip(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return fma(r, fma(-x, r, 1.0), r);
-}
-
 // The segments' coefficient derivers per layer (the layer's powers of T and partial pressures,
 // computed once by alpha_prep_kernel): [nl][kSegs][kPrepWords] doubles, set 0's deriver in
 // words 0-7, set 1's (the same one outside the nh3_hs inversion lines) in 8-15, and their FP32
