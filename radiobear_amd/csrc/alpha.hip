@@ -1064,7 +1064,7 @@ struct TanLines<true> {   // shifted denominator (x - Ds)^2 + h, as the forward 
     alignas(16) double D[2][kWaveLines + kSetPad];       // Ds
     alignas(16) double h[2][kWaveLines + kSetPad];
     alignas(16) float2 ac[2][kWaveLines + kSetPad];      // a s, c s
-    alignas(16) float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s), d(c s), dh, dDs
+    alignas(16) float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s), d(c s), dh, -2 dDs
 };
 template <>
 struct TanLines<false> {
@@ -1122,7 +1122,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                         const int d = sl[m];
                         st.dq[s][m][lane] =
                             make_float4(static_cast<float>(q.a.d[d] * sc), static_cast<float>(q.c.d[d] * sc),
-                                        static_cast<float>(h.d[d]), static_cast<float>(Ds.d[d]));
+                                        static_cast<float>(h.d[d]), static_cast<float>(-2.0 * Ds.d[d]));
                     }
                 } else {
                     const Dual *c5[5] = {&q.D, &q.a, &q.c, &q.g, &q.G};
@@ -1150,7 +1150,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                     const float4 AC2 = sAC[j / 2];
                     const double dd = x - ((u & 1) ? D2.y : D2.x);
                     const float Q = static_cast<float>(fma(dd, dd, (u & 1) ? H2.y : H2.x));
-                    const float m2d = -2.0f * static_cast<float>(dd);
+                    const float ddf = static_cast<float>(dd);
                     const float2 ac = (u & 1) ? make_float2(AC2.z, AC2.w) : make_float2(AC2.x, AC2.y);
                     const float N = fmaf(ac.x, xf, ac.y);
                     const float r = __builtin_amdgcn_rcpf(Q);
@@ -1160,7 +1160,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                     for (int m = 0; m < 2; ++m) {
                         const float4 dq = st.dq[mine][m][j];
                         const float dN = fmaf(dq.x, xf, dq.y);
-                        const float dQ = fmaf(m2d, dq.w, dq.z);   // d[(x - Ds)^2 + h]
+                        const float dQ = fmaf(ddf, dq.w, dq.z);   // d[(x - Ds)^2 + h], dq.w = -2 dDs
                         dp[m] = fmaf(fmaf(-t, dQ, dN), r, dp[m]);
                     }
                 }
