@@ -208,8 +208,11 @@ def autotune(prob, prec, args):
         la, rec_s = at.tune(split, key, lambda L: prob.launch_alpha(prec, L), la, path=args.tune_cache)
         li, rec_i = at.tune_all(('int_unroll', 'int_map'), key, lambda L: prob.launch_integrate(prec, L),
                                 path=args.tune_cache)
-        # the integration in request order or in cut order (order + reordered table included)
+        # the integration in request order or in cut order (order + reordered table included),
+        # on this step's own table and paths (a tune cache skips the trial launches above)
         prob.launch[prec] = dict(la, **li)
+        prob.launch_paths()
+        prob.launch_alpha(prec)
         order_ms = {}
         for on in (False, True):
             prob.launch_integrate(prec, ordered=on)
