@@ -105,6 +105,32 @@ def test_brightness_single_profiles(jplanet, jupiter_lines):
     assert br.single([0.9, 0.9], freqs, atm, al) == [2.725, 2.725]
 
 
+def test_brightness_batch_cut_order(jplanet, monkeypatch):
+    """Brightness.batch integrates large batches with the frequencies in the order they pass the
+    optical-depth cut (kernels.integrate(ordered=True)); forced on a small batch, Tb agrees with
+    the request order to within what the cut itself changes (2 T_max e^-cut)."""
+    import torch
+    from radiobear_amd import brightness, kernels
+    from radiobear_amd.alpha import Alpha
+    atm = jplanet.atmos[0]
+    freqs = list(np.geomspace(0.6, 90.0, 70))
+    al = Alpha(config=jplanet.config, verbose=False)
+    al.get_layers(freqs, atm)
+    rng = np.random.default_rng(8)
+    r, th = np.sqrt(rng.uniform(0.0, 0.95, 96)), rng.uniform(0.0, 2 * np.pi, 96)
+    bs = [[float(a), float(b)] for a, b in zip(r * np.cos(th), r * np.sin(th))]
+    br = brightness.Brightness(config=jplanet.config, verbose=False)
+    monkeypatch.setattr(brightness, 'ORDER_MIN_RAY_FREQS', 1 << 60)
+    plain = br.batch(bs, freqs, atm, al)
+    monkeypatch.setattr(brightness, 'ORDER_MIN_RAY_FREQS', 1)
+    ordered = br.batch(bs, freqs, atm, al)
+    cut = kernels.FP64_TAU_CUT if al.table.dtype == torch.float64 else kernels.FP32_TAU_CUT
+    Tmax = float(np.max(atm.gas[C['T']]))
+    ok = np.isfinite(plain)
+    assert np.array_equal(ok, np.isfinite(ordered)) and ok.any()
+    assert np.max(np.abs(ordered[ok] - plain[ok])) < 1e-9 + 2.0 * Tmax * np.exp(-cut)
+
+
 def test_raypath_compute_ds(jplanet):
     from radiobear_amd import raypath
     atm = jplanet.atmos[0]
