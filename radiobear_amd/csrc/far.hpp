@@ -20,6 +20,8 @@
 // the same 16 terms at ratio 0.15 (at most ~8e-12 of a far line's value) and sums its near
 // lines in FP64 from their (D, a, c, g, G), as the direct FP64 sums do.
 //
+// alpha_prep_kernel (one thread per layer and segment) first writes every layer's
+// coefficient derivers to the workspace; both passes below read them from there.
 // alpha_far_kernel (block = layer x group of kFarTiles tiles x part of the layer's
 // staging rounds; each wavefront a contiguous run of 64-line blocks): each lane derives
 // one line's coefficients per coefficient set (once per line and layer, its catalogue
