@@ -302,7 +302,9 @@ def test_workload_full_shape_sampled(K, raw_lines, jupiter_lines, name):
     production optical-depth cuts (FP32 20, FP64 40), checked on a sample the oracle finishes
     quickly: 3 layers of alpha at every frequency (both precisions, from the FP32 and FP64
     tables the bench times), and 4 rays integrated over those tables with the oracle's
-    geometry.  Also the built-in launch shapes, FP32 table within 1e-6 of the tuned one."""
+    geometry, in request order and in cut order (the bench's); the two orders within the cut's
+    own bound on every ray.  Also the built-in launch shapes, FP32 table within 1e-6 of the
+    tuned one."""
     import torch
     w = synth.workload(name)
     atm = w['atm']
@@ -331,15 +333,27 @@ def test_workload_full_shape_sampled(K, raw_lines, jupiter_lines, name):
     hds, ns = K.raypaths(pl, atm.R, atm.N, rays)
     Tb = K.integrate(a64, atm.T, hds, ns, precision='fp64', tau_cut=K.FP64_TAU_CUT, launch=L).cpu().numpy()
     Tb32 = K.integrate(a32, atm.T, hds, ns, precision='fp32', tau_cut=K.FP32_TAU_CUT, launch=L).cpu().numpy()
+    # the bench integrates with the frequencies in the order they pass the cut
+    Tbo = K.integrate(a64, atm.T, hds, ns, precision='fp64', tau_cut=K.FP64_TAU_CUT, launch=L,
+                      ordered=True).cpu().numpy()
+    Tb32o = K.integrate(a32, atm.T, hds, ns, precision='fp32', tau_cut=K.FP32_TAU_CUT, launch=L,
+                        ordered=True).cpu().numpy()
     tab = a64.cpu().numpy().T                                            # (nf, nl)
     nsn = ns.cpu().numpy()
     on = np.flatnonzero(nsn > 0)
+    Tmax = float(np.max(atm.T))
+    for got, want, cut in ((Tbo, Tb, K.FP64_TAU_CUT), (Tb32o, Tb32, K.FP32_TAU_CUT)):
+        fin = np.isfinite(want[on])
+        assert np.array_equal(fin, np.isfinite(got[on]))
+        assert np.max(np.abs(got[on][fin] - want[on][fin])) < 1e-9 + 2.0 * Tmax * np.exp(-cut)
     picks = [on[0], on[len(on) // 3], on[-1], int(np.flatnonzero(nsn < 0)[0])]
     for i in picks:
         r = geometry.compute_ds(atm.R, atm.N, rays[i], atm.Req, atm.Rpol)
         tb = np.array(rt.single(tab, atm.T, r.ds, r.layer4ds))
         assert np.max(np.abs(Tb[i] - tb)) < 1e-6, i
         assert np.max(np.abs(Tb32[i] - tb)) < 0.01, i
+        assert np.max(np.abs(Tbo[i] - tb)) < 1e-6, i
+        assert np.max(np.abs(Tb32o[i] - tb)) < 0.01, i
     del torch
 
 
