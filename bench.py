@@ -634,6 +634,24 @@ def main():
             # the HBM side of the same kernel: measured bytes over its live duration, vs 8 TB/s
             roof['hbm_GBps'] = roof['traffic'] / (kt32[dom] * 1e-3) / 1e9
             roof['hbm_frac'] = roof['hbm_GBps'] / PEAK['hbm']
+    # the integration's side: its HBM rate against 8 TB/s (it is bound by VALU issue; its
+    # compulsory bytes -- the FP32 table, the ds rows, T, Tb -- would move in tens of
+    # microseconds), and its rate in FLOP of the uncut paths
+    steps_all = float(np.sum(np.maximum(np.minimum(ns, prob.nl - 1) - 1, 0)))
+    roof_int = {'kernel': 'rbx_integrate', 'bound': 'valu', 'ms': kt32['rbx_integrate'],
+                'algorithmic_bytes_per_launch': float(prob.nl * prob.nf * 4 + prob.nray * prob.ld * 8 + prob.nl * 8
+                                                      + prob.nray * prob.nf * 8),
+                'algorithmic_flop_per_launch': steps_all * prob.nf * FLOP_RT_STEP,
+                'basis': '{} FLOP per ray x frequency x step of the uncut paths; the tau cut skips '
+                         'the steps past it'.format(FLOP_RT_STEP)}
+    roof_int['achieved_TFLOPs_in_uncut_unit'] = roof_int['algorithmic_flop_per_launch'] / (kt32['rbx_integrate'] * 1e-3) / 1e12
+    if os.path.exists(tr_path):
+        with open(tr_path) as fp:
+            tri = json.load(fp)['bytes_per_launch'].get('rbx_integrate')
+        if tri:
+            roof_int['traffic'] = tri['read'] + tri['write']
+            roof_int['hbm_GBps'] = roof_int['traffic'] / (kt32['rbx_integrate'] * 1e-3) / 1e9
+            roof_int['hbm_frac'] = roof_int['hbm_GBps'] / PEAK['hbm']
     cpu = cpu_baseline(prob, cpu_meas) if cpu_meas is not None else None
     value = prob.on_planet * prob.nf / dt32
     line = {
@@ -649,7 +667,8 @@ def main():
                                 'steps and tau prefix sum; FP32 weights and trapezoid sums added into FP64 '
                                 'every batch of steps; tau cut at {:g} (FP64 mode {:g})'.format(
                                     prob.K.FP32_TAU_CUT, prob.K.FP64_TAU_CUT)},
-        'kernel_ms': kt32, 'launch_shapes': tune32, 'roofline': roof, 'cpu_baseline': cpu,
+        'kernel_ms': kt32, 'launch_shapes': tune32, 'roofline': roof, 'roofline_integrate': roof_int,
+        'cpu_baseline': cpu,
     }
     line.update(res)
     print(json.dumps(line), flush=True)
