@@ -130,3 +130,17 @@ def test_order_rejects_bad_arguments(K):
     assert L.rbx_integrate_order(*args, 30.0, None, K._ptr(a_cols), K._stream()) == -1
     big = [K._ptr(tab), _lib.FP64, 1, K.MAX_ORDER_FREQS + 1, K._ptr(hds), hds.shape[1], K._ptr(ns), 3]
     assert L.rbx_integrate_order(*big, 30.0, K._ptr(cols), K._ptr(a_cols), K._stream()) == -1
+
+
+def test_order_without_on_planet_rays(K):
+    """No ray on the planet: every column's cut step is 0, so the order is the request order,
+    and the ordered integration returns what the plain one does (T_cmb rows)."""
+    import torch
+    tab, T, hds, ns, *_ = case(40, 130, 5, 3, torch.float32)
+    ns.fill_(-1)
+    cols, a_cols = order(K, tab, hds, ns, 20.0)
+    assert np.array_equal(cols, np.arange(130))
+    assert torch.equal(a_cols, tab)
+    plain = K.integrate(tab, T, hds, ns, precision='fp32', tau_cut=20.0).cpu().numpy()
+    ordered = K.integrate(tab, T, hds, ns, precision='fp32', tau_cut=20.0, ordered=True).cpu().numpy()
+    assert np.array_equal(plain, ordered) and np.all(plain == 2.725)
