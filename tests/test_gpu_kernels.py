@@ -77,7 +77,7 @@ def test_alpha_unsorted_frequencies(K, jcat, bench_atm, jupiter_lines):
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
 def test_alpha_synthetic_c1(K, raw_lines, jupiter_lines, precision, tol):
-    """Workload 1: 150 layers, 2000 synthetic NH3 inversion lines, 20 freqs."""
+    """Workload 1: 150 layers, 2000 synthetic NH3 inversion lines, 20 freqs, one nadir ray."""
     w = synth.workload('c1_plain_fp64')
     atm = w['atm']
     cat = K.jupiter_catalog(raw_lines, nh3_inversion=w['nh3_inv'])
@@ -90,6 +90,16 @@ def test_alpha_synthetic_c1(K, raw_lines, jupiter_lines, precision, tol):
     out = K.alpha(cat, K.absorbers_struct(JUPITER_ALPHA), atm.P, atm.T, K.mix_table(atm.gas, C), w['freqs'],
                   precision)
     assert rel(out.double().cpu().numpy().T, ref) < tol
+    # its nadir ray: Tb against the oracle's geometry and integration over the oracle's table
+    # (FP64 with the FP64 cut at 40; FP32 with its cut at 20, the 0.01 K bar)
+    pl = K.planet_desc(atm.Req, atm.Rpol)
+    hds, ns = K.raypaths(pl, atm.R, atm.N, w['rays'])
+    cut = K.FP64_TAU_CUT if precision == 'fp64' else K.FP32_TAU_CUT
+    Tb = K.integrate(out, atm.T, hds, ns, precision=precision, tau_cut=cut).cpu().numpy()
+    for i, b in enumerate(w['rays']):
+        r = geometry.compute_ds(atm.R, atm.N, b, atm.Req, atm.Rpol)
+        tb = np.array(rt.single(ref, atm.T, r.ds, r.layer4ds))
+        assert np.max(np.abs(Tb[i] - tb)) < (1e-6 if precision == 'fp64' else 0.01), i
 
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-9), ('fp32', 1e-5)])
