@@ -126,6 +126,7 @@ struct AlphaArgs {
     double *far_prep;   // the segments' coefficient derivers per layer [nl][kSegs][kPrepWords]
     int far_words, far_split;
     int far_off[kFarStreams];   // first mask word of each stream
+    double *tan_prep;   // split tangent pass: the segments' dual-number derivers per layer
     // tangent pass split into tan_parts parts of a layer's tan_rounds 64-line rounds;
     // tan_seg0[s]: the first round of segment s (h2o, h2s, nh3 inversion, rot, v2, ph3)
     int tan_parts, tan_rounds;
@@ -1212,6 +1213,71 @@ constexpr int kTanComps = 3;   // value, d/dT, d/dq_own
 
 __device__ __host__ inline int tan_rounds(int n) { return (n + kWaveLines - 1) / kWaveLines; }
 
+// The layer's coefficient sets on dual numbers (value, d/dT, d/dq_NH3, d/dq_H2O), as the
+// tangent pass builds them from the layer state.
+__device__ inline LayerT<Dual> dual_layer(const AlphaArgs &A, int layer) {
+    const LayerT<double> L0 = layer_of(A, layer);
+    LayerT<Dual> L;
+    L.P = L0.P;
+    L.T = Dual(L0.T, 1.0, 0.0, 0.0);
+    L.h2 = L0.h2;
+    L.he = L0.he;
+    L.ch4 = L0.ch4;
+    L.nh3 = Dual(L0.nh3, 0.0, L0.P, 0.0);   // d(P q)/dq = P
+    L.h2o = Dual(L0.h2o, 0.0, 0.0, L0.P);
+    L.h2s = L0.h2s;
+    L.ph3 = L0.ph3;
+    return L;
+}
+
+// Split tangent pass: every part of a (layer, tile) would build the layer's dual-number
+// derivers again, so tangent_prep_kernel writes them once per layer, [nl][kTanPrepWords]
+// doubles, segment s at kTanPrepOff[s] (the nh3_hs inversion lines' two sets one after the
+// other), and the parts read them there.
+template <class P>
+constexpr int prep_words() {
+    static_assert(sizeof(P) % sizeof(double) == 0, "a deriver is whole doubles");
+    return static_cast<int>(sizeof(P) / sizeof(double));
+}
+constexpr int kTanPrepOff[7] = {
+    0,
+    prep_words<H2oPrep<Dual>>(),
+    prep_words<H2oPrep<Dual>>() + prep_words<H2sPrep<Dual>>(),
+    prep_words<H2oPrep<Dual>>() + prep_words<H2sPrep<Dual>>() + 2 * prep_words<InvPrep<Dual>>(),
+    prep_words<H2oPrep<Dual>>() + prep_words<H2sPrep<Dual>>() + 2 * prep_words<InvPrep<Dual>>() +
+        prep_words<RotPrep<Dual>>(),
+    prep_words<H2oPrep<Dual>>() + prep_words<H2sPrep<Dual>>() + 2 * prep_words<InvPrep<Dual>>() +
+        prep_words<RotPrep<Dual>>() + prep_words<V2Prep<Dual>>(),
+    prep_words<H2oPrep<Dual>>() + prep_words<H2sPrep<Dual>>() + 2 * prep_words<InvPrep<Dual>>() +
+        prep_words<RotPrep<Dual>>() + prep_words<V2Prep<Dual>>() + prep_words<Ph3Prep<Dual>>()};
+constexpr int kTanPrepWords = kTanPrepOff[6];
+static_assert(sizeof(Dual) % sizeof(double) == 0, "dual numbers are whole doubles");
+
+template <class P>
+__device__ inline void put_prep(double *row, const P &p) {
+    const double *w = reinterpret_cast<const double *>(&p);
+    for (int k = 0; k < prep_words<P>(); ++k) row[k] = w[k];
+}
+
+__global__ __launch_bounds__(256) void tangent_prep_kernel(AlphaArgs A) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.nl * kTanSegs) return;
+    const int layer = e / kTanSegs, seg = e % kTanSegs;
+    const LayerT<Dual> L = dual_layer(A, layer);
+    double *row = A.tan_prep + static_cast<size_t>(layer) * kTanPrepWords + kTanPrepOff[seg];
+    switch (seg) {
+        case 0: put_prep(row, H2oPrep<Dual>(L)); break;
+        case 1: put_prep(row, H2sPrep<Dual>(L)); break;
+        case 2:
+            put_prep(row, InvPrep<Dual>(L, nh3_low_set(A.abs.nh3)));
+            put_prep(row + prep_words<InvPrep<Dual>>(), InvPrep<Dual>(L, 1));
+            break;
+        case 3: put_prep(row, RotPrep<Dual>(L)); break;
+        case 4: put_prep(row, V2Prep<Dual>(L)); break;
+        default: put_prep(row, Ph3Prep<Dual>(L)); break;
+    }
+}
+
 template <bool F32, int MODE>
 __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan<F32> &sh) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
@@ -1233,16 +1299,9 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
 
     const LayerT<double> L0 = layer_of(A, layer);
     const int nl = A.nl;
-    LayerT<Dual> L;
-    L.P = L0.P;
-    L.T = Dual(L0.T, 1.0, 0.0, 0.0);
-    L.h2 = L0.h2;
-    L.he = L0.he;
-    L.ch4 = L0.ch4;
-    L.nh3 = Dual(L0.nh3, 0.0, L0.P, 0.0);   // d(P q)/dq = P
-    L.h2o = Dual(L0.h2o, 0.0, 0.0, L0.P);
-    L.h2s = L0.h2s;
-    L.ph3 = L0.ph3;
+    const LayerT<Dual> L = dual_layer(A, layer);
+    // the split pass reads the layer's derivers from its table (typed views of its doubles)
+    const double *trow = MODE == kTanPart ? A.tan_prep + static_cast<size_t>(layer) * kTanPrepWords : nullptr;
     const double f[1] = {A.freq[cc]};
     const double fn = A.freq_nh3 ? A.freq_nh3[cc] : f[0];
     const double x = f[0] * f[0], xn = fn * fn;
@@ -1318,8 +1377,13 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
                 base.d[2] += ws_base[(static_cast<size_t>(p) * kTanComps + 2) * nl + layer];
             }
         } else {
-            const H2oPrep<Dual> prep[1] = {H2oPrep<Dual>(L)};
-            lines(0, A.cat.blk[RBX_LINES_H2O], prep, 1, 2, x, v, base);
+            if (MODE == kTanPart) {
+                lines(0, A.cat.blk[RBX_LINES_H2O], reinterpret_cast<const H2oPrep<Dual> *>(trow + kTanPrepOff[0]), 1,
+                      2, x, v, base);
+            } else {
+                const H2oPrep<Dual> prep[1] = {H2oPrep<Dual>(L)};
+                lines(0, A.cat.blk[RBX_LINES_H2O], prep, 1, 2, x, v, base);
+            }
             base = wave_sum(base);
         }
         if (MODE == kTanPart) {
@@ -1338,8 +1402,13 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         if (MODE == kTanFinish) {
             v = slot_in(kSlotH2s, 1);
         } else {
-            const H2sPrep<Dual> prep[1] = {H2sPrep<Dual>(L)};
-            lines(1, A.cat.blk[RBX_LINES_H2S], prep, 1, 1, x, v, unused);
+            if (MODE == kTanPart) {
+                lines(1, A.cat.blk[RBX_LINES_H2S], reinterpret_cast<const H2sPrep<Dual> *>(trow + kTanPrepOff[1]), 1,
+                      1, x, v, unused);
+            } else {
+                const H2sPrep<Dual> prep[1] = {H2sPrep<Dual>(L)};
+                lines(1, A.cat.blk[RBX_LINES_H2S], prep, 1, 1, x, v, unused);
+            }
         }
         if (MODE == kTanPart)
             slot_out(kSlotH2s, 1, v);
@@ -1353,17 +1422,26 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
             if (MODE == kTanFinish) {
                 hs = slot_in(kSlotNh3, 1);
             } else {
-                {
-                    const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, nh3_low_set(sel)), InvPrep<Dual>(L, 1)};
-                    lines(2, A.cat.blk[RBX_LINES_NH3_INV], prep, 2, 1, xn, hs, unused);
-                }
-                {
-                    const RotPrep<Dual> prep[1] = {RotPrep<Dual>(L)};
-                    lines(3, A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, 1, xn, hs, unused);
-                }
-                {
-                    const V2Prep<Dual> prep[1] = {V2Prep<Dual>(L)};
-                    lines(4, A.cat.blk[RBX_LINES_NH3_V2], prep, 1, 1, xn, hs, unused);
+                if (MODE == kTanPart) {
+                    lines(2, A.cat.blk[RBX_LINES_NH3_INV], reinterpret_cast<const InvPrep<Dual> *>(trow + kTanPrepOff[2]),
+                          2, 1, xn, hs, unused);
+                    lines(3, A.cat.blk[RBX_LINES_NH3_ROT], reinterpret_cast<const RotPrep<Dual> *>(trow + kTanPrepOff[3]),
+                          1, 1, xn, hs, unused);
+                    lines(4, A.cat.blk[RBX_LINES_NH3_V2], reinterpret_cast<const V2Prep<Dual> *>(trow + kTanPrepOff[4]),
+                          1, 1, xn, hs, unused);
+                } else {
+                    {
+                        const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, nh3_low_set(sel)), InvPrep<Dual>(L, 1)};
+                        lines(2, A.cat.blk[RBX_LINES_NH3_INV], prep, 2, 1, xn, hs, unused);
+                    }
+                    {
+                        const RotPrep<Dual> prep[1] = {RotPrep<Dual>(L)};
+                        lines(3, A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, 1, xn, hs, unused);
+                    }
+                    {
+                        const V2Prep<Dual> prep[1] = {V2Prep<Dual>(L)};
+                        lines(4, A.cat.blk[RBX_LINES_NH3_V2], prep, 1, 1, xn, hs, unused);
+                    }
                 }
             }
             if (MODE == kTanPart) slot_out(kSlotNh3, 1, hs);
@@ -1380,8 +1458,13 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         if (MODE == kTanFinish) {
             v = slot_in(kSlotPh3, 1);
         } else {
-            const Ph3Prep<Dual> prep[1] = {Ph3Prep<Dual>(L)};
-            lines(5, A.cat.blk[RBX_LINES_PH3], prep, 1, 1, x, v, unused);
+            if (MODE == kTanPart) {
+                lines(5, A.cat.blk[RBX_LINES_PH3], reinterpret_cast<const Ph3Prep<Dual> *>(trow + kTanPrepOff[5]), 1,
+                      1, x, v, unused);
+            } else {
+                const Ph3Prep<Dual> prep[1] = {Ph3Prep<Dual>(L)};
+                lines(5, A.cat.blk[RBX_LINES_PH3], prep, 1, 1, x, v, unused);
+            }
         }
         if (MODE == kTanPart)
             slot_out(kSlotPh3, 1, v);
@@ -1868,12 +1951,14 @@ static void plan_tangent(AlphaArgs &A, const rbx_launch &L) {
     A.tan_parts = std::max(1, std::min(parts, std::max(r, 1)));
 }
 
+static size_t tangent_ws_doubles_before_prep(const AlphaArgs &A) {
+    const size_t plane = static_cast<size_t>(A.nl) * A.nf;
+    return static_cast<size_t>(A.tan_parts) * kTanSlots * kTanComps * plane +
+           static_cast<size_t>(A.tan_parts) * kTanComps * A.nl;
+}
 static size_t tangent_ws_bytes(const AlphaArgs &A) {
     if (A.tan_parts <= 1) return 0;
-    const size_t plane = static_cast<size_t>(A.nl) * A.nf;
-    return (static_cast<size_t>(A.tan_parts) * kTanSlots * kTanComps * plane +
-            static_cast<size_t>(A.tan_parts) * kTanComps * A.nl) *
-           sizeof(double);
+    return (tangent_ws_doubles_before_prep(A) + static_cast<size_t>(A.nl) * kTanPrepWords) * sizeof(double);
 }
 
 static int tangent_args(AlphaArgs &A, rbx_launch &L, const rbx_catalog *cat, const rbx_absorbers *absorbers,
@@ -1936,6 +2021,7 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
                       workspace_bytes, need);
         RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha_tangent: workspace not 8-byte aligned");
         A.ws = static_cast<double *>(workspace);
+        A.tan_prep = A.ws + rbx::tangent_ws_doubles_before_prep(A);
     }
     const long long items = static_cast<long long>(nl) * ((nf + 63) / 64);
     const long long blocks = (items + rbx::kThreads / 64 - 1) / (rbx::kThreads / 64);
@@ -1950,6 +2036,10 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
             hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanAll>, g1, dim3(rbx::kThreads), 0, st, A);
         return rbx::launch_status("alpha_tangent_kernel");
     }
+    hipLaunchKernelGGL(rbx::tangent_prep_kernel, dim3(static_cast<unsigned>((static_cast<long long>(nl) * rbx::kTanSegs + 255) / 256)),
+                       dim3(256), 0, st, A);
+    rc = rbx::launch_status("tangent_prep_kernel");
+    if (rc) return rc;
     if (f32)
         hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanPart>, gp, dim3(rbx::kThreads), 0, st, A);
     else
