@@ -277,9 +277,10 @@ int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *absorbers,
                       const rbx_launch *launch, void *workspace, size_t workspace_bytes,
                       void *stream);
 
-/* Workspace bytes rbx_alpha_tangent needs for this shape and launch (host call; 0 when
- * each layer's lines are summed in one part, the built-in choice once layers x 64-column
- * tiles fill the device; with P parts 96 P bytes per layer and frequency). */
+/* Workspace bytes rbx_alpha_tangent needs for this shape and launch (host call): each
+ * layer's dual-number coefficient derivers, 1168 bytes per layer, and, when each layer's lines
+ * are summed in P > 1 parts (the built-in choice until layers x 64-column tiles fill the
+ * device), 96 P bytes per layer and frequency more (plus 24 P per layer). */
 int rbx_alpha_tangent_workspace_size(const rbx_catalog *cat, const rbx_absorbers *absorbers, int nl, int nf,
                                      int nf_nh3_low, int precision, const rbx_launch *launch, size_t *bytes);
 

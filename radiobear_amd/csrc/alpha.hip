@@ -126,7 +126,7 @@ struct AlphaArgs {
     double *far_prep;   // the segments' coefficient derivers per layer [nl][kSegs][kPrepWords]
     int far_words, far_split;
     int far_off[kFarStreams];   // first mask word of each stream
-    double *tan_prep;   // split tangent pass: the segments' dual-number derivers per layer
+    double *tan_prep;   // tangent pass: the segments' dual-number derivers per layer
     // tangent pass split into tan_parts parts of a layer's tan_rounds 64-line rounds;
     // tan_seg0[s]: the first round of segment s (h2o, h2s, nh3 inversion, rot, v2, ph3)
     int tan_parts, tan_rounds;
@@ -1230,10 +1230,10 @@ __device__ inline LayerT<Dual> dual_layer(const AlphaArgs &A, int layer) {
     return L;
 }
 
-// Split tangent pass: every part of a (layer, tile) would build the layer's dual-number
-// derivers again, so tangent_prep_kernel writes them once per layer, [nl][kTanPrepWords]
-// doubles, segment s at kTanPrepOff[s] (the nh3_hs inversion lines' two sets one after the
-// other), and the parts read them there.
+// Every (layer, tile) wavefront of the tangent pass, and every part of one when it is split,
+// would build the layer's dual-number derivers again, so tangent_prep_kernel writes them once
+// per layer, [nl][kTanPrepWords] doubles, segment s at kTanPrepOff[s] (the nh3_hs inversion
+// lines' two sets one after the other), and the wavefronts read them there.
 template <class P>
 constexpr int prep_words() {
     static_assert(sizeof(P) % sizeof(double) == 0, "a deriver is whole doubles");
@@ -1300,8 +1300,8 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
     const LayerT<double> L0 = layer_of(A, layer);
     const int nl = A.nl;
     const LayerT<Dual> L = dual_layer(A, layer);
-    // the split pass reads the layer's derivers from its table (typed views of its doubles)
-    const double *trow = MODE == kTanPart ? A.tan_prep + static_cast<size_t>(layer) * kTanPrepWords : nullptr;
+    // the layer's derivers from their table (typed views of its doubles)
+    const double *trow = MODE != kTanFinish ? A.tan_prep + static_cast<size_t>(layer) * kTanPrepWords : nullptr;
     const double f[1] = {A.freq[cc]};
     const double fn = A.freq_nh3 ? A.freq_nh3[cc] : f[0];
     const double x = f[0] * f[0], xn = fn * fn;
@@ -1377,7 +1377,7 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
                 base.d[2] += ws_base[(static_cast<size_t>(p) * kTanComps + 2) * nl + layer];
             }
         } else {
-            if (MODE == kTanPart) {
+            if (MODE != kTanFinish) {
                 lines(0, A.cat.blk[RBX_LINES_H2O], reinterpret_cast<const H2oPrep<Dual> *>(trow + kTanPrepOff[0]), 1,
                       2, x, v, base);
             } else {
@@ -1402,7 +1402,7 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         if (MODE == kTanFinish) {
             v = slot_in(kSlotH2s, 1);
         } else {
-            if (MODE == kTanPart) {
+            if (MODE != kTanFinish) {
                 lines(1, A.cat.blk[RBX_LINES_H2S], reinterpret_cast<const H2sPrep<Dual> *>(trow + kTanPrepOff[1]), 1,
                       1, x, v, unused);
             } else {
@@ -1422,7 +1422,7 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
             if (MODE == kTanFinish) {
                 hs = slot_in(kSlotNh3, 1);
             } else {
-                if (MODE == kTanPart) {
+                if (MODE != kTanFinish) {
                     lines(2, A.cat.blk[RBX_LINES_NH3_INV], reinterpret_cast<const InvPrep<Dual> *>(trow + kTanPrepOff[2]),
                           2, 1, xn, hs, unused);
                     lines(3, A.cat.blk[RBX_LINES_NH3_ROT], reinterpret_cast<const RotPrep<Dual> *>(trow + kTanPrepOff[3]),
@@ -1458,7 +1458,7 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         if (MODE == kTanFinish) {
             v = slot_in(kSlotPh3, 1);
         } else {
-            if (MODE == kTanPart) {
+            if (MODE != kTanFinish) {
                 lines(5, A.cat.blk[RBX_LINES_PH3], reinterpret_cast<const Ph3Prep<Dual> *>(trow + kTanPrepOff[5]), 1,
                       1, x, v, unused);
             } else {
@@ -1956,9 +1956,10 @@ static size_t tangent_ws_doubles_before_prep(const AlphaArgs &A) {
     return static_cast<size_t>(A.tan_parts) * kTanSlots * kTanComps * plane +
            static_cast<size_t>(A.tan_parts) * kTanComps * A.nl;
 }
+// partial sums of a split pass (none unsplit), then the derivers' table
 static size_t tangent_ws_bytes(const AlphaArgs &A) {
-    if (A.tan_parts <= 1) return 0;
-    return (tangent_ws_doubles_before_prep(A) + static_cast<size_t>(A.nl) * kTanPrepWords) * sizeof(double);
+    const size_t parts = A.tan_parts > 1 ? tangent_ws_doubles_before_prep(A) : 0;
+    return (parts + static_cast<size_t>(A.nl) * kTanPrepWords) * sizeof(double);
 }
 
 static int tangent_args(AlphaArgs &A, rbx_launch &L, const rbx_catalog *cat, const rbx_absorbers *absorbers,
@@ -2014,21 +2015,23 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
     const int parts = A.tan_parts;
     A.value = alpha;
     const size_t need = rbx::tangent_ws_bytes(A);
-    if (need > 0) {
-        RBX_CHECK_ARG(workspace && workspace_bytes >= need,
-                      "rbx_alpha_tangent: workspace of %zu bytes, this call needs %zu "
-                      "(rbx_alpha_tangent_workspace_size)",
-                      workspace_bytes, need);
-        RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha_tangent: workspace not 8-byte aligned");
-        A.ws = static_cast<double *>(workspace);
-        A.tan_prep = A.ws + rbx::tangent_ws_doubles_before_prep(A);
-    }
+    RBX_CHECK_ARG(workspace && workspace_bytes >= need,
+                  "rbx_alpha_tangent: workspace of %zu bytes, this call needs %zu "
+                  "(rbx_alpha_tangent_workspace_size)",
+                  workspace_bytes, need);
+    RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha_tangent: workspace not 8-byte aligned");
+    A.ws = static_cast<double *>(workspace);
+    A.tan_prep = A.ws + (parts > 1 ? rbx::tangent_ws_doubles_before_prep(A) : 0);
     const long long items = static_cast<long long>(nl) * ((nf + 63) / 64);
     const long long blocks = (items + rbx::kThreads / 64 - 1) / (rbx::kThreads / 64);
     RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_alpha_tangent: too many layers x frequencies");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const dim3 g1(static_cast<unsigned>(blocks)), gp(static_cast<unsigned>(blocks), static_cast<unsigned>(parts));
     const bool f32 = precision == RBX_FP32;
+    hipLaunchKernelGGL(rbx::tangent_prep_kernel, dim3(static_cast<unsigned>((static_cast<long long>(nl) * rbx::kTanSegs + 255) / 256)),
+                       dim3(256), 0, st, A);
+    rc = rbx::launch_status("tangent_prep_kernel");
+    if (rc) return rc;
     if (parts <= 1) {
         if (f32)
             hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanAll>, g1, dim3(rbx::kThreads), 0, st, A);
@@ -2036,10 +2039,6 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
             hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanAll>, g1, dim3(rbx::kThreads), 0, st, A);
         return rbx::launch_status("alpha_tangent_kernel");
     }
-    hipLaunchKernelGGL(rbx::tangent_prep_kernel, dim3(static_cast<unsigned>((static_cast<long long>(nl) * rbx::kTanSegs + 255) / 256)),
-                       dim3(256), 0, st, A);
-    rc = rbx::launch_status("tangent_prep_kernel");
-    if (rc) return rc;
     if (f32)
         hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanPart>, gp, dim3(rbx::kThreads), 0, st, A);
     else

@@ -59,8 +59,15 @@ def test_argument_errors_are_reported_without_gpu_work():
     rc = lib.rbx_alpha_tangent(h, ctypes.byref(cl), dummy, dummy, dummy, 3, dummy, None, 2, 0, None, 0, dummy,
                                dummy, None, None, 0, None)
     assert rc == -1 and 'no tangent' in _lib.last_error()
-    # workspace: partial sums [4 slots][parts][nl][nf] + h2o base [parts][nl] doubles
+    # tangent workspace: the derivers' table, 146 doubles per layer, plus with P > 1 parts the
+    # partial sums [P][4 slots][3][nl][nf] and h2o base terms [P][3][nl]
     nb = ctypes.c_size_t(0)
+    h2t = kernels.absorbers_struct({'h2': 'h2_jj_ddb'})
+    one = _lib.launch_struct({'tan_split': 1})
+    assert lib.rbx_alpha_tangent_workspace_size(h, ctypes.byref(h2t), 10, 7, 0, 1, _lib.launch_ref(one),
+                                                ctypes.byref(nb)) == 0
+    assert nb.value == 146 * 10 * 8
+    # workspace: partial sums [4 slots][parts][nl][nf] + h2o base [parts][nl] doubles
     h2 = kernels.absorbers_struct({'h2': 'h2_jj_ddb'})
     assert lib.rbx_alpha_workspace_size(h, ctypes.byref(h2), 10, 7, 0, 1, None, ctypes.byref(nb)) == 0
     assert nb.value == (4 * 10 * 7 + 10) * 8                      # no lines: one part
