@@ -1073,34 +1073,43 @@ struct TanLines<false> {
     double d[2][2][5][kWaveLines + kSetPad];   // [set][slot]
 };
 
-// each wavefront's own staging buffer (wavefronts of a block run independently)
+// each wavefront's staging buffer: its share of the lines the block's wavefronts sum
 template <bool F32>
 union WaveTan {
     TanLines<F32> lines;
     SjsLines<64> sjs;
 };
 
-template <bool F32>
+// W wavefronts: the frequency tiles of one layer that share the lines' derivation
+template <bool F32, int W>
 struct SharedTan {
-    WaveTan<F32> w[kThreads / 64];
+    WaveTan<F32> w[W];
     // each wavefront's coefficient sets of the block being staged (uniform across the
     // wavefront: kept in LDS, not registers, while the staged lines are summed)
-    alignas(16) double prep[kThreads / 64][64];
+    alignas(16) double prep[W][64];
+    double base[W][4];   // the wavefronts' shares of the h2o base term (value, 3 tangents)
 };
 
 // value + (T, q_own) tangent sums of one line block for this lane's column.
 // prep[0..1]: coefficient sets (two only for the nh3_hs inversion lines);
-// `need` = sets this wavefront's lanes use, `mine` = this lane's set;
+// `need` = sets any lane of the block uses, `mine` = this lane's set;
 // scale[s] = FP32 power-of-two weight scale of set s (1 in FP64 mode).
+// The block's nw wavefronts (frequency tiles of one layer) take the lines in rounds of
+// nw x 64: wavefront wi derives and stages the round's lines [wi 64, wi 64 + 64) into its
+// buffer bufs[wi], and after a block barrier every wavefront sums all nw buffers, in line
+// order, for its own columns -- each line is derived once per block rather than once per
+// tile.  Every wavefront of the block must make the same calls (same lb, le).
 template <bool F32, class Prep>
 __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int mine, const double *scale,
-                             int qslot, double x, double &acc, double (&dacc)[2], TanLines<F32> &st, Dual &base,
-                             int lb, int le) {
+                             int qslot, double x, double &acc, double (&dacc)[2], WaveTan<F32> *bufs, int nw,
+                             int wi, Dual &base, int lb, int le) {
     const int lane = threadIdx.x % 64;
     const float xf = static_cast<float>(x);
-    for (int l0 = lb; l0 < le; l0 += kWaveLines) {
-        const int i = l0 + lane;
-        const int cnt = min(kWaveLines, le - l0);
+    for (int l0 = lb; l0 < le; l0 += nw * kWaveLines) {
+        TanLines<F32> &st = bufs[wi].lines;
+        const int b0 = l0 + wi * kWaveLines;
+        const int i = b0 + lane;
+        const int cnt = max(0, min(kWaveLines, le - b0));
         const int cntp = (cnt + kSubT - 1) / kSubT * kSubT;
         if (lane < cntp) {
             CoefT<Dual> qs[2] = {{Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)},
@@ -1136,12 +1145,16 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                 }
             }
         }
-        wave_sync();
+        __syncthreads();
+        for (int w = 0; w < nw; ++w) {
+        const TanLines<F32> &sw = bufs[w].lines;
+        const int cw = max(0, min(kWaveLines, le - (l0 + w * kWaveLines)));
+        const int cwp = (cw + kSubT - 1) / kSubT * kSubT;
         if constexpr (F32) {
-            const double2 *sD = reinterpret_cast<const double2 *>(st.D[mine]);
-            const double2 *sH = reinterpret_cast<const double2 *>(st.h[mine]);
-            const float4 *sAC = reinterpret_cast<const float4 *>(st.ac[mine]);
-            for (int j0 = 0; j0 < cntp; j0 += kSubT) {
+            const double2 *sD = reinterpret_cast<const double2 *>(sw.D[mine]);
+            const double2 *sH = reinterpret_cast<const double2 *>(sw.h[mine]);
+            const float4 *sAC = reinterpret_cast<const float4 *>(sw.ac[mine]);
+            for (int j0 = 0; j0 < cwp; j0 += kSubT) {
                 float p = 0.0f, dp[2] = {0.0f, 0.0f};
 #pragma unroll
                 for (int u = 0; u < kSubT; ++u) {
@@ -1159,7 +1172,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                     p += t;
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
-                        const float4 dq = st.dq[mine][m][j];
+                        const float4 dq = sw.dq[mine][m][j];
                         const float dN = fmaf(dq.x, xf, dq.y);
                         const float dQ = fmaf(ddf, dq.w, dq.z);   // d[(x - Ds)^2 + h], dq.w = -2 dDs
                         dp[m] = fmaf(fmaf(-t, dQ, dN), r, dp[m]);
@@ -1170,9 +1183,9 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                 dacc[1] += static_cast<double>(dp[1]);
             }
         } else {
-            for (int j = 0; j < cntp; ++j) {
-                const double D = st.v[mine][0][j], a = st.v[mine][1][j], c = st.v[mine][2][j];
-                const double g = st.v[mine][3][j], G = st.v[mine][4][j];
+            for (int j = 0; j < cwp; ++j) {
+                const double D = sw.v[mine][0][j], a = sw.v[mine][1][j], c = sw.v[mine][2][j];
+                const double g = sw.v[mine][3][j], G = sw.v[mine][4][j];
                 const double d = x - D;
                 const double N = fma(a, x, c);
                 const double r = 1.0 / fma(d, d, fma(g, x, G));
@@ -1180,7 +1193,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                 acc += t;
 #pragma unroll
                 for (int m = 0; m < 2; ++m) {
-                    const double *dd = &st.d[mine][m][0][0];
+                    const double *dd = &sw.d[mine][m][0][0];
                     const int S = kWaveLines + kSetPad;
                     const double dN = fma(dd[1 * S + j], x, dd[2 * S + j]);
                     const double dQ = fma(-2.0 * d, dd[j], fma(dd[3 * S + j], x, dd[4 * S + j]));
@@ -1188,7 +1201,8 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                 }
             }
         }
-        wave_sync();
+        }
+        __syncthreads();   // (the next round overwrites the buffers)
     }
 }
 
@@ -1278,20 +1292,35 @@ __global__ __launch_bounds__(256) void tangent_prep_kernel(AlphaArgs A) {
     }
 }
 
-template <bool F32, int MODE>
-__device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan<F32> &sh) {
+// Blocks: the line-summing modes (TAN_ALL, TAN_PART) take W consecutive 64-column tiles of one
+// layer per block (block = layer x group of W tiles, W = blockDim.x / 64; a wavefront past the
+// last tile derives its share of the lines and sums none), so that the block derives each line
+// once (line_sum_tan); block barriers follow, so the whole block returns together or not at
+// all.  TAN_FINISH takes any W (layer, tile) items per block, independent of each other.
+template <bool F32, int MODE, int W>
+__device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan<F32, W> &sh) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
-    const long long item = static_cast<long long>(blockIdx.x) * (kThreads / 64) + wave;
     const int tiles = (A.nf + 63) / 64;
-    if (item >= static_cast<long long>(A.nl) * tiles) return;   // the whole wavefront: no block barriers follow
-    const int layer = static_cast<int>(item / tiles);
-    const int tile = static_cast<int>(item % tiles);
+    int layer, tile, g = 0;
+    if constexpr (MODE == kTanFinish) {
+        const long long item = static_cast<long long>(blockIdx.x) * W + wave;
+        if (item >= static_cast<long long>(A.nl) * tiles) return;   // the whole wavefront: no block barriers
+        layer = static_cast<int>(item / tiles);
+        tile = static_cast<int>(item % tiles);
+    } else {
+        const int G = (tiles + W - 1) / W;                           // groups of W tiles per layer
+        layer = static_cast<int>(blockIdx.x / G);
+        g = static_cast<int>(blockIdx.x % G);
+        if (layer >= A.nl) return;                                   // the whole block
+        tile = g * W + wave;
+    }
     const int col = tile * 64 + lane;
     const bool ok = col < A.nf;
     const int cc = ok ? col : A.nf - 1;
     const int mine = cc < A.nf_low ? 0 : 1;                      // nh3_hs set of this lane's column
-    const int need = (__any(mine == 0) ? 1 : 0) | (__any(mine == 1) ? 2 : 0);
-    TanLines<F32> &st = sh.w[wave].lines;
+    // coefficient sets any column of the block's tiles uses (the block stages them for all)
+    const int gc0 = g * W * 64, gc1 = min((g + 1) * W * 64, A.nf);
+    const int need = MODE == kTanFinish ? 0 : (gc0 < A.nf_low ? 1 : 0) | (gc1 > A.nf_low ? 2 : 0);
     const int parts = A.tan_parts;
     const int part = MODE == kTanPart ? static_cast<int>(blockIdx.y) : 0;
     const int r0 = MODE == kTanPart ? part * A.tan_rounds / parts : 0;
@@ -1358,7 +1387,7 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         if (lane == 0)
             for (int k = 0; k < nset; ++k) lp[k] = prep[k];
         wave_sync();
-        line_sum_tan<F32>(B, lp, nd, me, s, qslot, xx, acc, dacc, st, base, lb, le);
+        line_sum_tan<F32>(B, lp, nd, me, s, qslot, xx, acc, dacc, sh.w, W, wave, base, lb, le);
         wave_sync();   // the next block's sets overwrite these
         Dual r(xx * acc / s[me]);
         r.d[0] = xx * dacc[0] / s[me];
@@ -1377,14 +1406,21 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
                 base.d[2] += ws_base[(static_cast<size_t>(p) * kTanComps + 2) * nl + layer];
             }
         } else {
-            if (MODE != kTanFinish) {
-                lines(0, A.cat.blk[RBX_LINES_H2O], reinterpret_cast<const H2oPrep<Dual> *>(trow + kTanPrepOff[0]), 1,
-                      2, x, v, base);
-            } else {
-                const H2oPrep<Dual> prep[1] = {H2oPrep<Dual>(L)};
-                lines(0, A.cat.blk[RBX_LINES_H2O], prep, 1, 2, x, v, base);
-            }
+            lines(0, A.cat.blk[RBX_LINES_H2O], reinterpret_cast<const H2oPrep<Dual> *>(trow + kTanPrepOff[0]), 1, 2, x,
+                  v, base);
+            // each wavefront derived its share of the lines: the block's base term, added in order
             base = wave_sum(base);
+            if (lane == 0) {
+                sh.base[wave][0] = base.v;
+                for (int m = 0; m < kNd; ++m) sh.base[wave][1 + m] = base.d[m];
+            }
+            __syncthreads();
+            base = Dual();
+            for (int w = 0; w < W; ++w) {
+                base.v += sh.base[w][0];
+                for (int m = 0; m < kNd; ++m) base.d[m] += sh.base[w][1 + m];
+            }
+            __syncthreads();
         }
         if (MODE == kTanPart) {
             slot_out(kSlotH2o, 2, v);
@@ -1402,13 +1438,8 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         if (MODE == kTanFinish) {
             v = slot_in(kSlotH2s, 1);
         } else {
-            if (MODE != kTanFinish) {
-                lines(1, A.cat.blk[RBX_LINES_H2S], reinterpret_cast<const H2sPrep<Dual> *>(trow + kTanPrepOff[1]), 1,
-                      1, x, v, unused);
-            } else {
-                const H2sPrep<Dual> prep[1] = {H2sPrep<Dual>(L)};
-                lines(1, A.cat.blk[RBX_LINES_H2S], prep, 1, 1, x, v, unused);
-            }
+            lines(1, A.cat.blk[RBX_LINES_H2S], reinterpret_cast<const H2sPrep<Dual> *>(trow + kTanPrepOff[1]), 1, 1, x,
+                  v, unused);
         }
         if (MODE == kTanPart)
             slot_out(kSlotH2s, 1, v);
@@ -1422,27 +1453,12 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
             if (MODE == kTanFinish) {
                 hs = slot_in(kSlotNh3, 1);
             } else {
-                if (MODE != kTanFinish) {
-                    lines(2, A.cat.blk[RBX_LINES_NH3_INV], reinterpret_cast<const InvPrep<Dual> *>(trow + kTanPrepOff[2]),
-                          2, 1, xn, hs, unused);
-                    lines(3, A.cat.blk[RBX_LINES_NH3_ROT], reinterpret_cast<const RotPrep<Dual> *>(trow + kTanPrepOff[3]),
-                          1, 1, xn, hs, unused);
-                    lines(4, A.cat.blk[RBX_LINES_NH3_V2], reinterpret_cast<const V2Prep<Dual> *>(trow + kTanPrepOff[4]),
-                          1, 1, xn, hs, unused);
-                } else {
-                    {
-                        const InvPrep<Dual> prep[2] = {InvPrep<Dual>(L, nh3_low_set(sel)), InvPrep<Dual>(L, 1)};
-                        lines(2, A.cat.blk[RBX_LINES_NH3_INV], prep, 2, 1, xn, hs, unused);
-                    }
-                    {
-                        const RotPrep<Dual> prep[1] = {RotPrep<Dual>(L)};
-                        lines(3, A.cat.blk[RBX_LINES_NH3_ROT], prep, 1, 1, xn, hs, unused);
-                    }
-                    {
-                        const V2Prep<Dual> prep[1] = {V2Prep<Dual>(L)};
-                        lines(4, A.cat.blk[RBX_LINES_NH3_V2], prep, 1, 1, xn, hs, unused);
-                    }
-                }
+                lines(2, A.cat.blk[RBX_LINES_NH3_INV], reinterpret_cast<const InvPrep<Dual> *>(trow + kTanPrepOff[2]), 2,
+                      1, xn, hs, unused);
+                lines(3, A.cat.blk[RBX_LINES_NH3_ROT], reinterpret_cast<const RotPrep<Dual> *>(trow + kTanPrepOff[3]), 1,
+                      1, xn, hs, unused);
+                lines(4, A.cat.blk[RBX_LINES_NH3_V2], reinterpret_cast<const V2Prep<Dual> *>(trow + kTanPrepOff[4]), 1,
+                      1, xn, hs, unused);
             }
             if (MODE == kTanPart) slot_out(kSlotNh3, 1, hs);
             if (hs.v < 0.0) hs = Dual(1.0E-8);   // nh3_hs.py:337-338
@@ -1458,13 +1474,8 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
         if (MODE == kTanFinish) {
             v = slot_in(kSlotPh3, 1);
         } else {
-            if (MODE != kTanFinish) {
-                lines(5, A.cat.blk[RBX_LINES_PH3], reinterpret_cast<const Ph3Prep<Dual> *>(trow + kTanPrepOff[5]), 1,
-                      1, x, v, unused);
-            } else {
-                const Ph3Prep<Dual> prep[1] = {Ph3Prep<Dual>(L)};
-                lines(5, A.cat.blk[RBX_LINES_PH3], prep, 1, 1, x, v, unused);
-            }
+            lines(5, A.cat.blk[RBX_LINES_PH3], reinterpret_cast<const Ph3Prep<Dual> *>(trow + kTanPrepOff[5]), 1, 1, x,
+                  v, unused);
         }
         if (MODE == kTanPart)
             slot_out(kSlotPh3, 1, v);
@@ -1481,16 +1492,16 @@ __device__ __forceinline__ void alpha_tangent_body(const AlphaArgs &A, SharedTan
 
 // FP32: ask for 3 wavefronts per SIMD (the staged-line buffers take 35 KB per
 // block); FP64's 65 KB of staging allows 2 blocks per CU whatever the registers.
-template <int MODE>
+template <int MODE, int W>
 __global__ __launch_bounds__(kThreads, 3) void alpha_tangent_kernel_f32(AlphaArgs A) {
-    __shared__ SharedTan<true> sh;
-    alpha_tangent_body<true, MODE>(A, sh);
+    __shared__ SharedTan<true, W> sh;
+    alpha_tangent_body<true, MODE, W>(A, sh);
 }
 
-template <int MODE>
+template <int MODE, int W>
 __global__ __launch_bounds__(kThreads) void alpha_tangent_kernel_f64(AlphaArgs A) {
-    __shared__ SharedTan<false> sh;
-    alpha_tangent_body<false, MODE>(A, sh);
+    __shared__ SharedTan<false, W> sh;
+    alpha_tangent_body<false, MODE, W>(A, sh);
 }
 
 // check that the catalogue holds what the selected absorbers read
@@ -2022,33 +2033,46 @@ extern "C" int rbx_alpha_tangent(const rbx_catalog *cat, const rbx_absorbers *ab
     RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "rbx_alpha_tangent: workspace not 8-byte aligned");
     A.ws = static_cast<double *>(workspace);
     A.tan_prep = A.ws + (parts > 1 ? rbx::tangent_ws_doubles_before_prep(A) : 0);
-    const long long items = static_cast<long long>(nl) * ((nf + 63) / 64);
-    const long long blocks = (items + rbx::kThreads / 64 - 1) / (rbx::kThreads / 64);
-    RBX_CHECK_ARG(blocks < (1LL << 31), "rbx_alpha_tangent: too many layers x frequencies");
+    // line-summing passes: a block per layer and group of W tiles (W = 1, 2 or 4), so the
+    // block derives each line once for its W tiles; the finish: 4 independent items a block
+    const int tiles = (nf + 63) / 64;
+    const int W = tiles >= 3 ? 4 : tiles;
+    const long long groups = static_cast<long long>(nl) * ((tiles + W - 1) / W);
+    const long long items = static_cast<long long>(nl) * tiles;
+    const long long fblocks = (items + rbx::kThreads / 64 - 1) / (rbx::kThreads / 64);
+    RBX_CHECK_ARG(groups < (1LL << 31) && fblocks < (1LL << 31), "rbx_alpha_tangent: too many layers x frequencies");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const dim3 g1(static_cast<unsigned>(blocks)), gp(static_cast<unsigned>(blocks), static_cast<unsigned>(parts));
+    const dim3 gs(static_cast<unsigned>(groups), static_cast<unsigned>(std::max(parts, 1))), bs(64 * W);
     const bool f32 = precision == RBX_FP32;
     hipLaunchKernelGGL(rbx::tangent_prep_kernel, dim3(static_cast<unsigned>((static_cast<long long>(nl) * rbx::kTanSegs + 255) / 256)),
                        dim3(256), 0, st, A);
     rc = rbx::launch_status("tangent_prep_kernel");
     if (rc) return rc;
+#define RBX_TAN_LAUNCH(MODE)                                                                                    \
+    do {                                                                                                       \
+        if (f32) {                                                                                             \
+            if (W == 4) hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f32<MODE, 4>), gs, bs, 0, st, A);        \
+            else if (W == 2) hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f32<MODE, 2>), gs, bs, 0, st, A);   \
+            else hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f32<MODE, 1>), gs, bs, 0, st, A);               \
+        } else {                                                                                               \
+            if (W == 4) hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f64<MODE, 4>), gs, bs, 0, st, A);        \
+            else if (W == 2) hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f64<MODE, 2>), gs, bs, 0, st, A);   \
+            else hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f64<MODE, 1>), gs, bs, 0, st, A);               \
+        }                                                                                                      \
+    } while (0)
     if (parts <= 1) {
-        if (f32)
-            hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanAll>, g1, dim3(rbx::kThreads), 0, st, A);
-        else
-            hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanAll>, g1, dim3(rbx::kThreads), 0, st, A);
+        RBX_TAN_LAUNCH(rbx::kTanAll);
         return rbx::launch_status("alpha_tangent_kernel");
     }
-    if (f32)
-        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanPart>, gp, dim3(rbx::kThreads), 0, st, A);
-    else
-        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanPart>, gp, dim3(rbx::kThreads), 0, st, A);
+    RBX_TAN_LAUNCH(rbx::kTanPart);
+#undef RBX_TAN_LAUNCH
     rc = rbx::launch_status("alpha_tangent_kernel (parts)");
     if (rc) return rc;
+    const dim3 g1(static_cast<unsigned>(fblocks));
     if (f32)
-        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f32<rbx::kTanFinish>, g1, dim3(rbx::kThreads), 0, st, A);
+        hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f32<rbx::kTanFinish, 4>), g1, dim3(rbx::kThreads), 0, st, A);
     else
-        hipLaunchKernelGGL(rbx::alpha_tangent_kernel_f64<rbx::kTanFinish>, g1, dim3(rbx::kThreads), 0, st, A);
+        hipLaunchKernelGGL((rbx::alpha_tangent_kernel_f64<rbx::kTanFinish, 4>), g1, dim3(rbx::kThreads), 0, st, A);
     return rbx::launch_status("alpha_tangent_kernel (finish)");
 }
 
