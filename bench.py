@@ -115,6 +115,7 @@ class Problem:
         self.cols = torch.empty((self.nf,), dtype=torch.int32, device=dev)
         self.alpha32_cols = torch.empty_like(self.alpha32)
         self.alpha64_cols = torch.empty_like(self.alpha64)
+        self.factors = torch.empty((self.nray, self.ld, 2), dtype=torch.float32, device=dev)   # FP32 point rays
 
     # the three launches of one step, on torch's current stream
     def launch_alpha(self, prec, launch=None):
@@ -138,7 +139,18 @@ class Problem:
                                            K._ptr(self.b), self.nray, 1, K._ptr(self.hds), self.ld,
                                            K._ptr(self.ns), K._stream()))
 
-    def launch_integrate(self, prec, launch=None, tau_cut=None, Tb=None, ordered=None):
+    def launch_factors(self):
+        """Each ray's trapezoid factors for the FP32 cut-order integration (paths and T only)."""
+        from radiobear_amd import _lib
+        K = self.K
+        _lib.check(_lib.lib().rbx_integrate_factors(K._ptr(self.T), self.nl, K._ptr(self.hds), self.ld,
+                                                    K._ptr(self.ns), self.nray, K._ptr(self.factors), K._stream()))
+
+    def uses_factors(self, prec):
+        from radiobear_amd import _lib
+        return prec == _lib.FP32 and self.ordered.get(prec, False)
+
+    def launch_integrate(self, prec, launch=None, tau_cut=None, Tb=None, ordered=None, factors_ready=False):
         from radiobear_amd import _lib
         K = self.K
         a = self.alpha32 if prec == _lib.FP32 else self.alpha64
@@ -154,10 +166,12 @@ class Problem:
             _lib.check(_lib.lib().rbx_integrate_order(K._ptr(a), prec, self.nl, self.nf, K._ptr(self.hds), self.ld,
                                                       K._ptr(self.ns), self.nray, tau_cut, K._ptr(self.cols),
                                                       K._ptr(ac), K._stream()))
+            if prec == _lib.FP32 and not factors_ready:
+                self.launch_factors()
             _lib.check(_lib.lib().rbx_integrate_cols(K._ptr(ac), prec, K._ptr(self.cols), K._ptr(self.T), self.nl,
                                                      self.nf, K._ptr(self.hds), self.ld, K._ptr(self.ns), self.nray,
-                                                     0, prec, tau_cut, K._ptr(Tb), None, _lib.launch_ref(ls),
-                                                     K._stream()))
+                                                     0, prec, tau_cut, K._ptr(Tb), None, K._ptr(self.factors),
+                                                     _lib.launch_ref(ls), K._stream()))
             return
         _lib.check(_lib.lib().rbx_integrate(K._ptr(a), prec, K._ptr(self.T), self.nl, self.nf, K._ptr(self.hds),
                                             self.ld, K._ptr(self.ns), self.nray, 0, prec, tau_cut, K._ptr(Tb), None,
@@ -169,18 +183,22 @@ class Problem:
 
     def step(self, prec):
         """Absorption and ray paths are independent: the paths (a few hundred
-        wavefronts, latency-bound) run on a side stream beside the line sums, and
-        the integration waits for both."""
+        wavefronts, latency-bound) and, for the FP32 cut-order integration, each ray's
+        trapezoid factors run on a side stream beside the line sums, and the
+        integration waits for both."""
         import torch
         main = torch.cuda.current_stream()
         if self.side is None:
             self.side = torch.cuda.Stream()
+        fac = self.uses_factors(prec)
         self.side.wait_stream(main)
         with torch.cuda.stream(self.side):
             self.launch_paths()
+            if fac:
+                self.launch_factors()
         self.launch_alpha(prec)
         main.wait_stream(self.side)
-        self.launch_integrate(prec)
+        self.launch_integrate(prec, factors_ready=fac)
 
     def alpha_flop(self):
         n = self.nlines
@@ -273,17 +291,21 @@ def timed_graph(prob, prec, steps, warmup):
 def kernel_times(prob, prec, steps):
     """Average duration of each launch, HIP events on the launch stream."""
     import torch
-    names = ('rbx_alpha', 'rbx_raypaths', 'rbx_integrate')
+    fac = prob.uses_factors(prec)
+    names = ('rbx_alpha', 'rbx_raypaths') + (('rbx_integrate_factors',) if fac else ()) + ('rbx_integrate',)
     acc = {n: 0.0 for n in names}
     for _ in range(steps):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
         ev[0].record()
         prob.launch_alpha(prec)
         ev[1].record()
         prob.launch_paths()
         ev[2].record()
-        prob.launch_integrate(prec)
-        ev[3].record()
+        if fac:
+            prob.launch_factors()
+            ev[3].record()
+        prob.launch_integrate(prec, factors_ready=fac)
+        ev[-1].record()
         torch.cuda.synchronize()
         for i, n in enumerate(names):
             acc[n] += ev[i].elapsed_time(ev[i + 1])

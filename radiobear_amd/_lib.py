@@ -7,7 +7,7 @@ import ctypes
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'librbx.so')
-ABI_VERSION = 9
+ABI_VERSION = 10
 # rows of rbx_atmosphere_props (RBX_PROP_*: the reference's property-table order)
 PROP_ROWS = ('Z', 'R', 'P', 'GM', 'AMU', 'REFR', 'N', 'H', 'LAPSE', 'LAPSEP', 'g')
 
@@ -40,7 +40,7 @@ LAUNCH_FIELDS = ('alpha_k', 'alpha_series', 'alpha_split', 'far_split', 'int_unr
 
 EXPORTS = ('rbx_abi_version', 'rbx_last_error', 'rbx_device_count', 'rbx_catalog_create',
            'rbx_catalog_set_lines', 'rbx_catalog_lines', 'rbx_catalog_destroy', 'rbx_alpha',
-           'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_order', 'rbx_integrate_cols', 'rbx_integrate_profiles',
+           'rbx_alpha_workspace_size', 'rbx_alpha_tangent', 'rbx_raypaths', 'rbx_integrate', 'rbx_integrate_order', 'rbx_integrate_cols', 'rbx_integrate_factors', 'rbx_integrate_profiles',
            'rbx_jacobian', 'rbx_raypaths_batch', 'rbx_integrate_batch', 'rbx_jacobian_batch',
            'rbx_atmosphere_batch', 'rbx_atmosphere_props', 'rbx_alpha_rescale', 'rbx_alpha_noalloc', 'rbx_integrate_adjoint',
            'rbx_integrate_adjoint_batch', 'rbx_alpha_vjp', 'rbx_alpha_tangent_workspace_size',
@@ -118,7 +118,8 @@ _SIGS = {
     'rbx_raypaths': (_I, [ctypes.POINTER(PlanetDesc), _VP, _VP, _I, _VP, _I, _I, _VP, _I, _VP, _VP]),
     'rbx_integrate': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _LP, _VP]),
     'rbx_integrate_order': (_I, [_VP, _I, _I, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP]),
-    'rbx_integrate_cols': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _LP, _VP]),
+    'rbx_integrate_cols': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _D, _VP, _VP, _VP, _LP, _VP]),
+    'rbx_integrate_factors': (_I, [_VP, _I, _VP, _I, _VP, _I, _VP, _VP]),
     'rbx_integrate_profiles': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
     'rbx_jacobian': (_I, [_VP, _I, _VP, _VP, _I, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP]),
     'rbx_integrate_adjoint': (_I, [_VP, _I, _VP, _I, _I, _VP, _I, _VP, _I, _VP, _I, _D, _VP, _VP, _VP, _LP,

@@ -44,6 +44,9 @@ struct IntArgs {
     // column j of the table is request frequency cols[j]: Tb and intW go to column cols[j] of
     // their rows (null: column j; rbx_integrate_cols)
     const int *cols;
+    // FP32 point rays: each ray's trapezoid factors [nray][ld] float2, written by
+    // integrate_factors_kernel (null: each wavefront stages them in LDS)
+    const float2 *factors;
 };
 
 constexpr int kXcds = 8;   // MI355X: blocks are dealt round-robin to 8 XCDs (b % 8)
@@ -226,6 +229,38 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
     if (A.intW) A.intW[o] = (S < 0) ? 0.0 : intW;
 }
 
+// point k + 1's trapezoid factors of a path of n steps: (h_k + h_{k+1}, T_{k+1} (h_k + h_{k+1}))
+// rounded to FP32, with h_n = 0 (brightness.py:99-101 regrouped by path point)
+__device__ __forceinline__ float2 trapezoid_factor(const double *h, const double *Tl, int k, int n) {
+    const double hh = h[k] + (k + 1 < n ? h[k + 1] : 0.0);
+    return make_float2(static_cast<float>(hh), static_cast<float>(Tl[k + 1] * hh));
+}
+
+// Every frequency group of a ray stages the same factors; this writes them once per ray,
+// fac[ray][k] for k < n, and at k = n (a slot every path leaves free) a NaN when the path
+// holds a NaN half step or temperature (the integration then takes no cut) -- one wavefront
+// per ray.  They depend on the paths and T only, not on alpha, so a caller can write them
+// beside the absorption.
+__global__ __launch_bounds__(kWaves * kLanes) void integrate_factors_kernel(IntArgs A, float2 *fac) {
+    const int wave = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const int ray = ray_of(A, static_cast<int>(blockIdx.x) * kWaves + wave);
+    if (ray < 0) return;
+    int base, nlr;
+    const bool valid = ray_span(A, ray, base, nlr);
+    const int S = valid ? min(A.nsteps[ray], nlr - 1) : -1;
+    if (S <= 1) return;
+    const int n = S - 1;
+    const double *h = A.hds + static_cast<size_t>(ray) * A.ld, *Tl = A.T + base;
+    float2 *out = fac + static_cast<size_t>(ray) * A.ld;
+    bool bad = false;
+    for (int k = lane; k < n; k += kLanes) {
+        out[k] = trapezoid_factor(h, Tl, k, n);
+        bad = bad || __builtin_isnan(h[k]) || __builtin_isnan(Tl[k + 1]);
+    }
+    bad = __any(bad);
+    if (lane == 0) out[n] = make_float2(bad ? __builtin_nanf("") : 0.0f, 0.0f);
+}
+
 // ---- FP32 mode, point rays: FP32 weights and trapezoid sums, FP64 path and tau ----
 // The half steps and the tau prefix sum stay FP64: tau_{i+1} = tau_i + (a0 + a1) h_i in an
 // FP64 FMA at every step, with a0 + a1 (two FP32 table entries) added in FP32 and h_i the
@@ -247,9 +282,9 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_kernel(IntArgs A) {
 constexpr float kLog2ef = 1.44269504f;
 constexpr int kBufWord3 = 0x00020000;   // buffer descriptor word 3: 32-bit data format, no swizzle
 
-template <int U, bool BUF>
+template <int U, bool BUF, bool FAC>
 __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs A) {
-    extern __shared__ float2 hT_all[];   // [kWaves][ld]
+    extern __shared__ float2 hT_all[];   // [kWaves][ld] (none with FAC)
     const int wave = threadIdx.x / kLanes;
     const int lane = threadIdx.x % kLanes;
     int group, rblock;
@@ -266,7 +301,9 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
     const size_t nf = A.nf;
     const float *al = static_cast<const float *>(A.alpha) + static_cast<size_t>(base) * nf + jj;
     const double *Tl = A.T + base;
-    float2 *hT = hT_all + static_cast<size_t>(wave) * A.ld;
+    // the ray's factors: staged in this wavefront's LDS, or (FAC) read from the factor table
+    float2 *stage = hT_all + static_cast<size_t>(wave) * A.ld;
+    const float2 *hT = FAC ? A.factors + static_cast<size_t>(ray) * A.ld : stage;
     [[maybe_unused]] __amdgpu_buffer_rsrc_t rows;
     [[maybe_unused]] const int nfb = static_cast<int>(nf) * 4, lane_b = jj * 4;
     if constexpr (BUF)
@@ -283,14 +320,18 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
     if (S > 1) {
         const int n = S - 1;
         bool bad = false;   // a NaN half step or temperature: no cut (path_has_nan)
-        for (int k = lane; k < n; k += kLanes) {
-            const double hh = h[k] + (k + 1 < n ? h[k + 1] : 0.0);   // point k + 1's factor
-            hT[k] = make_float2(static_cast<float>(hh), static_cast<float>(Tl[k + 1] * hh));
-            bad = bad || __builtin_isnan(h[k]) || __builtin_isnan(Tl[k + 1]);
+        if constexpr (FAC) {
+            bad = __builtin_isnan(hT[n].x);   // the table marks such a path past its last factor
+        } else {
+            for (int k = lane; k < n; k += kLanes) {
+                stage[k] = trapezoid_factor(h, Tl, k, n);
+                bad = bad || __builtin_isnan(h[k]) || __builtin_isnan(Tl[k + 1]);
+            }
+            bad = __any(bad);
+            wave_sync();
         }
-        const double cut = (A.tau_cut > 0.0 && !__any(bad)) ? A.tau_cut : 0.0;
+        const double cut = (A.tau_cut > 0.0 && !bad) ? A.tau_cut : 0.0;
         const unsigned long long idle = __builtin_amdgcn_ballot_w64(!ok);   // lanes past the last column
-        wave_sync();
         float a0 = row(0), W1 = 0.0f;
         int i = 0;
         for (; i + U <= n; i += U) {
@@ -863,6 +904,7 @@ static void fill_int_args(IntArgs &A, const void *alpha, int alpha_precision, co
     A.natm = 0;
     A.order = nullptr;
     A.cols = nullptr;
+    A.factors = nullptr;
 }
 
 // ---- frequency order for the optical-depth cut ----
@@ -997,16 +1039,19 @@ static int launch_integrate(rbx::IntArgs A, const rbx_launch &L, void *stream) {
         if (A.f32) RBX_INT_LAUNCH(float, 1, false, 4); else RBX_INT_LAUNCH(double, 1, false, 4);
     } else if (A.f32) {
         const size_t lds = sizeof(float2) * rbx::kWaves * static_cast<size_t>(A.ld);
-        if (A.fast_exp && lds <= 64 * 1024) {
+        if (A.fast_exp && (A.factors || lds <= 64 * 1024)) {
             const bool buf = static_cast<size_t>(A.nl) * static_cast<size_t>(A.nf) * 4 < (size_t(1) << 31);
-            if (u4 && buf)
-                hipLaunchKernelGGL((rbx::integrate_f32_kernel<4, true>), grid, block, lds, st, A);
-            else if (u4)
-                hipLaunchKernelGGL((rbx::integrate_f32_kernel<4, false>), grid, block, lds, st, A);
-            else if (buf)
-                hipLaunchKernelGGL((rbx::integrate_f32_kernel<8, true>), grid, block, lds, st, A);
-            else
-                hipLaunchKernelGGL((rbx::integrate_f32_kernel<8, false>), grid, block, lds, st, A);
+            const size_t L = A.factors ? 0 : lds;
+#define RBX_F32_LAUNCH(U, B)                                                                                 \
+    do {                                                                                                    \
+        if (A.factors) hipLaunchKernelGGL((rbx::integrate_f32_kernel<U, B, true>), grid, block, L, st, A);  \
+        else hipLaunchKernelGGL((rbx::integrate_f32_kernel<U, B, false>), grid, block, L, st, A);           \
+    } while (0)
+            if (u4 && buf) RBX_F32_LAUNCH(4, true);
+            else if (u4) RBX_F32_LAUNCH(4, false);
+            else if (buf) RBX_F32_LAUNCH(8, true);
+            else RBX_F32_LAUNCH(8, false);
+#undef RBX_F32_LAUNCH
         } else if (A.fast_exp) { if (u4) RBX_INT_LAUNCH(float, 0, true, 4); else RBX_INT_LAUNCH(float, 0, true, 8); }
         else { if (u4) RBX_INT_LAUNCH(float, 0, false, 4); else RBX_INT_LAUNCH(float, 0, false, 8); }
     } else {
@@ -1072,8 +1117,8 @@ extern "C" int rbx_integrate_order(const void *alpha, int alpha_precision, int n
 
 extern "C" int rbx_integrate_cols(const void *alpha_cols, int alpha_precision, const int *cols, const double *T,
                                   int nl, int nf, const double *hds, int ld, const int *nsteps, int nray, int mode,
-                                  int precision, double tau_cut, double *Tb, double *intW, const rbx_launch *launch,
-                                  void *stream) {
+                                  int precision, double tau_cut, double *Tb, double *intW, const float *factors,
+                                  const rbx_launch *launch, void *stream) {
     int rc = rbx::check_int_args(alpha_cols, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
     if (rc) return rc;
     rbx_launch L;
@@ -1090,7 +1135,23 @@ extern "C" int rbx_integrate_cols(const void *alpha_cols, int alpha_precision, c
     A.Tb = Tb;
     A.intW = intW;
     A.cols = cols;
+    // FP32 point rays read each ray's trapezoid factors from rbx_integrate_factors' table
+    if (factors && mode == 0 && A.f32 && A.fast_exp) A.factors = reinterpret_cast<const float2 *>(factors);
     return launch_integrate(A, L, stream);
+}
+
+extern "C" int rbx_integrate_factors(const double *T, int nl, const double *hds, int ld, const int *nsteps,
+                                     int nray, float *factors, void *stream) {
+    RBX_CHECK_ARG(T && hds && nsteps && factors, "rbx_integrate_factors: null device pointer");
+    RBX_CHECK_ARG(nl >= 1 && nray >= 0, "rbx_integrate_factors: nl=%d nray=%d", nl, nray);
+    RBX_CHECK_ARG(ld >= nl - 1 && ld >= 1, "rbx_integrate_factors: ld=%d < nl-1=%d", ld, nl - 1);
+    if (nray == 0) return RBX_OK;
+    rbx::IntArgs A;
+    rbx::fill_int_args(A, nullptr, RBX_FP32, T, nl, 1, hds, ld, nsteps, nray, RBX_FP32, 0.0);
+    const unsigned blocks = static_cast<unsigned>((nray + rbx::kWaves - 1) / rbx::kWaves);
+    hipLaunchKernelGGL(rbx::integrate_factors_kernel, dim3(blocks), dim3(rbx::kWaves * rbx::kLanes), 0,
+                       static_cast<hipStream_t>(stream), A, reinterpret_cast<float2 *>(factors));
+    return rbx::launch_status("integrate_factors_kernel");
 }
 
 extern "C" int rbx_integrate_batch(const void *alpha, int alpha_precision, const double *T, int nl_total, int nf,

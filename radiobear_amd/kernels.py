@@ -376,7 +376,9 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
     """Tb (nray, nf) float64 from an alpha table (nl, nf) and ray paths.  ordered: integrate the
     frequencies in the order in which a reference ray passes tau_cut (rbx_integrate_order +
     rbx_integrate_cols; with a cut only): the wavefronts then stop together, and Tb differs
-    from the request order's only by what the cut itself changes."""
+    from the request order's only by what the cut itself changes.  FP32 point rays in that
+    order read each ray's trapezoid factors from a table written once per ray (8 nray ld
+    bytes) rather than staging them in LDS per frequency group."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
     prec = aprec if precision is None else _lib.PRECISIONS[precision]
     a = alpha_table.contiguous()
@@ -389,11 +391,16 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
     if ordered and tau_cut > 0.0 and nf <= MAX_ORDER_FREQS and nl <= MAX_ORDER_LAYERS and nray > 0:
         cols = torch.empty((nf,), dtype=torch.int32, device=device())
         a_cols = torch.empty_like(a)
+        fac = None
+        if not disc and aprec == _lib.FP32 and prec == _lib.FP32:
+            fac = torch.empty((nray, ld, 2), dtype=torch.float32, device=device())
         check(lib().rbx_integrate_order(_ptr(a), aprec, nl, nf, _ptr(hds), ld, _ptr(nsteps), nray, float(tau_cut),
                                         _ptr(cols), _ptr(a_cols), _stream()))
+        if fac is not None:
+            check(lib().rbx_integrate_factors(_ptr(T), nl, _ptr(hds), ld, _ptr(nsteps), nray, _ptr(fac), _stream()))
         check(lib().rbx_integrate_cols(_ptr(a_cols), aprec, _ptr(cols), _ptr(T), nl, nf, _ptr(hds), ld,
                                        _ptr(nsteps), nray, 1 if disc else 0, prec, float(tau_cut),
-                                       _ptr(Tb), _ptr(iw), _lib.launch_ref(launch), _stream()))
+                                       _ptr(Tb), _ptr(iw), _ptr(fac), _lib.launch_ref(launch), _stream()))
         return (Tb, iw) if with_intW else Tb
     check(lib().rbx_integrate(_ptr(a), aprec, _ptr(T), nl, nf, _ptr(hds), ld,
                               _ptr(nsteps), nray, 1 if disc else 0, prec, float(tau_cut),

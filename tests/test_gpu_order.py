@@ -87,15 +87,45 @@ def test_identity_order_is_bit_identical(K, precision, dtype_name, disc):
     prec = _lib.PRECISIONS[precision]
     Tgpu = K.as_device(T)
     ident = torch.arange(nf, dtype=torch.int32, device='cuda')
+    fac = torch.full((nray, ld, 2), np.nan, dtype=torch.float32, device='cuda')
+    _lib.check(_lib.lib().rbx_integrate_factors(K._ptr(Tgpu), nl, K._ptr(hds), ld, K._ptr(ns), nray, K._ptr(fac),
+                                                K._stream()))
     for cut in (0.0, 25.0):
         Tb0, iw0 = K.integrate(tab, T, hds, ns, disc=disc, precision=precision, tau_cut=cut, with_intW=True)
-        Tb1 = torch.full_like(Tb0, np.nan)
-        iw1 = torch.full_like(iw0, np.nan)
-        _lib.check(_lib.lib().rbx_integrate_cols(K._ptr(tab), aprec, K._ptr(ident), K._ptr(Tgpu), nl, nf,
-                                                 K._ptr(hds), ld, K._ptr(ns), nray, 1 if disc else 0, prec, cut,
-                                                 K._ptr(Tb1), K._ptr(iw1), None, K._stream()))
         on = ns.cpu().numpy() >= 0
-        assert torch.equal(Tb0[on], Tb1[on]) and torch.equal(iw0[on], iw1[on])
+        for factors in (None, fac):   # the factor table (FP32 point rays) or LDS staging: the same sums
+            Tb1 = torch.full_like(Tb0, np.nan)
+            iw1 = torch.full_like(iw0, np.nan)
+            _lib.check(_lib.lib().rbx_integrate_cols(K._ptr(tab), aprec, K._ptr(ident), K._ptr(Tgpu), nl, nf,
+                                                     K._ptr(hds), ld, K._ptr(ns), nray, 1 if disc else 0, prec, cut,
+                                                     K._ptr(Tb1), K._ptr(iw1), K._ptr(factors), None, K._stream()))
+            assert torch.equal(Tb0[on], Tb1[on]) and torch.equal(iw0[on], iw1[on])
+
+
+def test_factor_table_nan_path(K):
+    """A ray whose path holds a NaN half step keeps the NaN in Tb (no cut for it, as the staged
+    sums do), while the other rays in the factor table integrate as usual; bit-identical to
+    rbx_integrate."""
+    import torch
+    from radiobear_amd import _lib
+    tab, T, hds, ns, *_ = case(90, 150, 6, 8, torch.float32)
+    hds[2, 5] = float('nan')
+    nl, nf = tab.shape
+    nray, ld = hds.shape
+    ident = torch.arange(nf, dtype=torch.int32, device='cuda')
+    fac = torch.full((nray, ld, 2), 7.0, dtype=torch.float32, device='cuda')
+    Tgpu = K.as_device(T)
+    _lib.check(_lib.lib().rbx_integrate_factors(K._ptr(Tgpu), nl, K._ptr(hds), ld, K._ptr(ns), nray, K._ptr(fac),
+                                                K._stream()))
+    Tb0 = K.integrate(tab, T, hds, ns, precision='fp32', tau_cut=15.0)
+    Tb1 = torch.full_like(Tb0, -1.0)
+    _lib.check(_lib.lib().rbx_integrate_cols(K._ptr(tab), _lib.FP32, K._ptr(ident), K._ptr(Tgpu), nl, nf,
+                                             K._ptr(hds), ld, K._ptr(ns), nray, 0, _lib.FP32, 15.0, K._ptr(Tb1),
+                                             None, K._ptr(fac), None, K._stream()))
+    on = ns.cpu().numpy() >= 0
+    assert np.array_equal(Tb0[on].cpu().numpy(), Tb1[on].cpu().numpy(), equal_nan=True)
+    assert on[2] and ns[2].item() > 7 and torch.isnan(Tb1[2]).all()
+    assert not torch.isnan(Tb1[on][[r != 2 for r in np.flatnonzero(on)]]).any()
 
 
 @pytest.mark.parametrize('precision,dtype_name', [('fp64', 'float64'), ('fp32', 'float32')])
@@ -144,3 +174,36 @@ def test_order_without_on_planet_rays(K):
     plain = K.integrate(tab, T, hds, ns, precision='fp32', tau_cut=20.0).cpu().numpy()
     ordered = K.integrate(tab, T, hds, ns, precision='fp32', tau_cut=20.0, ordered=True).cpu().numpy()
     assert np.array_equal(plain, ordered) and np.all(plain == 2.725)
+
+
+def test_factor_table_values(K):
+    """rbx_integrate_factors: (h_k + h_{k+1}, T_{k+1} (h_k + h_{k+1})) in FP32 for k < n = nsteps - 1
+    (h_n = 0), a NaN flag at slot n only for a path holding a NaN, rows past n untouched, nothing
+    written for rays off the planet."""
+    import torch
+    from radiobear_amd import _lib
+    rng = np.random.default_rng(4)
+    nl, nray = 70, 5
+    ld = nl + 3
+    T = rng.uniform(100.0, 600.0, nl)
+    hds = rng.uniform(1.0e5, 9.0e5, (nray, ld))
+    ns = np.array([nl - 1, 40, -1, 2, nl - 1], dtype=np.int32)
+    hds[4, 17] = np.nan
+    fac = torch.full((nray, ld, 2), 5.0, dtype=torch.float32, device='cuda')
+    Tg, hg, ng = K.as_device(T), torch.as_tensor(hds, device='cuda'), torch.as_tensor(ns, device='cuda')
+    _lib.check(_lib.lib().rbx_integrate_factors(K._ptr(Tg), nl, K._ptr(hg), ld, K._ptr(ng), nray, K._ptr(fac),
+                                                K._stream()))
+    got = fac.cpu().numpy()
+    for r in range(nray):
+        S = min(int(ns[r]), nl - 1)
+        if S <= 1:
+            assert np.all(got[r] == 5.0)
+            continue
+        n = S - 1
+        h = hds[r, :n]
+        hh = h + np.append(hds[r, 1:n], 0.0)
+        want = np.stack([hh.astype(np.float32), (T[1:n + 1] * hh).astype(np.float32)], axis=1)
+        assert np.array_equal(got[r, :n], want, equal_nan=True)
+        assert np.isnan(got[r, n, 0]) == (r == 4) and got[r, n, 1] == 0.0
+        assert np.all(got[r, n + 1:] == 5.0)
+    assert np.isnan(got[4, 16:18]).any()
