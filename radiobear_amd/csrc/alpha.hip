@@ -1052,8 +1052,9 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
 //   t = N/Q, N = a x + c, Q = (x - D)^2 + g x + G,
 //   dt = (dN - t dQ)/Q, dN = da x + dc, dQ = -2 (x - D) dD + dg x + dG.
 // FP32 mode stores the staged values in FP32 (FP64 D), scales a, c and their
-// tangents by the layer's power of two, forms x - D in FP64 and sums blocks of
-// kSubT lines in FP32 before accumulating in FP64; FP64 mode is plain FP64.
+// tangents by the layer's power of two, forms x - D in FP64 and the denominator
+// from it in FP32, and sums blocks of kSubT lines in FP32 before accumulating in
+// FP64; FP64 mode is plain FP64.
 constexpr int kWaveLines = 64;
 constexpr int kSetPad = 4;   // moves set 1 to other LDS banks than set 0
 constexpr int kSubT = 4;
@@ -1063,7 +1064,7 @@ struct TanLines;
 template <>
 struct TanLines<true> {   // shifted denominator (x - Ds)^2 + h, as the forward FP32 staging
     alignas(16) double D[2][kWaveLines + kSetPad];       // Ds
-    alignas(16) double h[2][kWaveLines + kSetPad];
+    alignas(16) float h[2][kWaveLines + kSetPad];
     alignas(16) float2 ac[2][kWaveLines + kSetPad];      // a s, c s
     alignas(16) float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s), d(c s), dh, -2 dDs
 };
@@ -1125,7 +1126,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                     Dual Ds, h;
                     shifted_denominator(q, Ds, h);
                     st.D[s][lane] = Ds.v;
-                    st.h[s][lane] = h.v;
+                    st.h[s][lane] = static_cast<float>(h.v);
                     st.ac[s][lane] = make_float2(static_cast<float>(q.a.v * sc), static_cast<float>(q.c.v * sc));
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
@@ -1152,19 +1153,22 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
         const int cwp = (cw + kSubT - 1) / kSubT * kSubT;
         if constexpr (F32) {
             const double2 *sD = reinterpret_cast<const double2 *>(sw.D[mine]);
-            const double2 *sH = reinterpret_cast<const double2 *>(sw.h[mine]);
+            const float4 *sH = reinterpret_cast<const float4 *>(sw.h[mine]);
             const float4 *sAC = reinterpret_cast<const float4 *>(sw.ac[mine]);
             for (int j0 = 0; j0 < cwp; j0 += kSubT) {
                 float p = 0.0f, dp[2] = {0.0f, 0.0f};
 #pragma unroll
                 for (int u = 0; u < kSubT; ++u) {
                     const int j = j0 + u;
-                    // lines j, j + 1 come as one 16-byte read of each array (kSubT is even)
-                    const double2 D2 = sD[j / 2], H2 = sH[j / 2];
+                    // lines j, j + 1 come as one 16-byte read of D and of ac, lines j0 .. j0 + 3
+                    // as one of h
+                    const double2 D2 = sD[j / 2];
+                    const float4 H4 = sH[j / 4];
                     const float4 AC2 = sAC[j / 2];
                     const double dd = x - ((u & 1) ? D2.y : D2.x);
-                    const float Q = static_cast<float>(fma(dd, dd, (u & 1) ? H2.y : H2.x));
                     const float ddf = static_cast<float>(dd);
+                    const float hq = (u & 3) == 0 ? H4.x : (u & 3) == 1 ? H4.y : (u & 3) == 2 ? H4.z : H4.w;
+                    const float Q = fmaf(ddf, ddf, hq);
                     const float2 ac = (u & 1) ? make_float2(AC2.z, AC2.w) : make_float2(AC2.x, AC2.y);
                     const float N = fmaf(ac.x, xf, ac.y);
                     const float r = __builtin_amdgcn_rcpf(Q);
