@@ -1052,21 +1052,30 @@ __global__ __launch_bounds__(kThreads) void alpha_finish_kernel(AlphaArgs A, int
 //   t = N/Q, N = a x + c, Q = (x - D)^2 + g x + G,
 //   dt = (dN - t dQ)/Q, dN = da x + dc, dQ = -2 (x - D) dD + dg x + dG.
 // FP32 mode stores the staged values in FP32 (FP64 D), scales a, c and their
-// tangents by the layer's power of two, forms x - D in FP64 and the denominator
-// from it in FP32, and sums blocks of kSubT lines in FP32 before accumulating in
-// FP64; FP64 mode is plain FP64.
+// tangents by the layer's power of two, forms x - D in FP64 and sums blocks of kSubT
+// lines in FP32 before accumulating in FP64; FP64 mode is plain FP64.  The denominator
+// Q = (x - Ds)^2 + h is formed from the FP32 rounding of x - Ds in one FP32 FMA where
+// that stays accurate, else in FP64 and rounded: the FP32 form's relative error is
+// about (A + 1) FP32 roundings, A = (2 (x - Ds)^2 + |h|) / Q.  A <= 2 when h >= 0; when
+// h < 0 and Ds < 0, A is largest at x = 0 over x = f^2 >= 0, A0 = (2 Ds^2 - h) / (Ds^2 + h),
+// and FP32 is taken when A0 <= kTanAmp (2e-6); any other line (Ben-Reuven lines far
+// wider than their centre, h2s_ddb's deep in the atmosphere) takes FP64, and so does
+// every line of a staged buffer holding one.
 constexpr int kWaveLines = 64;
 constexpr int kSetPad = 4;   // moves set 1 to other LDS banks than set 0
 constexpr int kSubT = 4;
+constexpr double kTanAmp = 32.0;
 
 template <bool F32>
 struct TanLines;
 template <>
 struct TanLines<true> {   // shifted denominator (x - Ds)^2 + h, as the forward FP32 staging
     alignas(16) double D[2][kWaveLines + kSetPad];       // Ds
+    alignas(16) double hd[2][kWaveLines + kSetPad];      // h (the FP64 denominator's)
     alignas(16) float h[2][kWaveLines + kSetPad];
     alignas(16) float2 ac[2][kWaveLines + kSetPad];      // a s, c s
     alignas(16) float4 dq[2][2][kWaveLines + kSetPad];   // [set][slot] d(a s), d(c s), dh, -2 dDs
+    unsigned long long exact[2];                          // [set] lines whose denominator needs FP64
 };
 template <>
 struct TanLines<false> {
@@ -1112,6 +1121,7 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
         const int i = b0 + lane;
         const int cnt = max(0, min(kWaveLines, le - b0));
         const int cntp = (cnt + kSubT - 1) / kSubT * kSubT;
+        [[maybe_unused]] bool exact[2] = {false, false};
         if (lane < cntp) {
             CoefT<Dual> qs[2] = {{Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)},
                                  {Dual(0.0), Dual(0.0), Dual(0.0), Dual(0.0), Dual(1.0)}};   // padding: t = 0
@@ -1126,7 +1136,10 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                     Dual Ds, h;
                     shifted_denominator(q, Ds, h);
                     st.D[s][lane] = Ds.v;
+                    st.hd[s][lane] = h.v;
                     st.h[s][lane] = static_cast<float>(h.v);
+                    exact[s] = !(h.v >= 0.0 || (Ds.v < 0.0 && 2.0 * Ds.v * Ds.v - h.v <=
+                                                                   kTanAmp * (Ds.v * Ds.v + h.v)));
                     st.ac[s][lane] = make_float2(static_cast<float>(q.a.v * sc), static_cast<float>(q.c.v * sc));
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
@@ -1146,6 +1159,13 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                 }
             }
         }
+        if constexpr (F32) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(exact[s]);
+                if (lane == 0) st.exact[s] = m;
+            }
+        }
         __syncthreads();
         for (int w = 0; w < nw; ++w) {
         const TanLines<F32> &sw = bufs[w].lines;
@@ -1153,9 +1173,15 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
         const int cwp = (cw + kSubT - 1) / kSubT * kSubT;
         if constexpr (F32) {
             const double2 *sD = reinterpret_cast<const double2 *>(sw.D[mine]);
+            const double2 *sHd = reinterpret_cast<const double2 *>(sw.hd[mine]);
             const float4 *sH = reinterpret_cast<const float4 *>(sw.h[mine]);
             const float4 *sAC = reinterpret_cast<const float4 *>(sw.ac[mine]);
-            for (int j0 = 0; j0 < cwp; j0 += kSubT) {
+            // the buffer holds a line of a set this block reads whose denominator needs FP64: then
+            // all of its lines take the FP64 denominator (a choice per buffer keeps the FP32 loop
+            // free of branches)
+            const bool exact_lines =
+                uniform(((need & 1) ? sw.exact[0] : 0ull) | ((need & 2) ? sw.exact[1] : 0ull)) != 0ull;
+            auto block = [&](int j0, auto fp64_denominator) {
                 float p = 0.0f, dp[2] = {0.0f, 0.0f};
 #pragma unroll
                 for (int u = 0; u < kSubT; ++u) {
@@ -1163,12 +1189,17 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                     // lines j, j + 1 come as one 16-byte read of D and of ac, lines j0 .. j0 + 3
                     // as one of h
                     const double2 D2 = sD[j / 2];
-                    const float4 H4 = sH[j / 4];
                     const float4 AC2 = sAC[j / 2];
                     const double dd = x - ((u & 1) ? D2.y : D2.x);
                     const float ddf = static_cast<float>(dd);
-                    const float hq = (u & 3) == 0 ? H4.x : (u & 3) == 1 ? H4.y : (u & 3) == 2 ? H4.z : H4.w;
-                    const float Q = fmaf(ddf, ddf, hq);
+                    float Q;
+                    if constexpr (decltype(fp64_denominator)::value) {
+                        const double2 H2 = sHd[j / 2];
+                        Q = static_cast<float>(fma(dd, dd, (u & 1) ? H2.y : H2.x));
+                    } else {
+                        const float4 H4 = sH[j / 4];
+                        Q = fmaf(ddf, ddf, (u & 3) == 0 ? H4.x : (u & 3) == 1 ? H4.y : (u & 3) == 2 ? H4.z : H4.w);
+                    }
                     const float2 ac = (u & 1) ? make_float2(AC2.z, AC2.w) : make_float2(AC2.x, AC2.y);
                     const float N = fmaf(ac.x, xf, ac.y);
                     const float r = __builtin_amdgcn_rcpf(Q);
@@ -1185,7 +1216,11 @@ __device__ void line_sum_tan(const DevBlock &B, const Prep *prep, int need, int 
                 acc += static_cast<double>(p);
                 dacc[0] += static_cast<double>(dp[0]);
                 dacc[1] += static_cast<double>(dp[1]);
-            }
+            };
+            if (exact_lines)
+                for (int j0 = 0; j0 < cwp; j0 += kSubT) block(j0, std::true_type{});
+            else
+                for (int j0 = 0; j0 < cwp; j0 += kSubT) block(j0, std::false_type{});
         } else {
             for (int j = 0; j < cwp; ++j) {
                 const double D = sw.v[mine][0][j], a = sw.v[mine][1][j], c = sw.v[mine][2][j];

@@ -81,6 +81,41 @@ def test_fused_alpha_value_and_fp32_tangents(K, small_batch):
         assert np.max(np.abs(d32[m] - d64[m]) / scale) < 1e-5, m
 
 
+def deep_h2s_case(K, raw_lines):
+    """Layers from 150 to 3000 bar with only h2s_ddb (Ben-Reuven with zeta = gamma): its lines'
+    widths reach several times their centres (168 GHz and up), where the shifted denominator
+    (x - Ds)^2 + h cancels at every column of 1-50 GHz (h < 0, Ds^2 + h << Ds^2)."""
+    from radiobear_amd import _lib
+    cat = K.jupiter_catalog(raw_lines)
+    ab = K.absorbers_struct({'h2s': 'h2s_ddb'}, 'e')
+    nl = 16
+    P = np.geomspace(150.0, 3000.0, nl)
+    T = np.linspace(450.0, 1500.0, nl)
+    mix = np.zeros((len(_lib.SPECIES), nl))
+    mix[_lib.SPECIES.index('H2')] = 0.86
+    mix[_lib.SPECIES.index('HE')] = 0.135
+    mix[_lib.SPECIES.index('H2S')] = 3.0e-5
+    f = np.linspace(1.0, 50.0, 70)
+    return cat, ab, P, T, mix, f
+
+
+def test_fp32_tangent_denominator_guard(K, raw_lines):
+    """FP32 tangent mode forms (x - Ds)^2 + h in FP32 only where its error stays within 32
+    roundings; these lines cancel far beyond that (a million-fold amplification at the deepest
+    layer), so the pass takes their denominators in FP64, and its value and tangents stay within
+    1e-5 of FP64 mode (in FP32 the denominators would be wrong in their leading digits)."""
+    cat, ab, P, T, mix, f = deep_h2s_case(K, raw_lines)
+    a64, d64 = K.alpha_tangent(cat, ab, P, T, mix, f, precision='fp64', with_alpha=True)
+    a32, d32 = K.alpha_tangent(cat, ab, P, T, mix, f, precision='fp32', with_alpha=True)
+    a64, a32 = a64.cpu().numpy(), a32.cpu().numpy()
+    assert np.all(a64 > 0.0)
+    assert np.max(np.abs(a32 - a64) / a64) < 1e-5
+    d64, d32 = d64.cpu().numpy(), d32.cpu().numpy()
+    scale = np.max(np.abs(d64[0]), axis=1, keepdims=True)
+    assert np.all(scale > 0.0)
+    assert np.max(np.abs(d32[0] - d64[0]) / scale) < 1e-5
+
+
 def test_bad_atmosphere_index_is_flagged_not_read(K, small_batch):
     """An out-of-range atmosphere index in the device ray map gives nsteps -2 and Tb NaN
     (the kernels check it before any load); the host wrapper refuses it up front."""
