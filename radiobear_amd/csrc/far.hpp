@@ -118,6 +118,19 @@ __device__ __forceinline__ T uniform(const T &v) {
     return out;
 }
 
+// b when second, else a, chosen word by word: choosing between the two objects' addresses
+// instead makes the compiler keep both in scratch and read the chosen one from there
+template <class T>
+__device__ __forceinline__ T pick(const T &a, const T &b, bool second) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    T out = a;
+    const int *wa = reinterpret_cast<const int *>(&a), *wb = reinterpret_cast<const int *>(&b);
+    int *wo = reinterpret_cast<int *>(&out);
+#pragma unroll
+    for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) wo[w] = second ? wb[w] : wa[w];
+    return out;
+}
+
 // The segments' coefficient derivers per layer (the layer's powers of T and partial pressures,
 // computed once by alpha_prep_kernel): [nl][kSegs][kPrepWords] doubles, set 0's deriver in
 // words 0-7, set 1's (the same one outside the nh3_hs inversion lines) in 8-15, and their FP32
@@ -412,6 +425,8 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) 
                 // successive lines are 64 apart, so their entries change seldom)
                 const int nblk = b0 - a0;   // blocks per wavefront (kThreads / 64 per round)
                 const int blk0 = (a0 - seg_off) * kFarWaves + wave * nblk;
+                // the set-v deriver, chosen once for the part's lines
+                const auto pv = pick(p0, p1, v != 0);
                 // each line's catalogue columns are fetched one block ahead
                 LineRow<kNumCols[kSegKind[SEG]]> next;
                 next.load(B, min(blk0 * 64 + lane, B.n - 1));
@@ -422,7 +437,7 @@ __global__ __launch_bounds__(kThreads, 3) void alpha_far_kernel(AlphaArgs args) 
                     const auto row = next;
                     if (blk + 1 < blk0 + nblk) next.load(B, min(i + 64, B.n - 1));
                     CoefT<double> c = {0.0, 0.0, 0.0, 0.0, 1.0};
-                    if (valid) c = seg_coef(p0, p1, v, row, i, base);
+                    if (valid) c = seg_coef(p0, pv, 1, row, i, base);
                     double Ds, h;
                     shifted_denominator(c, Ds, h);
                     const double P0 = fma(Ds, Ds, h);
