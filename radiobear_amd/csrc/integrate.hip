@@ -274,8 +274,9 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_factors_kernel(IntA
 // T_p W_p, so a step adds W_p times two staged factors (two FMAs, where the pairwise form took
 // two adds, two FMAs and a multiply).  Each wavefront first copies its ray's factors to LDS as
 // FP32 pairs (h_{p-1} + h_p, T_p (h_{p-1} + h_p)), so a step reads them with one broadcast LDS
-// load; a wavefront stopped by the cut at point p takes W_p h_p and T_p W_p h_p back out (its
-// sums end at p, as the pairwise form's do).
+// load; with FAC they come instead from the per-ray table integrate_factors_kernel wrote, read
+// with scalar loads (no LDS).  A wavefront stopped by the cut at point p takes W_p h_p and
+// T_p W_p h_p back out (its sums end at p, as the pairwise form's do).
 // BUF: the table rows are read as raw buffer loads whose row offset is a scalar register
 // (the row's byte offset computed on the scalar unit), so a step's load costs no vector
 // address arithmetic; for tables under 2 GiB.
