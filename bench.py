@@ -115,7 +115,7 @@ class Problem:
         self.cols = torch.empty((self.nf,), dtype=torch.int32, device=dev)
         self.alpha32_cols = torch.empty_like(self.alpha32)
         self.alpha64_cols = torch.empty_like(self.alpha64)
-        self.factors = torch.empty((self.nray, self.ld, 2), dtype=torch.float32, device=dev)   # FP32 point rays
+        self.factors = torch.empty((self.nray, self.ld, 4), dtype=torch.float32, device=dev)   # FP32 point rays
 
     # the three launches of one step, on torch's current stream
     def launch_alpha(self, prec, launch=None):

@@ -323,21 +323,23 @@ int rbx_integrate(const void *alpha, int alpha_precision, const double *T, int n
  * are rbx_integrate's up to what the cut itself changes (e^-tau_cut of a ray's weight).
  * factors: NULL, or rbx_integrate_factors' table for the same T, hds, ld, nsteps and nray:
  * point rays (mode 0) in RBX_FP32 precision with an RBX_FP32 table then read each ray's
- * trapezoid factors from it, where without it every frequency group recomputes them into LDS;
- * the outputs are bit-identical either way, and other modes and precisions ignore it.
- * rbx_integrate_factors writes that table, factors [nray][ld] float pairs (8 nray ld bytes):
- * for path point p = k + 1 of a ray of n = nsteps - 1 steps, (h_k + h_{k+1},
- * T_{k+1} (h_k + h_{k+1})) rounded to FP32 (h_n = 0; brightness.py:99-101 regrouped by path
- * point), and slot n flags a path holding a NaN.  It reads no alpha, so it can run beside the
- * absorption on another stream (no counterpart in the reference). */
+ * path factors from it, where without it every frequency group recomputes them (the
+ * trapezoid factors into LDS); the outputs are bit-identical either way, and other modes and
+ * precisions ignore it.  rbx_integrate_factors writes that table, factors [nray][ld] records
+ * of 16 bytes (16 nray ld bytes, 16-byte aligned): for path step k of a ray of n = nsteps - 1
+ * steps, the FP64 half step times log2(e) (the FP32 integration scans tau log2 e) and two
+ * FP32 trapezoid factors of point k + 1, h_k + h_{k+1} and T_{k+1} (h_k + h_{k+1}) (h_n = 0;
+ * brightness.py:99-101 regrouped by path point); record n flags a path holding a NaN.  It
+ * reads no alpha, so it can run beside the absorption on another stream (no counterpart in
+ * the reference). */
 int rbx_integrate_order(const void *alpha, int alpha_precision, int nl, int nf, const double *hds, int ld,
                         const int *nsteps, int nray, double tau_cut, int *cols, void *alpha_cols, void *stream);
 int rbx_integrate_cols(const void *alpha_cols, int alpha_precision, const int *cols, const double *T, int nl,
                        int nf, const double *hds, int ld, const int *nsteps, int nray, int mode, int precision,
-                       double tau_cut, double *Tb, double *intW, const float *factors, const rbx_launch *launch,
+                       double tau_cut, double *Tb, double *intW, const void *factors, const rbx_launch *launch,
                        void *stream);
 int rbx_integrate_factors(const double *T, int nl, const double *hds, int ld, const int *nsteps, int nray,
-                          float *factors, void *stream);
+                          void *factors, void *stream);
 
 /* ---- per-point profiles (Brightness.single's .tau, .W, .Tb_lyr, brightness.py:53-120) ----
  * tau, W, Tb_lyr  [nray][nl][nf] float64 at path points 0..nsteps-1 (0 beyond);

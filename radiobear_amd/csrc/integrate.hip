@@ -22,6 +22,14 @@ namespace rbx {
 constexpr int kWaves = 4;
 constexpr int kLanes = 64;
 
+// One path point's entry in the per-ray factor table: the half step h_k times log2(e) (the FP32
+// integration's tau scan carries tau log2(e) in FP64) and point k + 1's trapezoid factors
+// (h_k + h_{k+1}, T_{k+1} (h_k + h_{k+1})) in FP32.
+struct alignas(16) PathFactor {
+    double hl;
+    float hh, thh;
+};
+
 struct IntArgs {
     const void *alpha;
     int f32;
@@ -44,10 +52,11 @@ struct IntArgs {
     // column j of the table is request frequency cols[j]: Tb and intW go to column cols[j] of
     // their rows (null: column j; rbx_integrate_cols)
     const int *cols;
-    // FP32 point rays: each ray's trapezoid factors [nray][ld] float2, written by
-    // integrate_factors_kernel (null: each wavefront stages them in LDS)
-    const float2 *factors;
+    // FP32 point rays: each ray's path factors [nray][ld], written by integrate_factors_kernel
+    // (null: each wavefront stages the trapezoid factors in LDS)
+    const PathFactor *factors;
 };
+
 
 constexpr int kXcds = 8;   // MI355X: blocks are dealt round-robin to 8 XCDs (b % 8)
 
@@ -237,11 +246,11 @@ __device__ __forceinline__ float2 trapezoid_factor(const double *h, const double
 }
 
 // Every frequency group of a ray stages the same factors; this writes them once per ray,
-// fac[ray][k] for k < n, and at k = n (a slot every path leaves free) a NaN when the path
+// fac[ray][k] for k < n, and at k = n (a slot every path leaves free) hh = NaN when the path
 // holds a NaN half step or temperature (the integration then takes no cut) -- one wavefront
 // per ray.  They depend on the paths and T only, not on alpha, so a caller can write them
 // beside the absorption.
-__global__ __launch_bounds__(kWaves * kLanes) void integrate_factors_kernel(IntArgs A, float2 *fac) {
+__global__ __launch_bounds__(kWaves * kLanes) void integrate_factors_kernel(IntArgs A, PathFactor *fac) {
     const int wave = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
     const int ray = ray_of(A, static_cast<int>(blockIdx.x) * kWaves + wave);
     if (ray < 0) return;
@@ -251,25 +260,28 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_factors_kernel(IntA
     if (S <= 1) return;
     const int n = S - 1;
     const double *h = A.hds + static_cast<size_t>(ray) * A.ld, *Tl = A.T + base;
-    float2 *out = fac + static_cast<size_t>(ray) * A.ld;
+    PathFactor *out = fac + static_cast<size_t>(ray) * A.ld;
     bool bad = false;
     for (int k = lane; k < n; k += kLanes) {
-        out[k] = trapezoid_factor(h, Tl, k, n);
+        const float2 f = trapezoid_factor(h, Tl, k, n);
+        out[k] = PathFactor{h[k] * kLog2e, f.x, f.y};
         bad = bad || __builtin_isnan(h[k]) || __builtin_isnan(Tl[k + 1]);
     }
     bad = __any(bad);
-    if (lane == 0) out[n] = make_float2(bad ? __builtin_nanf("") : 0.0f, 0.0f);
+    if (lane == 0) out[n] = PathFactor{0.0, bad ? __builtin_nanf("") : 0.0f, 0.0f};
 }
 
 // ---- FP32 mode, point rays: FP32 weights and trapezoid sums, FP64 path and tau ----
-// The half steps and the tau prefix sum stay FP64: tau_{i+1} = tau_i + (a0 + a1) h_i in an
-// FP64 FMA at every step, with a0 + a1 (two FP32 table entries) added in FP32 and h_i the
-// FP64 half step, read as a wave-uniform scalar load (U steps per s_load batch).  W, T W
+// The half steps and the tau prefix sum stay FP64.  The scan carries t = tau log2(e):
+// t_{i+1} = t_i + (a0 + a1) (h_i log2 e) in an FP64 FMA at every step, with a0 + a1 (two FP32
+// table entries) added in FP32 and h_i log2 e the FP64 half step times log2(e) (a wave-uniform
+// scalar load, U steps per s_load batch, from the factor table or formed from the ray's half
+// step; the same FP64 product either way), and the cut is taken at t > tau_cut log2(e).  W, T W
 // and the two trapezoid sums are FP32, added into FP64 totals after every batch of U steps
-// (blocked summation).  e^-tau is the hardware exp2 of t = -tau log2(e) with tau rounded to
-// FP32 first: the rounding moves the exponent by at most |t| 2^-24, so a weight moves by at
-// most tau e^-tau 2^-24 ~ 2e-8 of its alpha (largest at tau = 1), below the FP32 sums' own
-// error; past t = -150 the result is 0.  The trapezoid sums are taken by path point:
+// (blocked summation).  e^-tau is the hardware exp2 of -t with t rounded to FP32: the rounding
+// moves the exponent by at most t 2^-24, so a weight moves by at most tau e^-tau 2^-24 ~ 2e-8
+// of its alpha (largest at tau = 1), below the FP32 sums' own error; past t = 150 the result
+// is 0.  The trapezoid sums are taken by path point:
 // sum_i (W_i + W_{i+1}) h_i = sum_p W_p (h_{p-1} + h_p) (W_0 = 0, h_n = 0), and the same with
 // T_p W_p, so a step adds W_p times two staged factors (two FMAs, where the pairwise form took
 // two adds, two FMAs and a multiply).  Each wavefront first copies its ray's factors to LDS as
@@ -280,7 +292,6 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_factors_kernel(IntA
 // BUF: the table rows are read as raw buffer loads whose row offset is a scalar register
 // (the row's byte offset computed on the scalar unit), so a step's load costs no vector
 // address arithmetic; for tables under 2 GiB.
-constexpr float kLog2ef = 1.44269504f;
 constexpr int kBufWord3 = 0x00020000;   // buffer descriptor word 3: 32-bit data format, no swizzle
 
 template <int U, bool BUF, bool FAC>
@@ -304,7 +315,19 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
     const double *Tl = A.T + base;
     // the ray's factors: staged in this wavefront's LDS, or (FAC) read from the factor table
     float2 *stage = hT_all + static_cast<size_t>(wave) * A.ld;
-    const float2 *hT = FAC ? A.factors + static_cast<size_t>(ray) * A.ld : stage;
+    [[maybe_unused]] const PathFactor *fac = A.factors + static_cast<size_t>(ray) * A.ld;
+    auto factor = [&](int k) -> float2 {   // point k + 1's trapezoid factors
+        if constexpr (FAC)
+            return make_float2(fac[k].hh, fac[k].thh);
+        else
+            return stage[k];
+    };
+    auto hlog = [&](int k) -> double {   // h_k log2(e)
+        if constexpr (FAC)
+            return fac[k].hl;
+        else
+            return h[k] * kLog2e;
+    };
     [[maybe_unused]] __amdgpu_buffer_rsrc_t rows;
     [[maybe_unused]] const int nfb = static_cast<int>(nf) * 4, lane_b = jj * 4;
     if constexpr (BUF)
@@ -317,12 +340,12 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
         else
             return al[static_cast<size_t>(r) * nf];
     };
-    double tau = 0.0, intW = 0.0, num = 0.0;
+    double t = 0.0, intW = 0.0, num = 0.0;   // t = tau log2(e)
     if (S > 1) {
         const int n = S - 1;
         bool bad = false;   // a NaN half step or temperature: no cut (path_has_nan)
         if constexpr (FAC) {
-            bad = __builtin_isnan(hT[n].x);   // the table marks such a path past its last factor
+            bad = __builtin_isnan(fac[n].hh);   // the table marks such a path past its last factor
         } else {
             for (int k = lane; k < n; k += kLanes) {
                 stage[k] = trapezoid_factor(h, Tl, k, n);
@@ -331,7 +354,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
             bad = __any(bad);
             wave_sync();
         }
-        const double cut = (A.tau_cut > 0.0 && !bad) ? A.tau_cut : 0.0;
+        const double cut = (A.tau_cut > 0.0 && !bad) ? A.tau_cut * kLog2e : 0.0;   // on t
         const unsigned long long idle = __builtin_amdgcn_ballot_w64(!ok);   // lanes past the last column
         float a0 = row(0), W1 = 0.0f;
         int i = 0;
@@ -341,21 +364,21 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 ab[u] = row(i + u + 1);
-                hb[u] = h[i + u];
+                hb[u] = hlog(i + u);
             }
             float iW = 0.0f, nm = 0.0f;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const float2 ht = hT[i + u];
-                tau = fma(static_cast<double>(a0 + ab[u]), hb[u], tau);
-                W1 = ab[u] * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
+                const float2 ht = factor(i + u);
+                t = fma(static_cast<double>(a0 + ab[u]), hb[u], t);
+                W1 = ab[u] * __builtin_amdgcn_exp2f(-static_cast<float>(t));
                 iW = fmaf(W1, ht.x, iW);
                 nm = fmaf(W1, ht.y, nm);
                 a0 = ab[u];
             }
             intW += static_cast<double>(iW);
             num += static_cast<double>(nm);
-            if (cut > 0.0 && (__builtin_amdgcn_ballot_w64(tau > cut) | idle) == __builtin_amdgcn_read_exec()) {
+            if (cut > 0.0 && (__builtin_amdgcn_ballot_w64(t > cut) | idle) == __builtin_amdgcn_read_exec()) {
                 const int p = i + U;   // the sums end at point p
                 if (p < n) {
                     const double hp = h[p] * static_cast<double>(W1);
@@ -368,9 +391,9 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
         }
         for (; i < n; ++i) {
             const float a1 = row(i + 1);
-            const float2 ht = hT[i];
-            tau = fma(static_cast<double>(a0 + a1), h[i], tau);
-            W1 = a1 * __builtin_amdgcn_exp2f(static_cast<float>(tau) * -kLog2ef);
+            const float2 ht = factor(i);
+            t = fma(static_cast<double>(a0 + a1), hlog(i), t);
+            W1 = a1 * __builtin_amdgcn_exp2f(-static_cast<float>(t));
             intW += static_cast<double>(W1 * ht.x);
             num += static_cast<double>(W1 * ht.y);
             a0 = a1;
@@ -1118,7 +1141,7 @@ extern "C" int rbx_integrate_order(const void *alpha, int alpha_precision, int n
 
 extern "C" int rbx_integrate_cols(const void *alpha_cols, int alpha_precision, const int *cols, const double *T,
                                   int nl, int nf, const double *hds, int ld, const int *nsteps, int nray, int mode,
-                                  int precision, double tau_cut, double *Tb, double *intW, const float *factors,
+                                  int precision, double tau_cut, double *Tb, double *intW, const void *factors,
                                   const rbx_launch *launch, void *stream) {
     int rc = rbx::check_int_args(alpha_cols, alpha_precision, T, nl, nf, hds, ld, nsteps, nray, precision);
     if (rc) return rc;
@@ -1136,14 +1159,16 @@ extern "C" int rbx_integrate_cols(const void *alpha_cols, int alpha_precision, c
     A.Tb = Tb;
     A.intW = intW;
     A.cols = cols;
-    // FP32 point rays read each ray's trapezoid factors from rbx_integrate_factors' table
-    if (factors && mode == 0 && A.f32 && A.fast_exp) A.factors = reinterpret_cast<const float2 *>(factors);
+    // FP32 point rays read each ray's path factors from rbx_integrate_factors' table
+    RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(factors) % 16 == 0, "rbx_integrate_cols: factors not 16-byte aligned");
+    if (factors && mode == 0 && A.f32 && A.fast_exp) A.factors = static_cast<const rbx::PathFactor *>(factors);
     return launch_integrate(A, L, stream);
 }
 
 extern "C" int rbx_integrate_factors(const double *T, int nl, const double *hds, int ld, const int *nsteps,
-                                     int nray, float *factors, void *stream) {
+                                     int nray, void *factors, void *stream) {
     RBX_CHECK_ARG(T && hds && nsteps && factors, "rbx_integrate_factors: null device pointer");
+    RBX_CHECK_ARG(reinterpret_cast<uintptr_t>(factors) % 16 == 0, "rbx_integrate_factors: factors not 16-byte aligned");
     RBX_CHECK_ARG(nl >= 1 && nray >= 0, "rbx_integrate_factors: nl=%d nray=%d", nl, nray);
     RBX_CHECK_ARG(ld >= nl - 1 && ld >= 1, "rbx_integrate_factors: ld=%d < nl-1=%d", ld, nl - 1);
     if (nray == 0) return RBX_OK;
@@ -1151,7 +1176,7 @@ extern "C" int rbx_integrate_factors(const double *T, int nl, const double *hds,
     rbx::fill_int_args(A, nullptr, RBX_FP32, T, nl, 1, hds, ld, nsteps, nray, RBX_FP32, 0.0);
     const unsigned blocks = static_cast<unsigned>((nray + rbx::kWaves - 1) / rbx::kWaves);
     hipLaunchKernelGGL(rbx::integrate_factors_kernel, dim3(blocks), dim3(rbx::kWaves * rbx::kLanes), 0,
-                       static_cast<hipStream_t>(stream), A, reinterpret_cast<float2 *>(factors));
+                       static_cast<hipStream_t>(stream), A, static_cast<rbx::PathFactor *>(factors));
     return rbx::launch_status("integrate_factors_kernel");
 }
 

@@ -377,7 +377,7 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
     frequencies in the order in which a reference ray passes tau_cut (rbx_integrate_order +
     rbx_integrate_cols; with a cut only): the wavefronts then stop together, and Tb differs
     from the request order's only by what the cut itself changes.  FP32 point rays in that
-    order read each ray's trapezoid factors from a table written once per ray (8 nray ld
+    order read each ray's path factors from a table written once per ray (16 nray ld
     bytes) rather than staging them in LDS per frequency group."""
     aprec = _lib.FP32 if alpha_table.dtype == torch.float32 else _lib.FP64
     prec = aprec if precision is None else _lib.PRECISIONS[precision]
@@ -393,7 +393,7 @@ def integrate(alpha_table, T, hds, nsteps, disc=False, precision=None, tau_cut=0
         a_cols = torch.empty_like(a)
         fac = None
         if not disc and aprec == _lib.FP32 and prec == _lib.FP32:
-            fac = torch.empty((nray, ld, 2), dtype=torch.float32, device=device())
+            fac = torch.empty((nray, ld, 4), dtype=torch.float32, device=device())   # 16-byte records
         check(lib().rbx_integrate_order(_ptr(a), aprec, nl, nf, _ptr(hds), ld, _ptr(nsteps), nray, float(tau_cut),
                                         _ptr(cols), _ptr(a_cols), _stream()))
         if fac is not None:

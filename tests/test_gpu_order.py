@@ -87,7 +87,7 @@ def test_identity_order_is_bit_identical(K, precision, dtype_name, disc):
     prec = _lib.PRECISIONS[precision]
     Tgpu = K.as_device(T)
     ident = torch.arange(nf, dtype=torch.int32, device='cuda')
-    fac = torch.full((nray, ld, 2), np.nan, dtype=torch.float32, device='cuda')
+    fac = torch.full((nray, ld, 4), np.nan, dtype=torch.float32, device='cuda')
     _lib.check(_lib.lib().rbx_integrate_factors(K._ptr(Tgpu), nl, K._ptr(hds), ld, K._ptr(ns), nray, K._ptr(fac),
                                                 K._stream()))
     for cut in (0.0, 25.0):
@@ -113,7 +113,7 @@ def test_factor_table_nan_path(K):
     nl, nf = tab.shape
     nray, ld = hds.shape
     ident = torch.arange(nf, dtype=torch.int32, device='cuda')
-    fac = torch.full((nray, ld, 2), 7.0, dtype=torch.float32, device='cuda')
+    fac = torch.full((nray, ld, 4), 7.0, dtype=torch.float32, device='cuda')
     Tgpu = K.as_device(T)
     _lib.check(_lib.lib().rbx_integrate_factors(K._ptr(Tgpu), nl, K._ptr(hds), ld, K._ptr(ns), nray, K._ptr(fac),
                                                 K._stream()))
@@ -177,9 +177,9 @@ def test_order_without_on_planet_rays(K):
 
 
 def test_factor_table_values(K):
-    """rbx_integrate_factors: (h_k + h_{k+1}, T_{k+1} (h_k + h_{k+1})) in FP32 for k < n = nsteps - 1
-    (h_n = 0), a NaN flag at slot n only for a path holding a NaN, rows past n untouched, nothing
-    written for rays off the planet."""
+    """rbx_integrate_factors: 16-byte records of (h_k log2 e in FP64, then in FP32 h_k + h_{k+1} and
+    T_{k+1} (h_k + h_{k+1})) for k < n = nsteps - 1 (h_n = 0); record n holds (0, NaN if the path
+    holds a NaN else 0, 0); records past n untouched; nothing written for rays off the planet."""
     import torch
     from radiobear_amd import _lib
     rng = np.random.default_rng(4)
@@ -189,21 +189,42 @@ def test_factor_table_values(K):
     hds = rng.uniform(1.0e5, 9.0e5, (nray, ld))
     ns = np.array([nl - 1, 40, -1, 2, nl - 1], dtype=np.int32)
     hds[4, 17] = np.nan
-    fac = torch.full((nray, ld, 2), 5.0, dtype=torch.float32, device='cuda')
+    fac = torch.full((nray, ld, 4), 5.0, dtype=torch.float32, device='cuda')
     Tg, hg, ng = K.as_device(T), torch.as_tensor(hds, device='cuda'), torch.as_tensor(ns, device='cuda')
     _lib.check(_lib.lib().rbx_integrate_factors(K._ptr(Tg), nl, K._ptr(hg), ld, K._ptr(ng), nray, K._ptr(fac),
                                                 K._stream()))
-    got = fac.cpu().numpy()
+    raw = fac.cpu().numpy()
+    rec = raw.view(np.dtype([('hl', '<f8'), ('hh', '<f4'), ('thh', '<f4')]))[..., 0]
+    log2e = np.float64(1.4426950408889634)
+    assert log2e == np.log2(np.e)
     for r in range(nray):
         S = min(int(ns[r]), nl - 1)
         if S <= 1:
-            assert np.all(got[r] == 5.0)
+            assert np.all(raw[r] == 5.0)
             continue
         n = S - 1
         h = hds[r, :n]
         hh = h + np.append(hds[r, 1:n], 0.0)
-        want = np.stack([hh.astype(np.float32), (T[1:n + 1] * hh).astype(np.float32)], axis=1)
-        assert np.array_equal(got[r, :n], want, equal_nan=True)
-        assert np.isnan(got[r, n, 0]) == (r == 4) and got[r, n, 1] == 0.0
-        assert np.all(got[r, n + 1:] == 5.0)
-    assert np.isnan(got[4, 16:18]).any()
+        assert np.array_equal(rec['hl'][r, :n], h * log2e, equal_nan=True)
+        assert np.array_equal(rec['hh'][r, :n], hh.astype(np.float32), equal_nan=True)
+        assert np.array_equal(rec['thh'][r, :n], (T[1:n + 1] * hh).astype(np.float32), equal_nan=True)
+        assert rec['hl'][r, n] == 0.0 and rec['thh'][r, n] == 0.0
+        assert np.isnan(rec['hh'][r, n]) == (r == 4)
+        assert np.all(raw[r, n + 1:] == 5.0)
+    assert np.isnan(rec['hh'][4, 16:18]).all()
+
+
+def test_factor_table_rejects_misaligned(K):
+    import torch
+    from radiobear_amd import _lib
+    tab, T, hds, ns, *_ = case(20, 16, 3, 1, torch.float32)
+    fac = torch.zeros((3 * hds.shape[1] * 4 + 1,), dtype=torch.float32, device='cuda')
+    bad = ctypes_ptr_offset(fac, 4)
+    L = _lib.lib()
+    assert L.rbx_integrate_factors(K._ptr(K.as_device(T)), 20, K._ptr(hds), hds.shape[1], K._ptr(ns), 3, bad,
+                                   K._stream()) == -1
+
+
+def ctypes_ptr_offset(t, nbytes):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr() + nbytes)
