@@ -74,6 +74,26 @@ def test_order_is_stable_sort_by_cut_step(K, dtype_name, nf):
     assert nf < 100 or len(np.unique(key)) > 10   # the case spreads the cut steps
 
 
+@pytest.mark.parametrize('dtype_name', ['float32', 'float64'])
+def test_order_long_path(K, dtype_name):
+    """A reference path longer than the half steps the order kernel stages in LDS (8192): the
+    same stable sort by cut step, with columns passing the cut anywhere from a few thousand
+    steps to never."""
+    import torch
+    rng = np.random.default_rng(17)
+    nl, nf, nray = 9000, 70, 3
+    scale = 10.0 ** rng.uniform(-8.6, -7.9, nf)
+    alpha = scale[None, :] * np.exp(rng.uniform(-0.3, 0.3, (nl, nf)))
+    hds = rng.uniform(2.0e5, 6.0e5, (nray, nl))
+    ns = np.array([nl - 200, nl, 50], dtype=np.int32)
+    tab = torch.as_tensor(alpha, device='cuda').to(getattr(torch, dtype_name))
+    hg, ng = torch.as_tensor(hds, device='cuda'), torch.as_tensor(ns, device='cuda')
+    cols, a_cols = order(K, tab, hg, ng, 30.0)
+    key = cut_steps(tab.double().cpu().numpy(), hds, ns, 30.0)
+    assert np.array_equal(cols, np.argsort(key, kind='stable'))
+    assert key.min() > 2000 and (key == nl - 1).sum() > 5 and len(np.unique(key)) > 20
+
+
 @pytest.mark.parametrize('precision,dtype_name,disc', [('fp64', 'float64', False), ('fp32', 'float32', False),
                                                        ('fp64', 'float32', True), ('fp32', 'float32', True)])
 def test_identity_order_is_bit_identical(K, precision, dtype_name, disc):
