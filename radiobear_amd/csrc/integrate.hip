@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kWaves * kLanes) void integrate_f32_kernel(IntArgs 
     const double *Tl = A.T + base;
     // the ray's factors: staged in this wavefront's LDS, or (FAC) read from the factor table
     float2 *stage = hT_all + static_cast<size_t>(wave) * A.ld;
-    [[maybe_unused]] const PathFactor *fac = A.factors + static_cast<size_t>(ray) * A.ld;
+    [[maybe_unused]] const PathFactor *fac = FAC ? A.factors + static_cast<size_t>(ray) * A.ld : nullptr;
     auto factor = [&](int k) -> float2 {   // point k + 1's trapezoid factors
         if constexpr (FAC)
             return make_float2(fac[k].hh, fac[k].thh);
